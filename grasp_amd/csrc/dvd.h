@@ -1,0 +1,135 @@
+// dvd.h -- device-resident block Davidson (GDVD/DVDSON) over many problems.
+#pragma once
+#include "gci_internal.h"
+
+namespace gci {
+
+constexpr int MAXLIM = 80;    // largest LIM (2*nvec+60 for nvec <= 10)
+constexpr int MAXNUME = 16;   // largest number of wanted eigenpairs
+constexpr int CHUNK = 512;    // rows per tall-skinny work item
+
+// Per-problem stage.  Kernels act only on problems in the stage they serve.
+enum : int {
+    PH_EMPTY = -1,     // free slot
+    PH_ITER = 0,       // inside DVDRVR's loop
+    PH_FINAL = 1,      // converged / stopped: exit rotation + residuals pending
+    PH_DONE = 2,       // results ready
+    PH_INIT = 3,       // INITDVD step in progress
+    PH_INIT_IDLE = 4,  // INITDVD, nothing to do in the current sub-step
+};
+
+// One Davidson problem (a CI block).  Lives in device memory; the scalar
+// state is advanced by device kernels so an iteration needs no host sync.
+struct DvdProb {
+    // --- static description
+    int64_t n;
+    int lim, nume, mblock, maxiter;
+    double crite, critc, critr;
+    int mode;                       // SpMM operator mode (GCI_OP_*) of the bulk phase
+    const int64_t *rp;              // off-diagonal CSR of H
+    const int32_t *col;
+    const double *val;
+    const float *hi, *lo;
+    const double *diagH;            // FP64 diagonal of H (shifted)
+    double *basis, *ab;             // n x lim each (column-major, ld = n)
+    double *diag;                   // preconditioner diagonal (drifts as in GDVD)
+    double *xc, *rc;                // n x nume candidate Ritz vectors / residuals
+    double tol_rel;                 // >0: residual test ||r|| < tol_rel*|lambda+eshift|
+    double eshift;                  // EAV added back for the relative test
+    int64_t chunk0;                 // first entry in the flattened chunk list
+    int nchunks;
+    int slot;
+    // --- dynamic state
+    int kpass, nncv, nnew, rest, nloops, ierr, nmv;
+    int phase, ovflow, ncand, conv_all, niv;
+    int refine_phase;               // 0 bulk operator, 1 FP64 (hi+lo) operator
+    int refine_loops;
+    int bulk_loops, bulk_nmv;       // iterations/products of the bulk phase
+    // --- small arrays (1-based reference arrays stored 0-based)
+    double eigval[MAXLIM];
+    double oldval[MAXLIM + 1];
+    double svec[MAXLIM * MAXLIM];   // kpass x ifind, ld = kpass
+    double s[MAXLIM * (MAXLIM + 1) / 2];
+    double resn[MAXLIM];            // SCRA1(1:NUME) residual norms
+    double epsil[MAXLIM];           // SCRA1(LIM+1:) eigenvalue shifts (zero)
+    int icv[MAXNUME + 2];
+    int incv[MAXLIM + 1];
+    int slot_cand[MAXNUME];
+    double rhseps[MAXNUME];
+    double lam_add[MAXNUME];
+    double coef[MAXLIM * MAXNUME];
+    double rinv[MAXNUME * MAXNUME];
+};
+
+struct DvdSpec {
+    const HMat *h = nullptr;
+    int nume = 1, lim = 0, mblock = 1, maxiter = 0;
+    double crite = 1e-17, critc = 1e-9, critr = 1e-9;
+    int mode = GCI_OP_FP64;
+    double tol_rel = 0.0;   // bulk-phase relative residual target (0: CRITR)
+    double tol_final = 0.0; // refinement target (split forms)
+    double critr_final = 0.0;
+    double eshift = 0.0;
+    bool refine = false;    // run an FP64 refinement after the bulk phase
+};
+
+// Davidson over a fixed-capacity set of problem slots sharing one stream.
+// Problems can be admitted and released between steps (continuous
+// batching); an iteration is one fixed launch sequence over all slots,
+// replayed from a captured HIP graph while the slot set is unchanged.
+class DavidsonEngine {
+  public:
+    DavidsonEngine(int capacity, hipStream_t s);
+    ~DavidsonEngine();
+    DavidsonEngine(const DavidsonEngine &) = delete;
+    DavidsonEngine &operator=(const DavidsonEngine &) = delete;
+
+    // Admits problems; init[i] = host n x niv[i] vectors, or null for unit
+    // vectors on the niv smallest diagonal entries.  Runs INITDVD.
+    std::vector<int> admit(const std::vector<DvdSpec> &specs, const std::vector<const double *> &init,
+                           const std::vector<int> &niv);
+    void step(int iters);                 // enqueue iterations (no sync)
+    std::vector<int> poll();              // sync; slots newly in PH_DONE
+    // Restart finished bulk-phase slots with the FP64 operator from their
+    // Ritz vectors (the refinement pass).
+    void refine_restart(const std::vector<int> &slots);
+    bool needs_refine(int slot) const;
+    void extract(int slot, double *eval, double *evec, double *resid, gci_dvd_stats *st);
+    void release(int slot);
+    int active() const;
+    int capacity() const { return cap_; }
+    int phase(int slot) const { return phase_[slot]; }
+    static size_t workspace_bytes(const DvdSpec &s);
+    bool use_graph = true;
+
+  private:
+    void rebuild_chunks();
+    void launch_iteration();
+    void launch_finalize();
+    void launch_orth(int passes, int phase);
+    void launch_op(int phase);
+    void run_initdvd(const std::vector<int> &slots);
+
+    int cap_;
+    hipStream_t stream_;
+    std::vector<DvdSpec> spec_;
+    std::vector<int> phase_;               // last polled phases
+    std::vector<char> refined_;
+    std::vector<DBuf<double>> work_;
+    DBuf<DvdProb> dprob_;
+    DBuf<int> chunk_prob_;
+    DBuf<int64_t> chunk_row_;
+    DBuf<double> partial_;
+    DBuf<int> dphase_;
+    int64_t total_chunks_ = 0;
+    std::vector<int> hprob_;
+    std::vector<int64_t> hrow_;
+    hipGraphExec_t gexec_ = nullptr;
+    bool graph_valid_ = false;
+};
+
+// Dense lowest eigenpairs on the device (rocSOLVER dsyevx): a is n x n
+// column-major (device), eval/evec device outputs for il..iu (1-based).
+void dense_lowest(int n, double *d_a, int nev, double *d_eval, double *d_evec, hipStream_t s);
+
+}  // namespace gci

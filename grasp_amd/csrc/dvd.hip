@@ -1,0 +1,1106 @@
+// dvd.hip -- device-resident block Davidson for CI blocks (GDVD/DVDSON).
+//
+// Restates libdvd90/dvdson.f90 (DVDRVR :559-836, INITDVD :848-948, ADDS
+// :484-555, NEWVEC :1002-1175, OVFLOW :1179-1256, TSTSEL :1260-1413,
+// MGS_NRM :1417-1508) and the GDVD preconditioner (libdvd90/gdvd.f90:93-134)
+// as a sequence of device kernels over an array of problems:
+//
+//   k_rr            Rayleigh-Ritz: cyclic parallel Jacobi on the packed
+//                   KPASS x KPASS projection S (LDS), sort, TSTSEL, OVFLOW test
+//   k_rotate        basis/AB rotation by the Ritz coefficients (OVFLOW, exit)
+//   k_newvec        Ritz vectors x = B s and residuals r = AB s - lambda x for
+//                   all candidates at once (tall-skinny GEMM) + norms
+//   k_newvec_decide NEWVEC's sequential accept/converge logic, robust shift
+//   k_precond       (r - eps x) / (D - lambda) with GDVD's 1e-6 guard and its
+//                   in-place DIAG drift, reproduced exactly
+//   k_gram/k_update two-pass Gram-Schmidt against the basis (classical GS
+//   k_chol/k_scale  twice) and Cholesky-QR among the new vectors
+//   k_spmm_batch    the operator on the new vectors (all problems in one launch)
+//   k_gram + k_adds new columns of S = B^T AB_new
+//
+// Every kernel reads the per-problem state from device memory and skips
+// problems that are not in the stage, so one iteration over many problems
+// is a fixed launch sequence with no host synchronisation.  Reductions over
+// rows are two-stage (per-chunk partials, then a fixed-order sum), so
+// results are deterministic run to run.
+#include "dvd.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+
+namespace gci {
+
+namespace {
+
+constexpr int PSTRIDE = MAXLIM * MAXNUME;  // doubles per chunk partial
+
+__device__ inline double block_sum(double v, double *sh) {
+    // 256-thread block reduction, fixed order
+    const int t = threadIdx.x;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((t & 63) == 0) sh[t >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (t == 0) {
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += sh[w];
+    }
+    __syncthreads();
+    return r;  // valid on thread 0
+}
+
+// ------------------------------------------------------------ Jacobi RR
+
+__device__ inline int rr_pos(int j, int r, int np) { return j == 0 ? 0 : ((j - 1 + r) % (np - 1)) + 1; }
+
+__global__ __launch_bounds__(256) void k_rr(DvdProb *P) {
+    DvdProb &p = P[blockIdx.x];
+    if (p.phase != PH_ITER) return;
+    __shared__ double A[MAXLIM][MAXLIM + 1];
+    __shared__ double V[MAXLIM][MAXLIM + 1];
+    __shared__ double cs[MAXLIM / 2], sn[MAXLIM / 2];
+    __shared__ int pa[MAXLIM / 2], pb[MAXLIM / 2];
+    __shared__ int any_rot;
+    __shared__ int rank_of[MAXLIM];
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int k = p.kpass;
+    const int ifind = min(k, p.rest);
+    // DCOPY(IFIND, EIGVAL, OLDVAL) (dvdson.f90:722)
+    for (int i = t; i < ifind; i += nt) p.oldval[i] = p.eigval[i];
+    for (int idx = t; idx < k * k; idx += nt) {
+        const int i = idx % k, j = idx / k;
+        const int lo = min(i, j), hi = max(i, j);
+        A[i][j] = p.s[lo + hi * (hi + 1) / 2];
+        V[i][j] = (i == j) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const int np = k + (k & 1);
+    const int npairs = np / 2;
+    for (int sweep = 0; sweep < 40; ++sweep) {
+        __syncthreads();
+        if (t == 0) any_rot = 0;
+        __syncthreads();
+        for (int r = 0; r < np - 1; ++r) {
+            if (t < npairs) {
+                int a = rr_pos(t, r, np), b = rr_pos(np - 1 - t, r, np);
+                if (a > b) { int tmp = a; a = b; b = tmp; }
+                double c = 1.0, s = 0.0;
+                if (b < k) {
+                    const double app = A[a][a], aqq = A[b][b], apq = A[a][b];
+                    if (apq != 0.0 && fabs(apq) > 2.2e-17 * sqrt(fabs(app * aqq))) {
+                        const double tau = (aqq - app) / (2.0 * apq);
+                        const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + tt * tt);
+                        s = tt * c;
+                        any_rot = 1;
+                    }
+                }
+                pa[t] = a;
+                pb[t] = b;
+                cs[t] = c;
+                sn[t] = s;
+            }
+            __syncthreads();
+            // columns of A and V
+            for (int idx = t; idx < npairs * k; idx += nt) {
+                const int q = idx / k, i = idx % k;
+                const int a = pa[q], b = pb[q];
+                if (b >= k || sn[q] == 0.0) continue;
+                const double c = cs[q], s = sn[q];
+                const double x = A[i][a], y = A[i][b];
+                A[i][a] = c * x - s * y;
+                A[i][b] = s * x + c * y;
+                const double vx = V[i][a], vy = V[i][b];
+                V[i][a] = c * vx - s * vy;
+                V[i][b] = s * vx + c * vy;
+            }
+            __syncthreads();
+            // rows of A
+            for (int idx = t; idx < npairs * k; idx += nt) {
+                const int q = idx / k, i = idx % k;
+                const int a = pa[q], b = pb[q];
+                if (b >= k || sn[q] == 0.0) continue;
+                const double c = cs[q], s = sn[q];
+                const double x = A[a][i], y = A[b][i];
+                A[a][i] = c * x - s * y;
+                A[b][i] = s * x + c * y;
+            }
+            __syncthreads();
+            if (t < npairs) {
+                const int a = pa[t], b = pb[t];
+                if (b < k && sn[t] != 0.0) {
+                    A[a][b] = 0.0;
+                    A[b][a] = 0.0;
+                }
+            }
+            __syncthreads();
+        }
+        const int rotated = any_rot;
+        __syncthreads();  // every thread has read the flag before it is reset
+        if (!rotated) break;
+    }
+    // ascending order (ties by index)
+    for (int i = t; i < k; i += nt) {
+        const double di = A[i][i];
+        int rk = 0;
+        for (int j = 0; j < k; ++j) {
+            const double dj = A[j][j];
+            rk += (dj < di) || (dj == di && j < i);
+        }
+        rank_of[i] = rk;
+    }
+    __syncthreads();
+    for (int idx = t; idx < k * k; idx += nt) {
+        const int i = idx / k, r = idx % k;  // eigen index i, component r
+        const int rk = rank_of[i];
+        if (rk < ifind) {
+            p.svec[rk * k + r] = V[r][i];
+            if (r == 0) p.eigval[rk] = A[i][i];
+        }
+    }
+    __syncthreads();
+    if (t != 0) return;
+    // TSTSEL (dvdson.f90:1328-1411); ISELEC = 1..NUME
+    const int nume = p.nume;
+    int nnce = 0;
+    for (int i = 0; i < nume; ++i)
+        if (!(fabs(p.oldval[i] - p.eigval[i]) < p.crite)) ++nnce;
+    if (nnce == 0) {
+        p.phase = PH_FINAL;
+        return;
+    }
+    int icnt = 0;
+    int ind[MAXNUME];
+    double rowlast[MAXNUME];
+    for (int i = 0; i < nume; ++i) {
+        const int icur = k * (i + 1) - 1;
+        double tmax = fabs(p.svec[icur]);
+        for (int l = 1; l <= p.nncv - 1; ++l) tmax = fmax(tmax, fabs(p.svec[icur - l]));
+        if (tmax < p.critc) {
+            p.icv[i] = 1;
+        } else {
+            p.icv[i] = 0;
+            ind[icnt] = i + 1;
+            rowlast[icnt] = tmax;
+            ++icnt;
+        }
+    }
+    p.nncv = icnt;
+    for (int i = 0; i < icnt; ++i) {  // IDAMAX selection sort
+        int best = i;
+        for (int q = i + 1; q < icnt; ++q)
+            if (fabs(rowlast[q]) > fabs(rowlast[best])) best = q;
+        p.incv[i] = ind[best];
+        double tv = rowlast[best]; rowlast[best] = rowlast[i]; rowlast[i] = tv;
+        int ti = ind[best]; ind[best] = ind[i]; ind[i] = ti;
+    }
+    if (icnt == 0 || k >= p.n) {
+        p.phase = PH_FINAL;
+        return;
+    }
+    p.ovflow = (k == p.lim);
+    p.ncand = icnt;
+}
+
+// --------------------------------------------------- basis rotation
+// mode 0: OVFLOW (phase ITER, ovflow set): keep REST columns.
+// mode 1: exit (phase FINAL): keep NUME columns.
+__global__ __launch_bounds__(256) void k_rotate(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
+                                                int mode) {
+    DvdProb &p = P[chunk_prob[blockIdx.x]];
+    if (mode == 0 && !(p.phase == PH_ITER && p.ovflow)) return;
+    if (mode == 1 && p.phase != PH_FINAL) return;
+    __shared__ double sv[MAXLIM * MAXLIM];
+    __shared__ double tile[64][MAXLIM + 1];
+    const int kin = p.kpass;
+    const int kout = (mode == 0) ? p.rest : p.nume;
+    for (int i = threadIdx.x; i < kin * kout; i += blockDim.x) sv[i] = p.svec[i];
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    __syncthreads();
+    for (int which = 0; which < 2; ++which) {
+        double *M = which == 0 ? p.basis : p.ab;
+        for (int64_t rt = r0; rt < r1; rt += 64) {
+            const int rows = (int)min<int64_t>(64, r1 - rt);
+            for (int idx = threadIdx.x; idx < rows * kin; idx += blockDim.x) {
+                const int rr = idx % rows, i = idx / rows;
+                tile[rr][i] = M[rt + rr + (int64_t)i * p.n];
+            }
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < rows * kout; idx += blockDim.x) {
+                const int rr = idx % rows, j = idx / rows;
+                double acc = 0.0;
+                for (int i = 0; i < kin; ++i) acc = fma(tile[rr][i], sv[j * kin + i], acc);
+                M[rt + rr + (int64_t)j * p.n] = acc;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// after OVFLOW: S = diag(lambda_1..REST), SVEC = I, KPASS = REST (dvdson.f90:1240-1253)
+__global__ void k_ovf_fix(DvdProb *P, int nprob) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprob) return;
+    DvdProb &p = P[i];
+    if (!(p.phase == PH_ITER && p.ovflow)) return;
+    const int rest = p.rest;
+    for (int q = 0; q < rest * (rest + 1) / 2; ++q) p.s[q] = 0.0;
+    for (int q = 0; q < rest * rest; ++q) p.svec[q] = 0.0;
+    int ind = 0;
+    for (int j = 0; j < rest; ++j) {
+        ind += j;
+        p.s[ind + j] = p.eigval[j];
+        p.svec[j * rest + j] = 1.0;
+    }
+    p.kpass = rest;
+    p.ovflow = 0;
+}
+
+// ------------------------------------------------------------- NEWVEC
+__global__ __launch_bounds__(256) void k_newvec(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
+                                                double *partial) {
+    const int pi = chunk_prob[blockIdx.x];
+    DvdProb &p = P[pi];
+    if (p.phase != PH_ITER) return;
+    __shared__ double sv[MAXNUME][MAXLIM];
+    __shared__ double lam[MAXNUME];
+    __shared__ double red[8];
+    const int k = p.kpass, nc = p.ncand;
+    for (int idx = threadIdx.x; idx < nc * k; idx += blockDim.x) {
+        const int t = idx / k, i = idx % k;
+        sv[t][i] = p.svec[(p.incv[t] - 1) * k + i];
+    }
+    if (threadIdx.x < nc) lam[threadIdx.x] = p.eigval[p.incv[threadIdx.x] - 1];
+    __syncthreads();
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    double nrm[MAXNUME];
+#pragma unroll
+    for (int t = 0; t < MAXNUME; ++t) nrm[t] = 0.0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        double xb[MAXNUME], xa[MAXNUME];
+#pragma unroll
+        for (int t = 0; t < MAXNUME; ++t) { xb[t] = 0.0; xa[t] = 0.0; }
+        for (int i = 0; i < k; ++i) {
+            const double b = p.basis[r + (int64_t)i * p.n];
+            const double a = p.ab[r + (int64_t)i * p.n];
+#pragma unroll
+            for (int t = 0; t < MAXNUME; ++t) {
+                if (t < nc) {
+                    xb[t] = fma(b, sv[t][i], xb[t]);
+                    xa[t] = fma(a, sv[t][i], xa[t]);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < MAXNUME; ++t) {
+            if (t < nc) {
+                const double rr = fma(-lam[t], xb[t], xa[t]);
+                p.xc[r + (int64_t)t * p.n] = xb[t];
+                p.rc[r + (int64_t)t * p.n] = rr;
+                nrm[t] = fma(rr, rr, nrm[t]);
+            }
+        }
+    }
+    for (int t = 0; t < nc; ++t) {
+        const double s = block_sum(nrm[t], red);
+        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * PSTRIDE + t] = s;
+    }
+}
+
+__global__ void k_newvec_decide(DvdProb *P, int nprob, const double *partial) {
+    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= nprob) return;
+    DvdProb &p = P[pi];
+    if (p.phase != PH_ITER) return;
+    const int nc = p.ncand, k = p.kpass;
+    const int limadd = min(p.lim, p.mblock + k);
+    int nadded = 0, icvc = 0;
+    for (int t = 0; t < nc; ++t) {
+        double s = 0.0;
+        for (int c = 0; c < p.nchunks; ++c) s += partial[(p.chunk0 + c) * PSTRIDE + t];
+        const int indx = p.incv[t];
+        const double res = sqrt(s);
+        p.resn[indx - 1] = res;
+        const double lam = p.eigval[indx - 1];
+        const double thr = p.tol_rel > 0.0 ? p.tol_rel * fabs(lam + p.eshift) : p.critr;
+        if (res < thr) {
+            ++icvc;
+            p.icv[indx - 1] = 1;
+            if (icvc < nc) continue;
+            p.phase = PH_FINAL;
+            p.conv_all = 1;
+            return;
+        }
+        double gap;
+        if (indx == 1) {
+            gap = fabs(p.eigval[1] - p.eigval[0]);
+        } else {
+            const double gl = fabs(p.eigval[indx - 1] - p.eigval[indx - 2]);
+            const double gu = fabs(p.eigval[indx] - p.eigval[indx - 1]);
+            gap = fmin(gl, gu);
+        }
+        const double dl = fabs(p.oldval[indx - 1] - p.eigval[indx - 1]);
+        double rhs;
+        if (gap > res) rhs = sqrt(dl * res);
+        else rhs = fmin(dl, sqrt(dl));
+        p.epsil[indx - 1] = 0.0;
+        p.slot_cand[nadded] = t;
+        p.rhseps[nadded] = rhs;
+        p.lam_add[nadded] = lam - p.epsil[indx - 1];
+        p.incv[nadded] = indx;
+        ++nadded;
+        if (nadded + k == limadd) break;
+    }
+    p.nnew = nadded;
+    p.nncv = nadded;
+}
+
+// ------------------------------------------------- preconditioner (GDVD)
+__global__ __launch_bounds__(256) void k_precond(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row) {
+    DvdProb &p = P[chunk_prob[blockIdx.x]];
+    if (p.phase != PH_ITER) return;
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    const int nnew = p.nnew, k = p.kpass;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        double d0 = p.diag[r];
+        for (int a = 0; a < nnew; ++a) {
+            const int t = p.slot_cand[a];
+            const double w = fma(-p.rhseps[a], p.xc[r + (int64_t)t * p.n], p.rc[r + (int64_t)t * p.n]);
+            const double lam = p.lam_add[a];
+            const double d = d0 - lam;
+            p.basis[r + (int64_t)(k + a) * p.n] = (fabs(d) > 1.0e-06) ? w / d : w * 1.0e06;
+            d0 = d + lam;
+        }
+        p.diag[r] = d0;
+    }
+}
+
+// ---------------------------------------------------------- Gram kernels
+// partial[chunk][i*kb + j] = sum_r A[r, i] * B[r, j] over the chunk rows.
+// sel: 0 = (basis[:,0:kpass], basis[:,kpass:kpass+nnew])  (projection)
+//      1 = (basis[:,kpass:+nnew], itself)                 (Cholesky-QR)
+//      2 = (basis[:,0:kpass+nnew], ab[:,kpass:+nnew])     (ADDS)
+//      3 = (ab[:,0:nume] - eigval*basis[:,0:nume]) norms  (exit residuals)
+__global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
+                                              double *partial, int sel, int want_phase) {
+    DvdProb &p = P[chunk_prob[blockIdx.x]];
+    if (p.phase != want_phase) return;
+    __shared__ double ta[64][MAXLIM + 1];
+    __shared__ double tb[64][MAXNUME + 1];
+    const int k = p.kpass, nn = p.nnew;
+    const double *A, *B;
+    int ka, kb;
+    if (sel == 0) { A = p.basis; ka = k; B = p.basis + (int64_t)k * p.n; kb = nn; }
+    else if (sel == 1) { A = p.basis + (int64_t)k * p.n; ka = nn; B = A; kb = nn; }
+    else if (sel == 2) { A = p.basis; ka = k + nn; B = p.ab + (int64_t)k * p.n; kb = nn; }
+    else { A = p.basis; ka = p.nume; B = p.ab; kb = p.nume; }
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    const int nout = ka * kb;
+    constexpr int NQ = (PSTRIDE + 255) / 256;
+    double acc[NQ];  // outputs t, t+256, ...
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    for (int64_t rt = r0; rt < r1; rt += 64) {
+        const int rows = (int)min<int64_t>(64, r1 - rt);
+        for (int idx = threadIdx.x; idx < rows * ka; idx += blockDim.x) {
+            const int rr = idx % rows, i = idx / rows;
+            ta[rr][i] = A[rt + rr + (int64_t)i * p.n];
+        }
+        for (int idx = threadIdx.x; idx < rows * kb; idx += blockDim.x) {
+            const int rr = idx % rows, j = idx / rows;
+            double v = B[rt + rr + (int64_t)j * p.n];
+            if (sel == 3) v = fma(-p.eigval[j], A[rt + rr + (int64_t)j * p.n], v);
+            tb[rr][j] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int o = threadIdx.x + q * 256;
+            if (o < nout) {
+                const int i = o / kb, j = o % kb;
+                double s = acc[q];
+                if (sel == 3) {
+                    if (i == j)
+                        for (int rr = 0; rr < rows; ++rr) s = fma(tb[rr][j], tb[rr][j], s);
+                } else {
+                    for (int rr = 0; rr < rows; ++rr) s = fma(ta[rr][i], tb[rr][j], s);
+                }
+                acc[q] = s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int o = threadIdx.x + q * 256;
+        if (o < nout) partial[(int64_t)blockIdx.x * PSTRIDE + o] = acc[q];
+    }
+}
+
+// Sum chunk partials of problem blockIdx.x into out (fixed chunk order).
+__device__ void reduce_partials(const DvdProb &p, const double *partial, int nout, double *out) {
+    for (int o = threadIdx.x; o < nout; o += blockDim.x) {
+        double s = 0.0;
+        for (int c = 0; c < p.nchunks; ++c) s += partial[(p.chunk0 + c) * PSTRIDE + o];
+        out[o] = s;
+    }
+}
+
+// projection coefficients for the classical Gram-Schmidt pass
+__global__ void k_coef(DvdProb *P, const double *partial, int want_phase) {
+    DvdProb &p = P[blockIdx.x];
+    if (p.phase != want_phase) return;
+    reduce_partials(p, partial, p.kpass * p.nnew, p.coef);
+}
+
+// B[r, j] -= sum_i A[r, i] coef[i, j]
+__global__ __launch_bounds__(256) void k_update(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
+                                                int want_phase) {
+    DvdProb &p = P[chunk_prob[blockIdx.x]];
+    if (p.phase != want_phase) return;
+    __shared__ double cf[MAXLIM * MAXNUME];
+    const int k = p.kpass, nn = p.nnew;
+    for (int i = threadIdx.x; i < k * nn; i += blockDim.x) cf[i] = p.coef[i];
+    __syncthreads();
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    double *B = p.basis + (int64_t)k * p.n;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        double acc[MAXNUME];
+#pragma unroll
+        for (int j = 0; j < MAXNUME; ++j) acc[j] = 0.0;
+        for (int i = 0; i < k; ++i) {
+            const double a = p.basis[r + (int64_t)i * p.n];
+#pragma unroll
+            for (int j = 0; j < MAXNUME; ++j)
+                if (j < nn) acc[j] = fma(a, cf[i * nn + j], acc[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < MAXNUME; ++j)
+            if (j < nn) B[r + (int64_t)j * p.n] -= acc[j];
+    }
+}
+
+// Cholesky of the nnew x nnew Gram matrix; rinv = R^{-1} (upper).
+__global__ void k_chol(DvdProb *P, const double *partial, int want_phase) {
+    DvdProb &p = P[blockIdx.x];
+    if (p.phase != want_phase) return;
+    __shared__ double G[MAXNUME * MAXNUME];
+    const int nn = p.nnew;
+    reduce_partials(p, partial, nn * nn, G);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double R[MAXNUME][MAXNUME];
+    for (int i = 0; i < nn; ++i)
+        for (int j = 0; j < nn; ++j) R[i][j] = 0.0;
+    for (int j = 0; j < nn; ++j) {
+        double s = G[j * nn + j];
+        for (int q = 0; q < j; ++q) s -= R[q][j] * R[q][j];
+        if (!(s > 0.0)) {
+            p.ierr |= 1 << 20;  // orthogonalisation breakdown
+            s = 1e-300;
+        }
+        R[j][j] = sqrt(s);
+        for (int i = j + 1; i < nn; ++i) {
+            double v = G[j * nn + i];
+            for (int q = 0; q < j; ++q) v -= R[q][j] * R[q][i];
+            R[j][i] = v / R[j][j];
+        }
+    }
+    // invert upper triangular R
+    for (int j = 0; j < nn; ++j) {
+        for (int i = 0; i < nn; ++i) p.rinv[i * nn + j] = 0.0;
+        p.rinv[j * nn + j] = 1.0 / R[j][j];
+        for (int i = j - 1; i >= 0; --i) {
+            double s = 0.0;
+            for (int q = i + 1; q <= j; ++q) s += R[i][q] * p.rinv[q * nn + j];
+            p.rinv[i * nn + j] = -s / R[i][i];
+        }
+    }
+}
+
+// W = W R^{-1}
+__global__ __launch_bounds__(256) void k_scale(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
+                                               int want_phase) {
+    DvdProb &p = P[chunk_prob[blockIdx.x]];
+    if (p.phase != want_phase) return;
+    const int nn = p.nnew;
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    double *W = p.basis + (int64_t)p.kpass * p.n;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        double w[MAXNUME];
+#pragma unroll
+        for (int j = 0; j < MAXNUME; ++j) w[j] = (j < nn) ? W[r + (int64_t)j * p.n] : 0.0;
+#pragma unroll
+        for (int j = 0; j < MAXNUME; ++j) {
+            if (j < nn) {
+                double s = 0.0;
+                for (int i = 0; i <= j; ++i) s = fma(w[i], p.rinv[i * nn + j], s);
+                W[r + (int64_t)j * p.n] = s;
+            }
+        }
+    }
+}
+
+// ADDS: new columns of packed S, then KPASS += NNEW, NLOOPS += 1 (dvdson.f90:804-817)
+__global__ void k_adds(DvdProb *P, const double *partial, int want_phase, int count_loop) {
+    DvdProb &p = P[blockIdx.x];
+    if (p.phase != want_phase) return;
+    __shared__ double G[MAXLIM * MAXNUME];
+    const int k = p.kpass, nn = p.nnew;
+    reduce_partials(p, partial, (k + nn) * nn, G);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < (k + nn) * nn; idx += blockDim.x) {
+        const int i = idx / nn, iv = idx % nn;
+        const int j = k + iv;  // absolute column
+        if (i <= j) p.s[i + j * (j + 1) / 2] = G[idx];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    p.kpass = k + nn;
+    p.nmv += nn;
+    if (!count_loop) return;
+    if (p.refine_phase) p.refine_loops += 1;
+    p.nloops += 1;
+    if (p.nloops > p.maxiter) {
+        p.ierr += 2048;
+        p.nloops -= 1;
+        p.kpass -= nn;
+        p.phase = PH_FINAL;
+    }
+}
+
+// exit: OLDVAL = |OLDVAL - EIGVAL| (dvdson.f90:819-821)
+__global__ void k_final_pre(DvdProb *P, int nprob) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprob) return;
+    DvdProb &p = P[i];
+    if (p.phase != PH_FINAL) return;
+    for (int q = 0; q < p.nume; ++q) p.oldval[q] = fabs(p.oldval[q] - p.eigval[q]);
+}
+
+__global__ void k_final_post(DvdProb *P, const double *partial) {
+    DvdProb &p = P[blockIdx.x];
+    if (p.phase != PH_FINAL) return;
+    __shared__ double G[MAXNUME * MAXNUME];
+    const int nu = p.nume;
+    reduce_partials(p, partial, nu * nu, G);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < nu; ++i) p.resn[i] = sqrt(G[i * nu + i]);
+        p.phase = PH_DONE;
+    }
+}
+
+// ---------------------------------------------------- batched operator
+// y[:, j] = H x[:, j] for the NNEW new columns of every ITER problem
+// (x = basis[:, kpass..], y = ab[:, kpass..]); symmetric lower CSR, one
+// row per group of 16 lanes, atomic symmetric scatter.
+__global__ __launch_bounds__(256) void k_zero_new(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
+                                                  int want_phase) {
+    DvdProb &p = P[chunk_prob[blockIdx.x]];
+    if (p.phase != want_phase) return;
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    double *Y = p.ab + (int64_t)p.kpass * p.n;
+    for (int j = 0; j < p.nnew; ++j)
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) Y[r + (int64_t)j * p.n] = 0.0;
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void k_spmm_batch(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
+                                                    int want_phase) {
+    DvdProb &p = P[chunk_prob[blockIdx.x]];
+    if (p.phase != want_phase) return;
+    const int nn = p.nnew;
+    const int mode = p.refine_phase ? (p.mode == GCI_OP_FP64 ? GCI_OP_FP64 : GCI_OP_SPLIT_FULL) : p.mode;
+    const double *X = p.basis + (int64_t)p.kpass * p.n;
+    double *Y = p.ab + (int64_t)p.kpass * p.n;
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    const int lane = threadIdx.x % G;
+    for (int64_t r = r0 + threadIdx.x / G; r < r1; r += blockDim.x / G) {
+        const int64_t beg = p.rp[r], end = p.rp[r + 1];
+        for (int j = 0; j < nn; ++j) {
+            const double xr = X[r + (int64_t)j * p.n];
+            const double xrs = (mode == GCI_OP_SPLIT_BULK) ? (double)(float)xr : xr;
+            double acc = 0.0;
+            for (int64_t e = beg + lane; e < end; e += G) {
+                const int64_t c = p.col[e];
+                double a, xc = X[c + (int64_t)j * p.n];
+                if (mode == GCI_OP_FP64) a = p.val[e];
+                else if (mode == GCI_OP_SPLIT_BULK) { a = (double)p.hi[e]; xc = (double)(float)xc; }
+                else a = (double)p.hi[e] + (double)p.lo[e];
+                acc = fma(a, xc, acc);
+                unsafeAtomicAdd(&Y[c + (int64_t)j * p.n], a * xrs);
+            }
+            for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+            if (lane == 0) unsafeAtomicAdd(&Y[r + (int64_t)j * p.n], fma(p.diagH[r], xr, acc));
+        }
+    }
+}
+
+// ------------------------------------------------------------ INITDVD
+// INITDVD (dvdson.f90:865-946) sub-steps for problems in PH_INIT/PH_INIT_IDLE.
+struct InitStep {
+    int ist, iend, nnew;
+};
+
+// copy AB columns [ist-1, ist-1+new) to basis[:, iend..]
+__global__ __launch_bounds__(256) void k_init_copy(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
+                                                   const InitStep *steps) {
+    const int pi = chunk_prob[blockIdx.x];
+    DvdProb &p = P[pi];
+    if (p.phase != PH_INIT) return;
+    const InitStep st = steps[pi];
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    for (int j = 0; j < st.nnew; ++j)
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x)
+            p.basis[r + (int64_t)(st.iend + j) * p.n] = p.ab[r + (int64_t)(st.ist - 1 + j) * p.n];
+}
+
+// KPASS = IEND, NNEW = new for the next orthogonalisation/operator sub-step
+__global__ void k_init_state(DvdProb *P, int nprob, const InitStep *steps) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprob) return;
+    DvdProb &p = P[i];
+    if (p.phase != PH_INIT && p.phase != PH_INIT_IDLE) return;
+    const InitStep st = steps[i];
+    p.phase = st.nnew > 0 ? PH_INIT : PH_INIT_IDLE;
+    p.kpass = st.iend;
+    p.nnew = st.nnew;
+}
+
+// enter DVDRVR with KPASS = NIV (dvdson.f90:690-705)
+__global__ void k_init_done(DvdProb *P, int nprob) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprob) return;
+    DvdProb &p = P[i];
+    if (p.phase != PH_INIT && p.phase != PH_INIT_IDLE) return;
+    p.nncv = p.kpass;
+    p.nnew = 0;
+    p.nloops = 1;
+    p.ovflow = 0;
+    p.rest = min(max(p.lim / 2, 2 * p.nume), p.lim - 3);
+    for (int q = 0; q < p.nume; ++q) {
+        p.eigval[q] = 1.0e30;
+        p.icv[q] = 0;
+    }
+    p.phase = PH_ITER;
+}
+
+// refinement restart: FP64 operator, fresh GDVD state from the Ritz vectors
+// already in basis[:, 0:NUME]; the preconditioner diagonal restarts from H
+__global__ void k_refine_begin(DvdProb *P, int slot, double tol_final, double critr_final) {
+    DvdProb &p = P[slot];
+    p.bulk_loops = p.nloops;
+    p.bulk_nmv = p.nmv;
+    p.refine_phase = 1;
+    p.phase = PH_INIT;
+    p.niv = p.nume;
+    p.tol_rel = tol_final;
+    p.critr = critr_final;
+    p.nloops = 0;
+    p.conv_all = 0;
+}
+
+__global__ __launch_bounds__(256) void k_copy_diag(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row) {
+    DvdProb &p = P[chunk_prob[blockIdx.x]];
+    if (p.phase != PH_INIT) return;
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) p.diag[r] = p.diagH[r];
+}
+
+template <class K, class... Args>
+void launch(K kern, int grid, int block, hipStream_t s, Args... args) {
+    if (grid <= 0) return;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, args...);
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- host
+
+DavidsonEngine::DavidsonEngine(int capacity, hipStream_t s) : cap_(capacity), stream_(s) {
+    GCI_REQUIRE(capacity >= 1, "davidson: capacity < 1");
+    spec_.resize(cap_);
+    phase_.assign(cap_, PH_EMPTY);
+    refined_.assign(cap_, 0);
+    work_.resize(cap_);
+    dprob_.alloc(cap_);
+    dphase_.alloc(cap_);
+    std::vector<DvdProb> init(cap_);
+    for (int i = 0; i < cap_; ++i) {
+        std::memset(&init[i], 0, sizeof(DvdProb));
+        init[i].phase = PH_EMPTY;
+        init[i].slot = i;
+    }
+    GCI_HIP(hipMemcpy(dprob_.p, init.data(), cap_ * sizeof(DvdProb), hipMemcpyHostToDevice));
+    chunk_prob_.alloc(1);
+    chunk_row_.alloc(1);
+    partial_.alloc(PSTRIDE);
+}
+
+DavidsonEngine::~DavidsonEngine() {
+    if (gexec_) (void)hipGraphExecDestroy(gexec_);
+}
+
+size_t DavidsonEngine::workspace_bytes(const DvdSpec &s) {
+    const int64_t n = s.h ? s.h->n : 0;
+    const int lim = s.lim > 0 ? s.lim : (int)std::min<int64_t>(n, 2 * s.nume + 60);
+    return (size_t)n * (2 * lim + 1 + 2 * s.nume) * sizeof(double) + sizeof(DvdProb);
+}
+
+int DavidsonEngine::active() const {
+    int a = 0;
+    for (int i = 0; i < cap_; ++i) a += phase_[i] != PH_EMPTY;
+    return a;
+}
+
+void DavidsonEngine::rebuild_chunks() {
+    hprob_.clear();
+    hrow_.clear();
+    std::vector<std::pair<int64_t, int>> ch(cap_, {0, 0});
+    for (int i = 0; i < cap_; ++i) {
+        if (phase_[i] == PH_EMPTY) continue;
+        const int64_t n = spec_[i].h->n;
+        const int nc = (int)((n + CHUNK - 1) / CHUNK);
+        ch[i] = {(int64_t)hprob_.size(), nc};
+        for (int c = 0; c < nc; ++c) {
+            hprob_.push_back(i);
+            hrow_.push_back((int64_t)c * CHUNK);
+        }
+    }
+    total_chunks_ = (int64_t)hprob_.size();
+    if ((size_t)std::max<int64_t>(total_chunks_, 1) > chunk_prob_.n) {
+        GCI_HIP(hipStreamSynchronize(stream_));
+        chunk_prob_.alloc(total_chunks_);
+        chunk_row_.alloc(total_chunks_);
+        partial_.alloc((size_t)total_chunks_ * PSTRIDE);
+    }
+    if (total_chunks_) {
+        GCI_HIP(hipMemcpyAsync(chunk_prob_.p, hprob_.data(), total_chunks_ * sizeof(int), hipMemcpyHostToDevice,
+                               stream_));
+        GCI_HIP(hipMemcpyAsync(chunk_row_.p, hrow_.data(), total_chunks_ * sizeof(int64_t), hipMemcpyHostToDevice,
+                               stream_));
+    }
+    for (int i = 0; i < cap_; ++i) {
+        if (phase_[i] == PH_EMPTY) continue;
+        // chunk0 and nchunks are adjacent fields of DvdProb
+        struct {
+            int64_t c0;
+            int nc;
+        } v{ch[i].first, ch[i].second};
+        GCI_HIP(hipMemcpyAsync(&dprob_.p[i].chunk0, &v.c0, sizeof(int64_t), hipMemcpyHostToDevice, stream_));
+        GCI_HIP(hipMemcpyAsync(&dprob_.p[i].nchunks, &v.nc, sizeof(int), hipMemcpyHostToDevice, stream_));
+        GCI_HIP(hipStreamSynchronize(stream_));
+    }
+    GCI_HIP(hipStreamSynchronize(stream_));
+    graph_valid_ = false;
+}
+
+std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const std::vector<const double *> &init,
+                                       const std::vector<int> &niv) {
+    std::vector<int> slots;
+    for (size_t q = 0; q < specs.size(); ++q) {
+        const DvdSpec &sp = specs[q];
+        GCI_REQUIRE(sp.h && sp.h->n >= 2, "davidson: matrix order < 2");
+        const int64_t n = sp.h->n;
+        const int nume = sp.nume;
+        const int lim = sp.lim > 0 ? sp.lim : (int)std::min<int64_t>(n, 2 * nume + 60);
+        const int maxiter = sp.maxiter > 0 ? sp.maxiter : (int)std::max<int64_t>(200 * nume, n / 10);
+        GCI_REQUIRE(nume >= 1 && nume <= MAXNUME, "davidson: nvec must be in 1..16");
+        GCI_REQUIRE(lim >= 2 && lim <= MAXLIM, "davidson: LIM must be in 2..80");
+        if (sp.mode != GCI_OP_FP64 || sp.refine)
+            GCI_REQUIRE(sp.h->precision == GCI_SPLIT32, "davidson: split mode needs a split matrix");
+        // DVDSON input checks (dvdson.f90:319-398), ILOW = 1, IHIGH = NUME
+        int ierr = 0;
+        if (lim > n) ierr += 1;
+        if (nume > lim || (nume == lim && nume != n)) ierr += 128;
+        if (nume + 1 > lim) ierr += 128;  // INITDVD builds NUME+1 vectors
+        if (sp.mblock < 1 || sp.mblock > nume) ierr += 256;
+        GCI_REQUIRE(ierr == 0, "davidson: DVDSON input check failed, IERR = " + std::to_string(ierr));
+        GCI_REQUIRE(niv[q] >= 1 && niv[q] <= lim, "davidson: NIV out of range");
+        int slot = -1;
+        for (int i = 0; i < cap_; ++i)
+            if (phase_[i] == PH_EMPTY && std::find(slots.begin(), slots.end(), i) == slots.end()) {
+                slot = i;
+                break;
+            }
+        GCI_REQUIRE(slot >= 0, "davidson: no free slot");
+        slots.push_back(slot);
+        DvdSpec s2 = sp;
+        s2.lim = lim;
+        s2.maxiter = maxiter;
+        spec_[slot] = s2;
+        work_[slot].alloc((size_t)n * (2 * lim + 1 + 2 * nume));
+        double *w = work_[slot].p;
+        DvdProb p;
+        std::memset(&p, 0, sizeof(p));
+        p.n = n;
+        p.lim = lim;
+        p.nume = nume;
+        p.mblock = sp.mblock;
+        p.maxiter = maxiter;
+        p.crite = sp.crite;
+        p.critc = sp.critc;
+        p.critr = sp.critr;
+        p.mode = sp.mode;
+        p.rp = sp.h->rp.p;
+        p.col = sp.h->col.p;
+        p.val = sp.h->val.p;
+        p.hi = sp.h->hi.p;
+        p.lo = sp.h->lo.p;
+        p.diagH = sp.h->diag.p;
+        p.basis = w;
+        p.ab = w + (size_t)n * lim;
+        p.diag = w + (size_t)n * 2 * lim;
+        p.xc = p.diag + n;
+        p.rc = p.xc + (size_t)n * nume;
+        p.tol_rel = sp.tol_rel;
+        p.eshift = sp.eshift;
+        p.slot = slot;
+        p.phase = PH_INIT;
+        p.niv = niv[q];
+        GCI_HIP(hipMemsetAsync(w, 0, work_[slot].bytes(), stream_));
+        GCI_HIP(hipMemcpyAsync(p.diag, sp.h->diag.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        std::vector<double> iv;
+        const double *src = init[q];
+        if (!src) {
+            // unit vectors on the niv smallest diagonal elements
+            std::vector<double> d(n);
+            GCI_HIP(hipMemcpyAsync(d.data(), sp.h->diag.p, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+            GCI_HIP(hipStreamSynchronize(stream_));
+            std::vector<int64_t> idx(n);
+            std::iota(idx.begin(), idx.end(), 0);
+            std::partial_sort(idx.begin(), idx.begin() + niv[q], idx.end(),
+                              [&](int64_t a, int64_t b) { return d[a] < d[b] || (d[a] == d[b] && a < b); });
+            iv.assign((size_t)n * niv[q], 0.0);
+            for (int j = 0; j < niv[q]; ++j) iv[(size_t)j * n + idx[j]] = 1.0;
+            src = iv.data();
+        }
+        GCI_HIP(hipMemcpyAsync(p.basis, src, (size_t)n * niv[q] * sizeof(double), hipMemcpyHostToDevice, stream_));
+        GCI_HIP(hipMemcpyAsync(&dprob_.p[slot], &p, sizeof(DvdProb), hipMemcpyHostToDevice, stream_));
+        GCI_HIP(hipStreamSynchronize(stream_));
+        phase_[slot] = PH_INIT;
+        refined_[slot] = 0;
+    }
+    rebuild_chunks();
+    run_initdvd(slots);
+    return slots;
+}
+
+void DavidsonEngine::run_initdvd(const std::vector<int> &slots) {
+    // INITDVD (dvdson.f90:865-946) for the problems in PH_INIT: OP on the
+    // NIV initial vectors, then orthonormalised AB vectors up to NUME+1,
+    // then S = B^T AB.
+    const int G = (int)total_chunks_;
+    DBuf<InitStep> dsteps(cap_);
+    std::vector<InitStep> steps(cap_, InitStep{0, 0, 0});
+    std::vector<int> ist(cap_, 1), iend(cap_, 0), nume(cap_, 0);
+    std::vector<DvdProb> cur(1);
+    for (int sl : slots) {
+        GCI_HIP(hipMemcpy(cur.data(), &dprob_.p[sl], sizeof(DvdProb), hipMemcpyDeviceToHost));
+        iend[sl] = cur[0].niv;
+        nume[sl] = cur[0].nume;
+        steps[sl] = InitStep{1, 0, cur[0].niv};
+    }
+    auto push = [&]() {
+        GCI_HIP(hipMemcpyAsync(dsteps.p, steps.data(), cap_ * sizeof(InitStep), hipMemcpyHostToDevice, stream_));
+        launch(k_init_state, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, (const InitStep *)dsteps.p);
+        GCI_HIP(hipStreamSynchronize(stream_));
+    };
+    push();
+    launch_op(PH_INIT);
+    for (;;) {
+        bool any = false;
+        std::fill(steps.begin(), steps.end(), InitStep{0, 0, 0});
+        for (int sl : slots) {
+            const int need = nume[sl] + 1;
+            if (iend[sl] < need) {
+                const int nw = std::min(need - iend[sl], iend[sl] - ist[sl] + 1);
+                steps[sl] = InitStep{ist[sl], iend[sl], nw};
+                ist[sl] = iend[sl] + 1;
+                iend[sl] += nw;
+                any = true;
+            } else {
+                steps[sl] = InitStep{0, iend[sl], 0};
+            }
+        }
+        if (!any) break;
+        push();
+        launch(k_init_copy, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p,
+               (const InitStep *)dsteps.p);
+        launch_orth(2, PH_INIT);
+        launch_op(PH_INIT);
+    }
+    for (int sl : slots) steps[sl] = InitStep{0, 0, iend[sl]};
+    push();
+    launch(k_gram, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p,
+           partial_.p, 2, (int)PH_INIT);
+    launch(k_adds, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, (int)PH_INIT, 0);
+    launch(k_init_done, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_);
+    GCI_HIP(hipGetLastError());
+    GCI_HIP(hipStreamSynchronize(stream_));
+    for (int sl : slots) phase_[sl] = PH_ITER;
+}
+
+void DavidsonEngine::launch_orth(int passes, int phase) {
+    const int G = (int)total_chunks_;
+    const int *cp = chunk_prob_.p;
+    const int64_t *cr = chunk_row_.p;
+    for (int pass = 0; pass < passes; ++pass) {
+        launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 0, phase);
+        launch(k_coef, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, phase);
+        launch(k_update, G, 256, stream_, dprob_.p, cp, cr, phase);
+        launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 1, phase);
+        launch(k_chol, cap_, 64, stream_, dprob_.p, (const double *)partial_.p, phase);
+        launch(k_scale, G, 256, stream_, dprob_.p, cp, cr, phase);
+    }
+}
+
+void DavidsonEngine::launch_op(int phase) {
+    const int G = (int)total_chunks_;
+    launch(k_zero_new, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p, phase);
+    launch(k_spmm_batch<16>, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p,
+           phase);
+}
+
+void DavidsonEngine::launch_finalize() {
+    const int G = (int)total_chunks_;
+    const int *cp = chunk_prob_.p;
+    const int64_t *cr = chunk_row_.p;
+    launch(k_final_pre, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_);
+    launch(k_rotate, G, 256, stream_, dprob_.p, cp, cr, 1);
+    launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 3, (int)PH_FINAL);
+    launch(k_final_post, cap_, 256, stream_, dprob_.p, (const double *)partial_.p);
+}
+
+void DavidsonEngine::launch_iteration() {
+    const int G = (int)total_chunks_;
+    const int *cp = chunk_prob_.p;
+    const int64_t *cr = chunk_row_.p;
+    launch(k_rr, cap_, 256, stream_, dprob_.p);
+    launch(k_rotate, G, 256, stream_, dprob_.p, cp, cr, 0);
+    launch(k_ovf_fix, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_);
+    launch(k_newvec, G, 256, stream_, dprob_.p, cp, cr, partial_.p);
+    launch(k_newvec_decide, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, (const double *)partial_.p);
+    launch(k_precond, G, 256, stream_, dprob_.p, cp, cr);
+    launch_orth(2, PH_ITER);
+    launch_op(PH_ITER);
+    launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 2, (int)PH_ITER);
+    launch(k_adds, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, (int)PH_ITER, 1);
+    launch_finalize();
+}
+
+void DavidsonEngine::step(int iters) {
+    if (total_chunks_ == 0) return;
+    if (use_graph) {
+        if (!graph_valid_) {
+            if (gexec_) {
+                (void)hipGraphExecDestroy(gexec_);
+                gexec_ = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            GCI_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            launch_iteration();
+            GCI_HIP(hipStreamEndCapture(stream_, &g));
+            GCI_HIP(hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(g);
+            graph_valid_ = true;
+        }
+        for (int it = 0; it < iters; ++it) GCI_HIP(hipGraphLaunch(gexec_, stream_));
+    } else {
+        for (int it = 0; it < iters; ++it) launch_iteration();
+    }
+    GCI_HIP(hipGetLastError());
+}
+
+std::vector<int> DavidsonEngine::poll() {
+    GCI_HIP(hipMemcpy2DAsync(dphase_.p, sizeof(int), &dprob_.p[0].phase, sizeof(DvdProb), sizeof(int), cap_,
+                             hipMemcpyDeviceToDevice, stream_));
+    std::vector<int> ph(cap_);
+    GCI_HIP(hipMemcpyAsync(ph.data(), dphase_.p, cap_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    GCI_HIP(hipStreamSynchronize(stream_));
+    std::vector<int> done;
+    for (int i = 0; i < cap_; ++i) {
+        if (phase_[i] == PH_EMPTY) continue;
+        if (ph[i] == PH_DONE && phase_[i] != PH_DONE) done.push_back(i);
+        phase_[i] = ph[i];
+    }
+    return done;
+}
+
+bool DavidsonEngine::needs_refine(int slot) const { return spec_[slot].refine && !refined_[slot]; }
+
+void DavidsonEngine::refine_restart(const std::vector<int> &slots) {
+    if (slots.empty()) return;
+    for (int sl : slots) {
+        GCI_REQUIRE(phase_[sl] == PH_DONE && needs_refine(sl), "davidson: slot not ready for refinement");
+        const DvdSpec &sp = spec_[sl];
+        launch(k_refine_begin, 1, 1, stream_, dprob_.p, sl, sp.tol_final, sp.critr_final > 0 ? sp.critr_final : sp.critr);
+        refined_[sl] = 1;
+        phase_[sl] = PH_INIT;
+    }
+    launch(k_copy_diag, (int)total_chunks_, 256, stream_, dprob_.p, (const int *)chunk_prob_.p,
+           (const int64_t *)chunk_row_.p);
+    GCI_HIP(hipStreamSynchronize(stream_));
+    run_initdvd(slots);
+}
+
+void DavidsonEngine::extract(int slot, double *eval, double *evec, double *resid, gci_dvd_stats *st) {
+    DvdProb p;
+    GCI_HIP(hipMemcpy(&p, &dprob_.p[slot], sizeof(DvdProb), hipMemcpyDeviceToHost));
+    for (int q = 0; q < p.nume; ++q) {
+        if (eval) eval[q] = p.eigval[q];
+        if (resid) resid[q] = p.resn[q];
+    }
+    if (evec) GCI_HIP(hipMemcpy(evec, p.basis, (size_t)p.n * p.nume * sizeof(double), hipMemcpyDeviceToHost));
+    if (st) {
+        st->nloops = p.refine_phase ? p.bulk_loops + p.nloops : p.nloops;
+        st->nmv = p.nmv;
+        st->ierr = p.ierr;
+        st->refine_loops = p.refine_phase ? p.nloops : 0;
+    }
+}
+
+void DavidsonEngine::release(int slot) {
+    GCI_REQUIRE(slot >= 0 && slot < cap_ && phase_[slot] != PH_EMPTY, "davidson: bad slot");
+    GCI_HIP(hipStreamSynchronize(stream_));
+    const int empty = PH_EMPTY;
+    GCI_HIP(hipMemcpy(&dprob_.p[slot].phase, &empty, sizeof(int), hipMemcpyHostToDevice));
+    work_[slot].release();
+    spec_[slot] = DvdSpec{};
+    phase_[slot] = PH_EMPTY;
+    rebuild_chunks();
+}
+
+// ------------------------------------------------- dense eigensolver
+
+namespace {
+rocblas_handle g_rb = nullptr;
+std::mutex g_rb_mu;
+}  // namespace
+
+void dense_lowest(int n, double *d_a, int nev, double *d_eval, double *d_evec, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_rb_mu);
+    if (!g_rb) {
+        GCI_REQUIRE(rocblas_create_handle(&g_rb) == rocblas_status_success, "rocblas_create_handle failed");
+    }
+    rocblas_set_stream(g_rb, s);
+    DBuf<double> w(n);
+    DBuf<rocblas_int> ifail(n), info(1), nfound(1);
+    rocblas_status st = rocsolver_dsyevx(g_rb, rocblas_evect_original, rocblas_erange_index, rocblas_fill_upper, n,
+                                         d_a, n, 0.0, 0.0, 1, nev, 0.0, nfound.p, w.p, d_evec, n, ifail.p, info.p);
+    GCI_REQUIRE(st == rocblas_status_success, "rocsolver_dsyevx failed");
+    rocblas_int hinfo = 0;
+    GCI_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(rocblas_int), hipMemcpyDeviceToHost, s));
+    GCI_HIP(hipMemcpyAsync(d_eval, w.p, nev * sizeof(double), hipMemcpyDeviceToDevice, s));
+    GCI_HIP(hipStreamSynchronize(s));
+    if (hinfo != 0) throw Error{GCI_ERR_DEVICE, "maneig: Failure in DSYEVX (info " + std::to_string(hinfo) + ")"};
+}
+
+}  // namespace gci

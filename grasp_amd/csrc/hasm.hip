@@ -1,0 +1,296 @@
+// hasm.hip -- Hamiltonian assembly from CPU-produced angular lists.
+//
+// Replaces the SETHAM element loop (rci90/setham_gg.f90:162-385) for the
+// Dirac-Coulomb operator.  The angular lists (ONESCALAR's IA, IB, TSHELL and
+// RKCO_GG's LABEL/COEFF per CSF pair) are consumed exactly as the producer
+// emits them.  One lane reduces one pair's terms in list order (segmented
+// gather-multiply-reduce: one-body I(a,b) and two-body R^k gathers from the
+// device-resident tables, RKINTC canonicalisation and binary search), with
+// floating-point contraction off, so each element equals the sequential
+// Fortran sum bit for bit given the same integral tables.  Kept elements are
+// compacted into the lower-triangle CSR by an exclusive scan over the pairs
+// (pairs are sorted by row, then column, so the compacted order is the CSR
+// order).
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+
+#include "gci_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace gci {
+
+HMat *upload(int64_t n, const int64_t *iendc, const int32_t *irow, const double *emt, int base, double shift,
+             int precision);
+
+namespace {
+
+constexpr double CUTOFF = 1.0e-12;  // setham_gg.f90:105
+
+struct AsmIn {
+    int64_t npairs;
+    int nw;
+    int lfordr, iccut;
+    const int32_t *pic, *pir, *oia, *oib;
+    const int64_t *tsoff;
+    const double *tsh;
+    const int64_t *tbb, *tbe;
+    const int32_t *lab;
+    const double *cof;
+    const double *iab;
+    const int32_t *kst;
+    const int32_t *keys;
+    const double *vals;
+    int j2max;
+};
+
+// RKINTC (rci90/rkintc.f90:45-103); returns false where the Fortran STOPs.
+__device__ bool rkintc(const AsmIn &in, int ia, int ib, int ic, int id, int k, double &out) {
+    if (k < 0 || k > in.j2max) return false;
+    const int key = in.nw + 1;
+    int t;
+    if (ia > ic) { t = ic; ic = ia; ia = t; }
+    if (ib > id) { t = id; id = ib; ib = t; }
+    if (ia > ib) {
+        t = ib; ib = ia; ia = t;
+        t = id; id = ic; ic = t;
+    }
+    const int index = ((ia * key + ib) * key + ic) * key + id;
+    int jl = in.kst[k], ju = in.kst[k + 1] - 1;
+    if (ju < jl) return false;
+    if (index < in.keys[jl - 1] || index > in.keys[ju - 1]) return false;
+    while (ju - jl > 1) {
+        const int jm = (ju + jl) / 2;
+        if (in.keys[jm - 1] > index) ju = jm;
+        else jl = jm;
+    }
+    int loc;
+    if (index == in.keys[ju - 1]) loc = ju;
+    else if (index == in.keys[jl - 1]) loc = jl;
+    else return false;
+    out = in.vals[loc - 1];
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_elements(AsmIn in, double *__restrict__ elem, int *__restrict__ keep_off,
+                                                  int *__restrict__ err) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < in.npairs;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int ic = in.pic[p], ir = in.pir[p];
+        if (in.lfordr && ir > in.iccut && ir != ic) {  // setham_gg.f90:172-174
+            keep_off[p] = 0;
+            elem[p] = 0.0;
+            continue;
+        }
+        double e = 0.0;
+        const int ia = in.oia[p], ib = in.oib[p];
+        if (ia != 0) {  // setham_gg.f90:221-261
+            const double *ts = in.tsh + in.tsoff[p];
+            if (ia == ib) {
+                for (int a = 0; a < in.nw; ++a) {
+                    const double t = ts[a];
+                    if (fabs(t) > CUTOFF) e = e + in.iab[(size_t)a * in.nw + a] * t;
+                }
+            } else {
+                const double t = ts[0];
+                if (fabs(t) > CUTOFF) {
+                    const int lo = ia < ib ? ia : ib, hi = ia < ib ? ib : ia;  // IABINT swap
+                    e = e + in.iab[(size_t)(lo - 1) * in.nw + (hi - 1)] * t;
+                }
+            }
+        }
+        for (int64_t q = in.tbb[p]; q < in.tbe[p]; ++q) {  // setham_gg.f90:273-292
+            const double v = in.cof[q];
+            if (fabs(v) > CUTOFF) {
+                const int32_t *l = in.lab + 5 * q;
+                double tegral = 0.0;
+                if (!rkintc(in, l[0], l[1], l[2], l[3], l[4], tegral)) {
+                    atomicOr(err, 1);
+                    tegral = 0.0;
+                }
+                e = e + tegral * v;
+            }
+        }
+        elem[p] = e;
+        keep_off[p] = (ir != ic) && (fabs(e) > CUTOFF);  // setham_gg.f90:364
+    }
+}
+
+__global__ __launch_bounds__(256) void k_compact(int64_t npairs, const int32_t *__restrict__ pic,
+                                                 const int32_t *__restrict__ pir, const double *__restrict__ elem,
+                                                 const int *__restrict__ keep_off, const int64_t *__restrict__ pos,
+                                                 int32_t *__restrict__ col, double *__restrict__ val,
+                                                 double *__restrict__ diag) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npairs;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int ic = pic[p], ir = pir[p];
+        if (ir == ic) {
+            diag[ic - 1] = elem[p];
+        } else if (keep_off[p]) {
+            col[pos[p]] = ir - 1;
+            val[pos[p]] = elem[p];
+        }
+    }
+}
+
+__global__ void k_row_ptr(int64_t n, const int64_t *__restrict__ row_first, const int64_t *__restrict__ pos,
+                          int64_t total, int64_t *__restrict__ rp) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= n; i += (int64_t)gridDim.x * blockDim.x)
+        rp[i] = (i < n) ? pos[row_first[i]] : total;
+}
+
+__global__ void k_shift_split(int64_t n, double *__restrict__ diag, double shift, int64_t nnz,
+                              const double *__restrict__ v, float *__restrict__ hi, float *__restrict__ lo) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n || i < nnz;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (i < n) diag[i] = diag[i] + shift;
+        if (hi && i < nnz) {
+            const double a = v[i];
+            const float h = (float)a;
+            hi[i] = h;
+            lo[i] = (float)(a - (double)h);
+        }
+    }
+}
+
+template <class T>
+DBuf<T> to_dev(const T *h, size_t n) {
+    DBuf<T> d(n);
+    if (n) GCI_HIP(hipMemcpy(d.p, h, n * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+}  // namespace
+
+}  // namespace gci
+
+using namespace gci;
+
+extern "C" int gci_h_assemble(int ncf, int nw, const gci_pair_lists *L, const double *iab, const int32_t *kstart,
+                              const int32_t *keys, const double *vals, int64_t nkeys, int lfordr, int iccut,
+                              int precision, gci_hmat **out, double *eav, int64_t *nelmnt) {
+    return guarded([&] {
+        GCI_REQUIRE(L && iab && kstart && out && eav, "gci_h_assemble: null argument");
+        GCI_REQUIRE(ncf >= 1 && nw >= 1 && nw <= 127, "gci_h_assemble: bad ncf/nw");
+        GCI_REQUIRE(precision == GCI_FP64 || precision == GCI_SPLIT32, "gci_h_assemble: bad precision");
+        const int64_t np = L->npairs;
+        GCI_REQUIRE(np >= ncf, "gci_h_assemble: fewer pairs than rows");
+        // list validation (host, O(npairs)): rows ascending, columns ascending
+        // within a row, ir <= ic, the diagonal last in each row
+        std::vector<int64_t> row_first(ncf, -1);
+        int prev_ic = 0, prev_ir = 0;
+        for (int64_t p = 0; p < np; ++p) {
+            const int ic = L->pair_ic[p], ir = L->pair_ir[p];
+            GCI_REQUIRE(ic >= 1 && ic <= ncf && ir >= 1 && ir <= ic, "gci_h_assemble: pair out of the lower triangle");
+            if (ic != prev_ic) {
+                GCI_REQUIRE(ic == prev_ic + 1 && (prev_ic == 0 || prev_ir == prev_ic),
+                            "gci_h_assemble: rows must be consecutive and end with their diagonal");
+                row_first[ic - 1] = p;
+            } else {
+                GCI_REQUIRE(ir > prev_ir, "gci_h_assemble: columns not ascending");
+            }
+            prev_ic = ic;
+            prev_ir = ir;
+            const int ia = L->one_ia[p], ib = L->one_ib[p];
+            GCI_REQUIRE(ia >= 0 && ia <= nw && ib >= 0 && ib <= nw, "gci_h_assemble: one-body label out of range");
+            const int64_t cnt = ia == 0 ? 0 : (ia == ib ? nw : 1);
+            GCI_REQUIRE(cnt == 0 || (L->tsh_off[p] >= 0 && L->tsh_off[p] + cnt <= L->ntshell),
+                        "gci_h_assemble: TSHELL offset out of range");
+            GCI_REQUIRE(L->tb_beg[p] >= 0 && L->tb_beg[p] <= L->tb_end[p] && L->tb_end[p] <= L->nterms,
+                        "gci_h_assemble: two-body range out of bounds");
+        }
+        GCI_REQUIRE(prev_ic == ncf && prev_ir == ncf, "gci_h_assemble: last row incomplete");
+        int j2max = 0;
+        while (j2max < 21 && kstart[j2max + 1] <= nkeys + 1 && kstart[j2max + 1] >= kstart[j2max]) {
+            if (kstart[j2max + 1] == nkeys + 1) break;
+            ++j2max;
+        }
+        for (int64_t q = 0; q < L->nterms; ++q) {
+            const int32_t *l = L->label + 5 * q;
+            GCI_REQUIRE(l[0] >= 1 && l[0] <= nw && l[1] >= 1 && l[1] <= nw && l[2] >= 1 && l[2] <= nw && l[3] >= 1 &&
+                            l[3] <= nw && l[4] >= 0 && l[4] <= 20,
+                        "gci_h_assemble: two-body label out of range");
+        }
+        hipStream_t s = nullptr;
+        auto dpic = to_dev(L->pair_ic, np), dpir = to_dev(L->pair_ir, np);
+        auto doia = to_dev(L->one_ia, np), doib = to_dev(L->one_ib, np);
+        auto dtso = to_dev(L->tsh_off, np);
+        auto dtsh = to_dev(L->tshell, (size_t)L->ntshell);
+        auto dtbb = to_dev(L->tb_beg, np), dtbe = to_dev(L->tb_end, np);
+        auto dlab = to_dev(L->label, (size_t)L->nterms * 5);
+        auto dcof = to_dev(L->coeff, (size_t)L->nterms);
+        auto diab = to_dev(iab, (size_t)nw * nw);
+        auto dkst = to_dev(kstart, (size_t)j2max + 2);
+        auto dkeys = to_dev(keys, (size_t)nkeys);
+        auto dvals = to_dev(vals, (size_t)nkeys);
+        AsmIn in{np,     nw,     lfordr,  iccut,  dpic.p,  dpir.p,  doia.p, doib.p, dtso.p, dtsh.p,
+                 dtbb.p, dtbe.p, dlab.p,  dcof.p, diab.p,  dkst.p,  dkeys.p, dvals.p, j2max};
+        DBuf<double> elem(np);
+        DBuf<int> keep(np), err(1);
+        DBuf<int64_t> pos(np + 1);
+        GCI_HIP(hipMemset(err.p, 0, sizeof(int)));
+        const int64_t blocks = std::min<int64_t>((np + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_elements, dim3((unsigned)blocks), dim3(256), 0, s, in, elem.p, keep.p, err.p);
+        GCI_HIP(hipGetLastError());
+        // exclusive scan of the off-diagonal keep flags -> CSR positions
+        size_t tmp_bytes = 0;
+        GCI_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, keep.p, pos.p, np + 1, s));
+        DBuf<char> tmp(tmp_bytes);
+        // keep has np entries; scan np+1 so pos[np] is the total: use a
+        // transform iterator over keep with a zero past the end
+        {
+            DBuf<int> keep1(np + 1);
+            GCI_HIP(hipMemcpyAsync(keep1.p, keep.p, np * sizeof(int), hipMemcpyDeviceToDevice, s));
+            GCI_HIP(hipMemsetAsync(keep1.p + np, 0, sizeof(int), s));
+            GCI_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, keep1.p, pos.p, np + 1, s));
+        }
+        int herr = 0;
+        int64_t total = 0;
+        GCI_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        GCI_HIP(hipMemcpyAsync(&total, pos.p + np, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        GCI_HIP(hipStreamSynchronize(s));
+        if (herr) throw Error{GCI_ERR_NOTFOUND, "Rkintc Integral not found"};
+        auto *h = new gci_hmat();
+        try {
+            h->n = ncf;
+            h->nnz = total;
+            h->precision = precision;
+            GCI_HIP(hipStreamCreate(&h->stream));
+            h->rp.alloc(ncf + 1);
+            h->col.alloc(total);
+            h->val.alloc(total);
+            h->diag.alloc(ncf);
+            auto drf = to_dev(row_first.data(), row_first.size());
+            hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(256), 0, s, np, dpic.p, dpir.p, elem.p, keep.p,
+                               pos.p, h->col.p, h->val.p, h->diag.p);
+            hipLaunchKernelGGL(k_row_ptr, dim3((unsigned)std::min<int64_t>((ncf + 256) / 256, 4096)), dim3(256), 0, s,
+                               (int64_t)ncf, drf.p, pos.p, total, h->rp.p);
+            GCI_HIP(hipGetLastError());
+            // EAV = sum of diagonals / NCF (setham_gg.f90:382, genmat2.f90:73)
+            std::vector<double> dg(ncf);
+            GCI_HIP(hipMemcpyAsync(dg.data(), h->diag.p, ncf * sizeof(double), hipMemcpyDeviceToHost, s));
+            GCI_HIP(hipStreamSynchronize(s));
+            double sum = 0.0;
+            for (int i = 0; i < ncf; ++i) sum = sum + dg[i];
+            const double e = sum / (double)ncf;
+            *eav = e;
+            if (precision == GCI_SPLIT32) {
+                h->hi.alloc(total);
+                h->lo.alloc(total);
+            }
+            const int64_t nb = std::min<int64_t>((std::max<int64_t>(ncf, total) + 255) / 256, 8192);
+            hipLaunchKernelGGL(k_shift_split, dim3((unsigned)nb), dim3(256), 0, s, (int64_t)ncf, h->diag.p, -e, total,
+                               h->val.p, precision == GCI_SPLIT32 ? h->hi.p : nullptr,
+                               precision == GCI_SPLIT32 ? h->lo.p : nullptr);
+            GCI_HIP(hipGetLastError());
+            GCI_HIP(hipStreamSynchronize(s));
+            if (nelmnt) *nelmnt = total + ncf;
+        } catch (...) {
+            if (h->stream) (void)hipStreamDestroy(h->stream);
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}

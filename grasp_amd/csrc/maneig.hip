@@ -1,0 +1,231 @@
+// maneig.hip -- MANEIG dispatch, GDVD entry point and the batched solve.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "dvd.h"
+
+namespace gci {
+
+namespace {
+
+// Dense ns x ns leading block of the symmetric H (column-major, both
+// triangles), as INIEST2 and the NCF <= 2000 branch of MANEIG build it
+// (lib9290/iniest2.f90:62-72, rci90/maneig.f90:200-213).
+__global__ void k_dense_block(int64_t ns, const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                              const double *__restrict__ val, const double *__restrict__ diag,
+                              double *__restrict__ a) {
+    const int64_t i = blockIdx.x;
+    if (i >= ns) return;
+    for (int64_t e = rp[i] + threadIdx.x; e < rp[i + 1]; e += blockDim.x) {
+        const int64_t c = col[e];
+        if (c < ns) {
+            a[i + c * ns] = val[e];
+            a[c + i * ns] = val[e];
+        }
+    }
+    if (threadIdx.x == 0) a[i + i * ns] = diag[i];
+}
+
+void lowest_of_leading_block(const HMat &h, int64_t ns, int nev, std::vector<double> &eval,
+                             std::vector<double> &evec) {
+    DBuf<double> a((size_t)ns * ns), w(nev), z((size_t)ns * nev);
+    GCI_HIP(hipMemsetAsync(a.p, 0, a.bytes(), h.stream));
+    hipLaunchKernelGGL(k_dense_block, dim3((unsigned)ns), dim3(64), 0, h.stream, ns, h.rp.p, h.col.p, h.val.p,
+                       h.diag.p, a.p);
+    GCI_HIP(hipGetLastError());
+    dense_lowest((int)ns, a.p, nev, w.p, z.p, h.stream);
+    eval.resize(nev);
+    evec.resize((size_t)ns * nev);
+    GCI_HIP(hipMemcpy(eval.data(), w.p, nev * sizeof(double), hipMemcpyDeviceToHost));
+    GCI_HIP(hipMemcpy(evec.data(), z.p, evec.size() * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+void sign_fix(int64_t n, int nvec, double *evec) {
+    // rci90/maneig.f90:532-555: dominant component made positive
+    for (int j = 0; j < nvec; ++j) {
+        double *v = evec + (size_t)j * n, amax = 0.0;
+        int64_t ia = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            const double wa = std::fabs(v[i]);
+            if (wa <= amax) continue;
+            amax = wa;
+            ia = i;
+        }
+        if (v[ia] >= 0.0) continue;
+        for (int64_t i = 0; i < n; ++i) v[i] = -v[i];
+    }
+}
+
+DvdSpec resolve(const HMat &h, const gci_dvd_params *p) {
+    DvdSpec s;
+    s.h = &h;
+    s.nume = p->nvec;
+    GCI_REQUIRE(s.nume >= 1 && s.nume <= MAXNUME, "davidson: nvec must be in 1..16");
+    s.lim = p->lim > 0 ? p->lim : (int)std::min<int64_t>(h.n, 2 * s.nume + 60);
+    s.mblock = p->mblock > 0 ? p->mblock : 1;
+    s.maxiter = p->maxiter > 0 ? p->maxiter : (int)std::max<int64_t>(200 * s.nume, h.n / 10);
+    s.crite = p->crite > 0 ? p->crite : 1.0e-17;
+    s.critc = p->critc > 0 ? p->critc : 1.0e-9;
+    s.critr = p->critr > 0 ? p->critr : 1.0e-9;
+    s.mode = p->mode;
+    s.refine = p->mode != GCI_OP_FP64 && p->refine;
+    if (s.mode != GCI_OP_FP64) {
+        // the split-precision bulk phase stops at a residual the fp32
+        // operator can certify (1e-7); the FP64 refinement applies CRITR
+        s.critr_final = s.critr;
+        s.critr = std::max(s.critr, 1.0e-7);
+    }
+    return s;
+}
+
+// Runs an engine until every slot is done, restarting split-precision slots
+// once with the FP64 operator.
+void run_to_completion(DavidsonEngine &eng, int guard_iters) {
+    int it = 0;
+    for (;;) {
+        eng.step(8);
+        it += 8;
+        std::vector<int> done = eng.poll();
+        std::vector<int> ref;
+        for (int sl : done)
+            if (eng.needs_refine(sl)) ref.push_back(sl);
+        if (!ref.empty()) eng.refine_restart(ref);
+        bool all = true;
+        for (int i = 0; i < eng.capacity(); ++i) {
+            const int ph = eng.phase(i);
+            if (ph != PH_EMPTY && ph != PH_DONE) all = false;
+        }
+        if (all) break;
+        GCI_REQUIRE(it <= guard_iters, "davidson: iteration guard exceeded");
+    }
+}
+
+// One GDVD solve on one block; init: n x niv initial vectors (host) or null
+// for INIEST2.
+void solve_one(HMat &h, const DvdSpec &sp, const double *init, int niv, double *eval, double *evec,
+               double *resid, gci_dvd_stats *st) {
+    std::vector<double> iv;
+    if (!init || niv <= 0) {
+        // INIEST2 (lib9290/iniest2.f90): leading min(2000, n) block
+        const int64_t ns = std::min<int64_t>(2000, h.n);
+        std::vector<double> w, z;
+        lowest_of_leading_block(h, ns, sp.nume, w, z);
+        iv.assign((size_t)h.n * sp.nume, 0.0);
+        for (int j = 0; j < sp.nume; ++j)
+            std::copy(z.begin() + (size_t)j * ns, z.begin() + (size_t)(j + 1) * ns, iv.begin() + (size_t)j * h.n);
+        init = iv.data();
+        niv = sp.nume;
+    }
+    DavidsonEngine eng(1, h.stream);
+    std::vector<int> slots = eng.admit({sp}, {init}, {niv});
+    run_to_completion(eng, 2 * sp.maxiter + 128);
+    eng.extract(slots[0], eval, evec, resid, st);
+}
+
+}  // namespace
+
+}  // namespace gci
+
+using namespace gci;
+
+extern "C" {
+
+int gci_davidson(gci_hmat *h, const gci_dvd_params *p, double *eval, double *evec, double *resid,
+                 gci_dvd_stats *st) {
+    return guarded([&] {
+        GCI_REQUIRE(h && p && eval, "gci_davidson: null argument");
+        const auto t0 = std::chrono::steady_clock::now();
+        DvdSpec sp = resolve(*h, p);
+        if (sp.mode != GCI_OP_FP64) GCI_REQUIRE(h->precision == GCI_SPLIT32, "gci_davidson: split mode needs GCI_SPLIT32");
+        gci_dvd_stats local{};
+        solve_one(*h, sp, p->niv > 0 ? evec : nullptr, p->niv, eval, evec, resid, &local);
+        local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (st) *st = local;
+        if (local.ierr & 2048) throw Error{GCI_ERR_NOCONV, "gci_davidson: NLOOPS > MAXITER (IERR 2048)"};
+    });
+}
+
+int gci_maneig(gci_hmat *h, int nvec, double *eval, double *evec, gci_dvd_stats *st) {
+    return guarded([&] {
+        GCI_REQUIRE(h && eval && evec, "gci_maneig: null argument");
+        GCI_REQUIRE(nvec >= 1 && nvec <= h->n, "gci_maneig: nvec out of range");
+        const auto t0 = std::chrono::steady_clock::now();
+        gci_dvd_stats local{};
+        if (h->n == 1) {
+            // trivial case (maneig.f90:136-156)
+            GCI_HIP(hipMemcpy(eval, h->diag.p, sizeof(double), hipMemcpyDeviceToHost));
+            evec[0] = 1.0;
+        } else if (h->n <= 2000) {
+            // LAPACK branch (maneig.f90:157-242)
+            std::vector<double> w, z;
+            lowest_of_leading_block(*h, h->n, nvec, w, z);
+            std::copy(w.begin(), w.end(), eval);
+            std::copy(z.begin(), z.end(), evec);
+        } else {
+            // DVDSON branch, in-core sparse (maneig.f90:302-451)
+            gci_dvd_params p{};
+            p.nvec = nvec;
+            p.mode = GCI_OP_FP64;
+            DvdSpec sp = resolve(*h, &p);
+            solve_one(*h, sp, nullptr, 0, eval, evec, nullptr, &local);
+            if (local.ierr & 2048) throw Error{GCI_ERR_NOCONV, "maneig: DVDSON wrong (IERR 2048)"};
+        }
+        sign_fix(h->n, nvec, evec);
+        local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (st) *st = local;
+    });
+}
+
+// Batched form: every block's Davidson runs interleaved in one engine
+// (split-precision bulk operator when the block holds hi/lo, block
+// Davidson with MBLOCK = NEV, convergence on the relative residual
+// ||r|| < tol |lambda + EAV|, then an FP64 refinement pass).
+int gci_batch_solve(int nblocks, gci_hmat **blocks, const int *nev, const double *tol, const double *eav,
+                    double *eval, int64_t *eval_off, double *evec, int64_t *evec_off, gci_dvd_stats *stats) {
+    return guarded([&] {
+        GCI_REQUIRE(nblocks >= 1 && blocks && nev && tol && eval && eval_off, "gci_batch_solve: null argument");
+        const auto t0 = std::chrono::steady_clock::now();
+        hipStream_t s = blocks[0]->stream;
+        DavidsonEngine eng(nblocks, s);
+        std::vector<DvdSpec> specs(nblocks);
+        std::vector<const double *> init(nblocks, nullptr);
+        std::vector<int> niv(nblocks);
+        int guard = 0;
+        for (int b = 0; b < nblocks; ++b) {
+            GCI_REQUIRE(blocks[b] && blocks[b]->n >= 2, "gci_batch_solve: block order < 2");
+            DvdSpec &sp = specs[b];
+            sp.h = blocks[b];
+            sp.nume = nev[b];
+            sp.mblock = nev[b];
+            sp.lim = (int)std::min<int64_t>(blocks[b]->n, 2 * nev[b] + 60);
+            sp.maxiter = (int)std::max<int64_t>(200 * nev[b], blocks[b]->n / 10);
+            sp.crite = 0.0;  // convergence on the residual alone
+            sp.critc = 0.0;
+            const bool split = blocks[b]->precision == GCI_SPLIT32;
+            sp.mode = split ? GCI_OP_SPLIT_BULK : GCI_OP_FP64;
+            sp.refine = split;
+            sp.tol_rel = split ? std::max(0.5 * tol[b], 2.0e-10) : tol[b];
+            sp.tol_final = tol[b];
+            sp.critr = 0.0;
+            sp.eshift = eav ? eav[b] : 0.0;
+            niv[b] = nev[b];
+            guard = std::max(guard, 2 * sp.maxiter + 128);
+        }
+        eng.admit(specs, init, niv);
+        run_to_completion(eng, guard);
+        int64_t eo = 0, vo = 0;
+        for (int b = 0; b < nblocks; ++b) {
+            eval_off[b] = eo;
+            if (evec_off) evec_off[b] = vo;
+            gci_dvd_stats st{};
+            eng.extract(b, eval + eo, evec ? evec + vo : nullptr, nullptr, &st);
+            st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (stats) stats[b] = st;
+            eo += nev[b];
+            vo += blocks[b]->n * nev[b];
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,289 @@
+"""Host-side mirror of the rci90 hot-path interface, over libgraspci.
+
+Names and argument meanings follow the GRASP2018 routines each call
+replaces; errors raise ``GciError`` where the Fortran would STOP.
+
+* ``HamiltonianBlock`` -- the hmat_C module state (EMT, IROW, IENDC,
+  lib/libmod/hmat_C.f90:6-9) as a device-resident handle, loaded the way
+  MANEIG loads it (rci90/maneig.f90:312-332: diagonal shifted by -EAV).
+* ``spicmv2`` -- the GDVD operator OP(N, M, B, C) (lib9290/spicmv2.f90:2).
+* ``gdvd`` / ``maneig`` -- Davidson and the MANEIG dispatch.
+* ``genintrk`` / ``one_body`` / ``setham`` -- R^k table, I(ab), assembly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, dp, i32p, i64p
+
+
+class HamiltonianBlock:
+    """Device-resident block Hamiltonian (lower rows, diagonal last)."""
+
+    def __init__(self, handle, n: int, nnz: int, precision: int):
+        self._h = handle
+        self.n = n
+        self.nnz_offdiag = nnz
+        self.precision = precision
+
+    @classmethod
+    def from_rows(cls, iendc, irow, emt, shift: float = 0.0, precision: int = L.GCI_FP64,
+                  index_base: int = 1):
+        iendc = np.ascontiguousarray(iendc, dtype=np.int64)
+        irow = np.ascontiguousarray(irow, dtype=np.int32)
+        emt = np.ascontiguousarray(emt, dtype=np.float64)
+        n = len(iendc) - 1
+        h = C.c_void_p()
+        check(L.lib().gci_hmat_upload(C.c_int(n), i64p(iendc), i32p(irow), dp(emt), C.c_int(index_base),
+                                      C.c_double(shift), C.c_int(precision), C.byref(h)))
+        nn, nz, pr = C.c_int64(), C.c_int64(), C.c_int()
+        check(L.lib().gci_hmat_info(h, C.byref(nn), C.byref(nz), C.byref(pr)))
+        return cls(h, nn.value, nz.value, pr.value)
+
+    @classmethod
+    def from_csr(cls, csr, shift: float = 0.0, precision: int = L.GCI_FP64):
+        """From a workloads.LowerCSR (1-based columns, diagonal last)."""
+        return cls.from_rows(csr.row_end, csr.col, csr.val, shift, precision, 1)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def free(self):
+        if self._h:
+            check(L.lib().gci_hmat_free(self._h))
+            self._h = None
+
+    def release(self):
+        """Give up ownership (the server frees it)."""
+        h, self._h = self._h, None
+        return h
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def download(self, index_base: int = 1):
+        n = self.n
+        iendc = np.zeros(n + 1, dtype=np.int64)
+        irow = np.zeros(self.nnz_offdiag + n, dtype=np.int32)
+        emt = np.zeros(self.nnz_offdiag + n)
+        check(L.lib().gci_hmat_download(self._h, i64p(iendc), i32p(irow), dp(emt), C.c_int(index_base)))
+        return iendc, irow, emt
+
+    def spicmv2(self, B, mode: int = L.GCI_OP_FP64):
+        """C := H B (SPICMV2 semantics), B column-major n x m (or a vector)."""
+        B = np.asarray(B, dtype=np.float64)
+        vec = B.ndim == 1
+        Bf = np.asfortranarray(B.reshape(self.n, -1) if vec else B)
+        m = Bf.shape[1]
+        Cf = np.zeros((self.n, m), order="F")
+        check(L.lib().gci_spmm(self._h, C.c_int(m), dp(Bf), dp(Cf), C.c_int(mode)))
+        return Cf[:, 0] if vec else Cf
+
+
+def spicmv2(block: HamiltonianBlock, n: int, m: int, B, C_out=None, mode: int = L.GCI_OP_FP64):
+    """SPICMV2(N, M, B, C) with the block standing in for module hmat_C."""
+    if n != block.n:
+        raise ValueError("spicmv2: N does not match the loaded matrix")
+    res = block.spicmv2(np.asarray(B).reshape(n, m, order="F"), mode)
+    if C_out is not None:
+        C_out[...] = res.reshape(C_out.shape, order="F")
+        return C_out
+    return res
+
+
+def gdvd(block: HamiltonianBlock, nvec: int, lim: int = 0, mblock: int = 1, crite: float = 1.0e-17,
+         critc: float = 1.0e-9, critr: float = 1.0e-9, maxiter: int = 0, init=None,
+         mode: int = L.GCI_OP_FP64, refine: bool = True):
+    """GDVD/DVDSON (libdvd90/gdvd.f90) for the lowest nvec eigenpairs of the
+    block as held.  init: n x niv initial estimates (None = INIEST2).
+    Returns (eval, evec, resid, stats dict)."""
+    n = block.n
+    evec = np.zeros((n, nvec), order="F")
+    niv = 0
+    if init is not None:
+        init = np.asfortranarray(init, dtype=np.float64)
+        niv = init.shape[1]
+        if niv > nvec:
+            raise ValueError("gdvd: more initial vectors than nvec")
+        evec[:, :niv] = init
+    p = L.GciDvdParams(nvec, lim, mblock, maxiter, crite, critc, critr, niv, mode, int(refine))
+    ev = np.zeros(nvec)
+    res = np.zeros(nvec)
+    st = L.GciDvdStats()
+    check(L.lib().gci_davidson(block.handle, C.byref(p), dp(ev), dp(evec), dp(res), C.byref(st)))
+    return ev, evec, res, {"nloops": st.nloops, "nmv": st.nmv, "ierr": st.ierr,
+                           "refine_loops": st.refine_loops, "seconds": st.seconds}
+
+
+def maneig(block: HamiltonianBlock, nvec: int):
+    """MANEIG for one block (rci90/maneig.f90): dense for NCF <= 2000, else
+    INIEST2 + GDVD(SPICMV2); eigenvectors sign-fixed."""
+    n = block.n
+    ev = np.zeros(nvec)
+    evec = np.zeros((n, nvec), order="F")
+    st = L.GciDvdStats()
+    check(L.lib().gci_maneig(block.handle, C.c_int(nvec), dp(ev), dp(evec), C.byref(st)))
+    return ev, evec, {"nloops": st.nloops, "nmv": st.nmv, "ierr": st.ierr, "seconds": st.seconds}
+
+
+def _grid_struct(r, rp, rpor, n, h):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    rp = np.ascontiguousarray(rp, dtype=np.float64)
+    rpor = np.ascontiguousarray(rpor, dtype=np.float64)
+    g = L.GciGrid(n, h, h ** 6, dp(r), dp(rp), dp(rpor))
+    return g, (r, rp, rpor)
+
+
+def _orb_struct(P, Q, mf, nak, nkl, nkj):
+    arrs = (np.ascontiguousarray(P, dtype=np.float64), np.ascontiguousarray(Q, dtype=np.float64),
+            np.ascontiguousarray(mf, dtype=np.int32), np.ascontiguousarray(nak, dtype=np.int32),
+            np.ascontiguousarray(nkl, dtype=np.int32), np.ascontiguousarray(nkj, dtype=np.int32))
+    nw, ld = arrs[0].shape
+    o = L.GciOrbitals(nw, ld, dp(arrs[0]), dp(arrs[1]), i32p(arrs[2]), i32p(arrs[3]), i32p(arrs[4]),
+                      i32p(arrs[5]))
+    return o, arrs
+
+
+def radgrd(n: int = 590, rnt: float = 2.0e-6, h: float = 0.05):
+    """RADGRD (lib9290/radgrd.f90:41-60): r, rp, rpor on n+10 points."""
+    r, rp, rpor = np.zeros(n + 10), np.zeros(n + 10), np.zeros(n + 10)
+    check(L.lib().gci_radgrd(C.c_int(n), C.c_double(rnt), C.c_double(h), dp(r), dp(rp), dp(rpor)))
+    return r, rp, rpor
+
+
+def genintrk(r, rp, rpor, n, h, orbitals):
+    """GENINTRK (rci90/genintrk.f90): returns (kstart, keys, vals) with the
+    reference's enumeration, packing and KSTART; values computed on the GPU."""
+    g, keep_g = _grid_struct(r, rp, rpor, n, h)
+    o, keep_o = _orb_struct(orbitals.P, orbitals.Q, orbitals.mf, orbitals.nak, orbitals.nkl, orbitals.nkj)
+    j2 = C.c_int32(0)
+    cnt = C.c_int64(0)
+    kst = np.zeros(22, dtype=np.int32)
+    check(L.lib().gci_rk_table(C.byref(g), C.byref(o), C.byref(j2), i32p(kst), None, None, C.c_int64(0),
+                               C.byref(cnt)))
+    nk = cnt.value
+    keys = np.zeros(max(nk, 1), dtype=np.int32)
+    vals = np.zeros(max(nk, 1))
+    check(L.lib().gci_rk_table(C.byref(g), C.byref(o), C.byref(j2), i32p(kst), i32p(keys), dp(vals),
+                               C.c_int64(nk), C.byref(cnt)))
+    return kst[: j2.value + 2].copy(), keys[:nk], vals[:nk]
+
+
+def one_body(r, rp, rpor, n, h, orbitals, zz, c):
+    """I(a,b) = RINTI(a,b,0) for same-kappa pairs a <= b (nw x nw, upper)."""
+    g, keep_g = _grid_struct(r, rp, rpor, n, h)
+    o, keep_o = _orb_struct(orbitals.P, orbitals.Q, orbitals.mf, orbitals.nak, orbitals.nkl, orbitals.nkj)
+    nw = keep_o[0].shape[0]
+    iab = np.zeros((nw, nw))
+    zz = np.ascontiguousarray(zz, dtype=np.float64)
+    check(L.lib().gci_one_body(C.byref(g), C.byref(o), dp(zz), C.c_double(c), dp(iab)))
+    return iab
+
+
+def setham(ncf, nw, lists, iab, kstart, keys, vals, lfordr=False, iccut=0, precision=L.GCI_FP64):
+    """SETHAM element loop on the GPU: returns (HamiltonianBlock with the
+    diagonal shifted by -EAV, EAV, NELMNT)."""
+    arrs = {
+        "pair_ic": np.ascontiguousarray(lists["pair_ic"], dtype=np.int32),
+        "pair_ir": np.ascontiguousarray(lists["pair_ir"], dtype=np.int32),
+        "one_ia": np.ascontiguousarray(lists["one_ia"], dtype=np.int32),
+        "one_ib": np.ascontiguousarray(lists["one_ib"], dtype=np.int32),
+        "tsh_off": np.ascontiguousarray(lists["tsh_off"], dtype=np.int64),
+        "tshell": np.ascontiguousarray(lists["tshell"], dtype=np.float64),
+        "tb_beg": np.ascontiguousarray(lists["tb_beg"], dtype=np.int64),
+        "tb_end": np.ascontiguousarray(lists["tb_end"], dtype=np.int64),
+        "label": np.ascontiguousarray(lists["label"], dtype=np.int32),
+        "coeff": np.ascontiguousarray(lists["coeff"], dtype=np.float64),
+    }
+    pl = L.GciPairLists(len(arrs["pair_ic"]), i32p(arrs["pair_ic"]), i32p(arrs["pair_ir"]), i32p(arrs["one_ia"]),
+                        i32p(arrs["one_ib"]), i64p(arrs["tsh_off"]), dp(arrs["tshell"]), len(arrs["tshell"]),
+                        i64p(arrs["tb_beg"]), i64p(arrs["tb_end"]), i32p(arrs["label"]), dp(arrs["coeff"]),
+                        len(arrs["coeff"]))
+    iab = np.ascontiguousarray(iab, dtype=np.float64)
+    kstart = np.ascontiguousarray(kstart, dtype=np.int32)
+    keys = np.ascontiguousarray(keys, dtype=np.int32)
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    h = C.c_void_p()
+    eav = C.c_double(0.0)
+    nel = C.c_int64(0)
+    check(L.lib().gci_h_assemble(C.c_int(ncf), C.c_int(nw), C.byref(pl), dp(iab), i32p(kstart), i32p(keys),
+                                 dp(vals), C.c_int64(len(keys)), C.c_int(int(lfordr)), C.c_int(iccut),
+                                 C.c_int(precision), C.byref(h), C.byref(eav), C.byref(nel)))
+    nn, nz, pr = C.c_int64(), C.c_int64(), C.c_int()
+    check(L.lib().gci_hmat_info(h, C.byref(nn), C.byref(nz), C.byref(pr)))
+    return HamiltonianBlock(h, nn.value, nz.value, pr.value), eav.value, nel.value
+
+
+def batch_solve(blocks, nev, tol, eav=None):
+    """Batched form: all blocks' Davidson iterations interleaved on the
+    device.  blocks: list of HamiltonianBlock; nev/tol/eav: per block.
+    Returns (list of eigenvalue arrays, list of eigenvector arrays, stats)."""
+    nb = len(blocks)
+    nev = np.ascontiguousarray(np.broadcast_to(np.asarray(nev, dtype=np.int32), (nb,)))
+    tol = np.ascontiguousarray(np.broadcast_to(np.asarray(tol, dtype=np.float64), (nb,)))
+    eav = np.zeros(nb) if eav is None else np.ascontiguousarray(eav, dtype=np.float64)
+    handles = (C.c_void_p * nb)(*[b.handle for b in blocks])
+    ev = np.zeros(int(nev.sum()))
+    vec = np.zeros(int(sum(b.n * int(k) for b, k in zip(blocks, nev))))
+    eo = np.zeros(nb, dtype=np.int64)
+    vo = np.zeros(nb, dtype=np.int64)
+    st = (L.GciDvdStats * nb)()
+    check(L.lib().gci_batch_solve(C.c_int(nb), handles, i32p(nev), dp(tol), dp(eav), dp(ev), i64p(eo), dp(vec),
+                                  i64p(vo), st))
+    evals = [ev[eo[b]: eo[b] + nev[b]].copy() for b in range(nb)]
+    evecs = [vec[vo[b]: vo[b] + blocks[b].n * nev[b]].reshape(nev[b], blocks[b].n).T.copy() for b in range(nb)]
+    stats = [{"nloops": s.nloops, "nmv": s.nmv, "ierr": s.ierr, "refine_loops": s.refine_loops,
+              "seconds": s.seconds} for s in st]
+    return evals, evecs, stats
+
+
+class Server:
+    """Continuous-batching CI solve server (gci_server_*)."""
+
+    def __init__(self, hbm_budget_bytes: int, max_active: int = 64):
+        self._s = C.c_void_p()
+        check(L.lib().gci_server_create(C.c_size_t(int(hbm_budget_bytes)), C.c_int(max_active),
+                                        C.byref(self._s)))
+        self._meta = {}
+
+    def submit(self, block: HamiltonianBlock, nev: int, tol: float, eav: float = 0.0) -> int:
+        req = C.c_int64(0)
+        check(L.lib().gci_server_submit(self._s, block.handle, C.c_int(nev), C.c_double(tol), C.c_double(eav),
+                                        C.byref(req)))
+        self._meta[req.value] = (block.n, nev)
+        block.release()  # the server owns the matrix now
+        return req.value
+
+    def poll(self, req: int):
+        """(eval, evec, latency_s) when done, None while running."""
+        n, nev = self._meta[req]
+        ev = np.zeros(nev)
+        vec = np.zeros(n * nev)
+        lat = C.c_double(0.0)
+        code = L.lib().gci_server_poll(self._s, C.c_int64(req), dp(ev), dp(vec), C.byref(lat))
+        if code == L.GCI_ERR_BUSY:
+            return None
+        check(code)
+        del self._meta[req]
+        return ev, vec.reshape(nev, n).T.copy(), lat.value
+
+    def drain(self):
+        check(L.lib().gci_server_drain(self._s))
+
+    def close(self):
+        if self._s:
+            check(L.lib().gci_server_destroy(self._s))
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -204,13 +204,15 @@ class LowerCSR:
 
 def clustered_block(n: int, seed: int, fill: float = 0.05, diag_mean: float = -338.0,
                     diag_sd: float = 5.0, off_sd: float = 1.0e-2, group: tuple = (4, 48),
-                    inner: float = 0.6) -> LowerCSR:
+                    inner: float = 0.6, sorted_diag: bool = False) -> LowerCSR:
     """Seeded symmetric block: diagonal ~ N(diag_mean, diag_sd^2), off-diagonal
     entries ~ N(0, off_sd^2) clustered in CSF-group sub-blocks.  Rows are cut
     into groups of uniform random size in [group[0], group[1]]; a lower group
     pair (G, H) is coupled with probability fill/inner and, when coupled,
     each entry is present with probability inner, so the expected lower fill
-    is `fill`."""
+    is `fill`.  sorted_diag puts the diagonal in ascending order, as a CSF
+    list does when it starts from the reference configuration (so the
+    leading-block initial estimates of INIEST2 see the lowest states)."""
     rng = np.random.default_rng(seed)
     bounds = [0]
     while bounds[-1] < n:
@@ -242,6 +244,8 @@ def clustered_block(n: int, seed: int, fill: float = 0.05, diag_mean: float = -3
     order = np.lexsort((cols, rows))
     rows, cols, vals = rows[order], cols[order], vals[order]
     diag = rng.normal(diag_mean, diag_sd, size=n)
+    if sorted_diag:
+        diag = np.sort(diag)
     counts = np.bincount(rows, minlength=n) + 1
     row_end = np.zeros(n + 1, dtype=np.int64)
     row_end[1:] = np.cumsum(counts)
@@ -264,3 +268,73 @@ def batch_sizes(nblocks: int, seed: int, lo: float = 1.0e2, hi: float = 5.0e4):
     """Block sizes log-uniform in [lo, hi] (seeded)."""
     rng = np.random.default_rng(seed)
     return np.round(np.exp(rng.uniform(math.log(lo), math.log(hi), size=nblocks))).astype(np.int64)
+
+
+# ------------------------------------------------------ angular lists
+
+
+def synthetic_lists(csr_pattern: LowerCSR, nw: int, nkl, rk_labels, rk_k, seed: int,
+                    max_terms: int = 6, small_frac: float = 0.1):
+    """Angular-coefficient lists in the layout the CPU producer emits per CSF
+    pair (ONESCALAR's IA, IB, TSHELL; RKCO_GG's LABEL(1:5,i), COEFF(i)),
+    on the sparsity pattern of csr_pattern (pairs = its stored elements).
+
+    Two-body labels are drawn from the R^k table and permuted by the
+    symmetries RKINTC canonicalises ((a,b,c,d) -> (c,b,a,d), (a,d,c,b),
+    (c,d,a,b)); a fraction of coefficients is put below the 1e-12 CUTOFF.
+    Diagonal pairs carry IA = IB = 1 with NW TSHELL values (occupations);
+    off-diagonal pairs carry a same-kappa IA != IB pair or IA = 0."""
+    rng = np.random.default_rng(seed)
+    n = csr_pattern.n
+    rows = np.repeat(np.arange(1, n + 1), np.diff(csr_pattern.row_end)).astype(np.int32)
+    cols = csr_pattern.col.astype(np.int32)
+    npairs = len(rows)
+    nkl = np.asarray(nkl)
+    # same-l pairs a < b (kappa equal is enough; l equal is a superset check)
+    same = [(a, b) for a in range(1, nw + 1) for b in range(a + 1, nw + 1) if nkl[a - 1] == nkl[b - 1]]
+    one_ia = np.zeros(npairs, dtype=np.int32)
+    one_ib = np.zeros(npairs, dtype=np.int32)
+    tsh_off = np.zeros(npairs, dtype=np.int64)
+    tsh = []
+    off = 0
+    diag = rows == cols
+    choose = rng.random(npairs)
+    for p in range(npairs):
+        if diag[p]:
+            one_ia[p] = one_ib[p] = 1
+            occ = rng.integers(0, 3, size=nw).astype(np.float64)
+            occ[rng.random(nw) < 0.1] = 1e-13
+            tsh.append(occ)
+            tsh_off[p] = off
+            off += nw
+        elif same and choose[p] < 0.3:
+            a, b = same[int(rng.integers(len(same)))]
+            if rng.random() < 0.5:
+                a, b = b, a
+            one_ia[p], one_ib[p] = a, b
+            tsh.append(np.array([rng.normal(0, 0.3)]))
+            tsh_off[p] = off
+            off += 1
+    tshell = np.concatenate(tsh) if tsh else np.zeros(1)
+    nt = rng.integers(1, max_terms + 1, size=npairs)
+    tb_end = np.cumsum(nt).astype(np.int64)
+    tb_beg = (tb_end - nt).astype(np.int64)
+    nterms = int(tb_end[-1])
+    pick = rng.integers(0, len(rk_k), size=nterms)
+    lab = np.asarray(rk_labels)[pick].copy()
+    kk = np.asarray(rk_k)[pick]
+    sym = rng.integers(0, 4, size=nterms)
+    a, b, c, d = lab[:, 0].copy(), lab[:, 1].copy(), lab[:, 2].copy(), lab[:, 3].copy()
+    m1 = sym == 1
+    lab[m1, 0], lab[m1, 2] = c[m1], a[m1]
+    m2 = sym == 2
+    lab[m2, 1], lab[m2, 3] = d[m2], b[m2]
+    m3 = sym == 3
+    lab[m3, 0], lab[m3, 1], lab[m3, 2], lab[m3, 3] = c[m3], d[m3], a[m3], b[m3]
+    label = np.concatenate([lab, kk[:, None]], axis=1).astype(np.int32).ravel()
+    coeff = rng.normal(0.0, 0.1, size=nterms)
+    coeff[rng.random(nterms) < small_frac] = 1e-13
+    return {
+        "pair_ic": rows, "pair_ir": cols, "one_ia": one_ia, "one_ib": one_ib, "tsh_off": tsh_off,
+        "tshell": tshell, "tb_beg": tb_beg, "tb_end": tb_end, "label": label, "coeff": coeff,
+    }

@@ -1,0 +1,212 @@
+/*
+ * graspci.h -- C ABI of the MI355X rci90 engine (libgraspci.so).
+ *
+ * Plain C types only: pointers, sizes, int status codes (0 = ok).  Every
+ * entry point can be bound from Fortran through ISO_C_BINDING (see
+ * grasp_amd/fortran/gci_bind.f90) and from Python through ctypes.  Unless a
+ * name ends in _device, array arguments are host memory.
+ *
+ * Index conventions follow the Fortran callers: orbital labels and matrix
+ * column indices are 1-based unless an index_base argument says otherwise;
+ * row pointers are int64 cumulative row ends (IENDC with IENDC(0) = 0).
+ */
+#ifndef GRASPCI_H
+#define GRASPCI_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define GCI_OK 0
+#define GCI_ERR_ARG -1        /* invalid argument or malformed input       */
+#define GCI_ERR_DEVICE -2     /* HIP runtime / device failure               */
+#define GCI_ERR_NOMEM -3      /* device allocation failed or budget exceeded */
+#define GCI_ERR_NOTFOUND -4   /* R^k integral missing (RKINTC would STOP)    */
+#define GCI_ERR_NOCONV -5     /* Davidson hit MAXITER (DVDSON IERR 2048)     */
+#define GCI_ERR_BUSY -6       /* serving: request not finished               */
+
+typedef struct gci_hmat gci_hmat;       /* device-resident block Hamiltonian */
+typedef struct gci_server gci_server;   /* continuous-batching solver       */
+
+/* Last error message of the calling thread. */
+const char *gci_last_error(void);
+
+/* Select the device (hipSetDevice) and warm up the runtime. */
+int gci_init(int device);
+
+/* ------------------------------------------------------------------ grid
+ * Radial grid r_j = RNT (exp((j-1) H) - 1), RP = dr/dt, RPOR = RP / r,
+ * n + 10 points (lib9290/radgrd.f90:41-60; rci90/getcid.f90:233-235). */
+int gci_radgrd(int n, double rnt, double h, double *r, double *rp, double *rpor);
+
+typedef struct {
+    int n;              /* grid points N (590)                         */
+    double h;           /* step H                                       */
+    double accy;        /* H**6 (rci90/getcid.f90:262)                  */
+    const double *r;    /* n + 10 */
+    const double *rp;   /* n + 10 */
+    const double *rpor; /* n + 10 */
+} gci_grid;
+
+typedef struct {
+    int nw;             /* number of orbitals NW                        */
+    int ld;             /* leading dimension of P, Q (>= n)             */
+    const double *P;    /* nw x ld, orbital-major: P[j*ld + i] = PF(i+1,j+1) */
+    const double *Q;
+    const int32_t *mf;  /* MF: last tabulated point, 1-based            */
+    const int32_t *nak; /* kappa                                        */
+    const int32_t *nkl; /* l                                            */
+    const int32_t *nkj; /* 2j                                           */
+} gci_orbitals;
+
+/* --------------------------------------------- piece 1: R^k tables
+ * Replaces GENINTRK (rci90/genintrk.f90:3-109) and its SLATER/YZK/QUAD calls
+ * (lib9290/slater.f90, yzk.f90, quad.f90).  Same enumeration order, key
+ * packing ((ia*KEY+ib)*KEY+ic)*KEY+id with KEY = NW+1, and KSTART(0..j2max+1)
+ * (kstart gets j2max+2 entries, 1-based positions).  Pass keys = vals = NULL
+ * to get the count only. */
+int gci_rk_table(const gci_grid *grid, const gci_orbitals *orb, int32_t *j2max,
+                 int32_t *kstart, int32_t *keys, double *vals, int64_t cap, int64_t *n);
+
+/* One-body Dirac integrals I(a,b) = RINTI(a,b,0) (lib9290/rinti.f90:49-103)
+ * for every same-kappa pair a <= b, as IABINT memoises them
+ * (rci90/iabint.f90).  zz: -r V_nuc(r), n entries (lib9290/nucpot.f90).
+ * iab: nw x nw, iab[(a-1)*nw + (b-1)] for a <= b; other entries zero. */
+int gci_one_body(const gci_grid *grid, const gci_orbitals *orb, const double *zz, double c,
+                 double *iab);
+
+/* --------------------------------------------- piece 2: H assembly
+ * Angular lists exactly as the CPU producer emits them per CSF pair (ic, ir),
+ * ir <= ic, pairs sorted by row then column (1-based):
+ *   one-body  (ONESCALAR, librang90/onescalar.f90): IA, IB and TSHELL values
+ *             at tsh_off (NW values if IA == IB, one if IA != IB, none if 0);
+ *   two-body  (RKCO_GG/SPEAK, lib9290/speak.f90:46-59): entries
+ *             [tb_beg, tb_end) of LABEL (a,b,c,d,k per entry) and COEFF.
+ * Every row must carry its diagonal pair. */
+typedef struct {
+    int64_t npairs;
+    const int32_t *pair_ic;
+    const int32_t *pair_ir;
+    const int32_t *one_ia;
+    const int32_t *one_ib;
+    const int64_t *tsh_off;
+    const double *tshell;
+    int64_t ntshell;
+    const int64_t *tb_beg;
+    const int64_t *tb_end;
+    const int32_t *label;   /* 5 per two-body entry */
+    const double *coeff;
+    int64_t nterms;
+} gci_pair_lists;
+
+/* Replaces the SETHAM element loop (rci90/setham_gg.f90:162-385) for the
+ * Dirac-Coulomb operator: per pair, ELEMNT = sum of T*I(a,b) over one-body
+ * coefficients with |T| > 1e-12 plus V*R^k over two-body coefficients with
+ * |V| > 1e-12 (R^k looked up as RKINTC does, rci90/rkintc.f90), summed in
+ * list order; kept if diagonal or |ELEMNT| > 1e-12 (setham_gg.f90:364).
+ * LFORDR/ICCUT: off-diagonal pairs with ir > iccut are skipped
+ * (setham_gg.f90:172-174).  Produces the block Hamiltonian on the device
+ * (diagonal shifted by -EAV as MANEIG loads it, maneig.f90:326) and returns
+ * EAV = sum of diagonals / NCF (rci90/genmat2.f90:73) and the number of
+ * stored lower-triangle elements (NELMNT). */
+int gci_h_assemble(int ncf, int nw, const gci_pair_lists *lists, const double *iab,
+                   const int32_t *kstart, const int32_t *keys, const double *vals,
+                   int64_t nkeys, int lfordr, int iccut, int precision, gci_hmat **out,
+                   double *eav, int64_t *nelmnt);
+
+/* --------------------------------------------- block Hamiltonian handle
+ * .res-row ingestion (rci90/setham_gg.f90:378 layout): lower-triangle rows,
+ * diagonal last, cumulative row ends iendc[0..n] (iendc[0] = 0), column
+ * indices irow with the given index_base (1 for Fortran IROW).  The diagonal
+ * is shifted by `shift` (MANEIG passes -EAV, maneig.f90:326).
+ * precision: GCI_FP64 keeps FP64 off-diagonals (plain drop-in);
+ * GCI_SPLIT32 also keeps hi = fp32(v) and lo = fp32(v - hi) for the mixed
+ * form.  The diagonal is always FP64. */
+#define GCI_FP64 0
+#define GCI_SPLIT32 1
+int gci_hmat_upload(int n, const int64_t *iendc, const int32_t *irow, const double *emt,
+                    int index_base, double shift, int precision, gci_hmat **out);
+int gci_hmat_free(gci_hmat *h);
+int gci_hmat_info(const gci_hmat *h, int64_t *n, int64_t *nnz_offdiag, int *precision);
+/* Download as lower rows with diagonal last (inverse of the upload). */
+int gci_hmat_download(const gci_hmat *h, int64_t *iendc, int32_t *irow, double *emt,
+                      int index_base);
+
+/* --------------------------------------------- piece 3: SPICMV2
+ * C := H B for the symmetric H held as lower rows (lib9290/spicmv2.f90:2-60,
+ * dmerge.f90:36-40): B, C column-major n x m, C overwritten.  This is the
+ * GDVD operator contract OP(N, M, B, C) (libdvd90/gdvd.f90:139).
+ * mode: GCI_OP_FP64 (FP64 values), GCI_OP_SPLIT_BULK (fp32 hi values and
+ * fp32-rounded B, FP64 accumulation, FP64 diagonal), GCI_OP_SPLIT_FULL
+ * (hi + lo, FP64 B). */
+#define GCI_OP_FP64 0
+#define GCI_OP_SPLIT_BULK 1
+#define GCI_OP_SPLIT_FULL 2
+int gci_spmm(gci_hmat *h, int m, const double *B, double *C, int mode);
+/* Same on device pointers (leading dimension ld >= n), enqueued on the
+ * handle's stream; no synchronisation. */
+int gci_spmm_device(gci_hmat *h, int m, const double *dB, int ldb, double *dC, int ldc, int mode);
+
+/* --------------------------------------------- piece 4: Davidson
+ * GDVD/DVDSON contract (libdvd90/gdvd.f90:4-147, dvdson.f90): lowest nvec
+ * eigenpairs of H (as held, i.e. already shifted), LIM, CRITE, CRITC, CRITR,
+ * MAXITER, MBLOCK as in MANEIG (maneig.f90:389-420).  Gram-Schmidt, the
+ * Rayleigh-Ritz projection and basis rotations run on the device. */
+typedef struct {
+    int nvec;           /* IHIGH (ILOW = 1)                                 */
+    int lim;            /* LIM; 0 = min(n, 2*nvec+60)                       */
+    int mblock;         /* MBLOCK; 0 = 1 (in-core rci)                       */
+    int maxiter;        /* MAXITER; 0 = max(200*nvec, n/10)                  */
+    double crite;       /* 0 = 1e-17                                         */
+    double critc;       /* 0 = 1e-9                                          */
+    double critr;       /* 0 = 1e-9                                          */
+    int niv;            /* initial vectors supplied in evec (0 = INIEST2)     */
+    int mode;           /* operator mode (GCI_OP_*)                          */
+    int refine;         /* split forms: FP64 refinement passes at the end    */
+} gci_dvd_params;
+
+typedef struct {
+    int nloops;         /* NLOOPS */
+    int nmv;            /* NMV: matrix-vector products (columns)             */
+    int ierr;           /* DVDSON IERR bitmask                               */
+    int refine_loops;   /* FP64 iterations after the split-precision phase   */
+    double seconds;     /* wall time of the solve                            */
+} gci_dvd_stats;
+
+/* eval[nvec] (eigenvalues of H as held), evec n x nvec column-major,
+ * resid[nvec] residual norms (may be NULL). */
+int gci_davidson(gci_hmat *h, const gci_dvd_params *p, double *eval, double *evec,
+                 double *resid, gci_dvd_stats *st);
+
+/* MANEIG dispatch for one block (rci90/maneig.f90:130-555): NCF == 1
+ * trivial, NCF <= 2000 dense eigensolve, else INIEST2 + GDVD(SPICMV2);
+ * eigenvectors sign-fixed so the largest component is positive
+ * (maneig.f90:532-555). */
+int gci_maneig(gci_hmat *h, int nvec, double *eval, double *evec, gci_dvd_stats *st);
+
+/* --------------------------------------------- batched + serving
+ * Many blocks at once: each block's Davidson iterations are interleaved on
+ * the device (split-precision bulk operator, FP64 refinement).  tol[b] is
+ * the relative eigenvalue tolerance of block b; eval_total adds back
+ * shift[b] (EAV) so errors are relative to total energies. */
+int gci_batch_solve(int nblocks, gci_hmat **blocks, const int *nev, const double *tol,
+                    const double *eav, double *eval, int64_t *eval_off, double *evec,
+                    int64_t *evec_off, gci_dvd_stats *stats);
+
+int gci_server_create(size_t hbm_budget_bytes, int max_active, gci_server **out);
+int gci_server_destroy(gci_server *s);
+/* Submit one CI solve; the server takes ownership of h.  Returns a request
+ * id in *req. */
+int gci_server_submit(gci_server *s, gci_hmat *h, int nev, double tol, double eav, int64_t *req);
+/* Non-blocking poll: GCI_OK with results when done, GCI_ERR_BUSY otherwise. */
+int gci_server_poll(gci_server *s, int64_t req, double *eval, double *evec, double *latency_s);
+/* Block until every submitted request has finished. */
+int gci_server_drain(gci_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -867,7 +867,7 @@ int64_t orc_gdvd_worksize(int n, int lim, int nume)
 {
     (void)nume;
     /* dvdson.f90:405-417 layout with NOC = 0 */
-    return (int64_t)n * lim * 2 + (int64_t)lim * (lim + 1) + (int64_t)lim * lim + 9 * (int64_t)lim + 8;
+    return (int64_t)n * lim * 2 + (int64_t)lim * (lim + 1) + (int64_t)lim * lim + 11 * (int64_t)lim + 8;
 }
 
 /* libdvd90/gdvd.f90 + dvdson.f90 (ILOW = 1 .. IHIGH, NOC = 0, HIEND false).

@@ -1,0 +1,95 @@
+"""Pin the C oracle against GRASP2018's own routines compiled from source.
+
+oracle/build_ref.sh compiles the reference's radial numerics (RADGRD,
+SETQIC, QUAD, YZK, SLATER, RINTI, DPBDT, NUCPOT/ES), GENINTRK/RKINTC and
+SPICMV2 with amdflang into oracle/_ref/libgrasp_ref.so.  These tests require
+bit-for-bit agreement.  Skipped where the reference build is absent.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from grasp_amd import workloads as W
+
+REF = os.path.join(os.path.dirname(O.__file__), "_ref", "libgrasp_ref.so")
+pytestmark = pytest.mark.skipif(not os.path.exists(REF), reason="oracle/_ref not built (no reference tree)")
+
+
+@pytest.fixture()
+def ref(tmp_path):
+    # YZK caches powers of r in SAVEd arrays on first use (yzk.f90:70-93), so
+    # each grid gets a fresh copy of the library
+    import shutil
+    path = tmp_path / "libgrasp_ref_copy.so"
+    shutil.copy(REF, path)
+    R = C.CDLL(str(path))
+    R.ref_slater.restype = C.c_double
+    R.ref_rinti.restype = C.c_double
+    R.ref_quad.restype = C.c_double
+    R.ref_yzk.restype = C.c_int
+    R.ref_genintrk.restype = C.c_int
+    return R
+
+
+def _setup(ref, z, names, occupied, parm):
+    n, h = 590, 0.05
+    rnt = 2e-6 / z
+    orb = W.screened_orbitals(z, names, occupied=occupied)
+    g = O.Grid(n, rnt, h)
+    o = O.Orbitals(orb.P, orb.Q, orb.mf, orb.nak, orb.nkl, orb.nkj)
+    p = np.array(parm if parm is not None else [0.0, 0.0])
+    ref.ref_init(C.c_int(n), C.c_double(rnt), C.c_double(h), C.c_double(W.C_LIGHT), C.c_double(z),
+                 C.c_int(2 if parm is not None else 0), O.dp(p))
+    ref.ref_set_orbitals(C.c_int(o.nw), C.c_int(o.ld), O.dp(o.P), O.dp(o.Q), O.i32p(o.mf), O.i32p(o.nak),
+                         O.i32p(o.nkl), O.i32p(o.nkj))
+    return g, o, orb
+
+
+@pytest.mark.parametrize("z,nmax,lmax,occ,amass", [(6.0, 4, 3, {1: 2, 2: 2}, 12.0), (26.0, 3, 2, {1: 2, 2: 8}, 56.0)])
+def test_grid_nucpot_slater_rinti_bitexact(ref, z, nmax, lmax, occ, amass):
+    parm = W.fermi_parameters(amass)
+    g, o, orb = _setup(ref, z, W.subshells(nmax, lmax), occ, parm)
+    n = g.n
+    r, rp, rpor, zz = np.zeros(n + 10), np.zeros(n + 10), np.zeros(n + 10), np.zeros(n)
+    ref.ref_grid(O.dp(r), O.dp(rp), O.dp(rpor), O.dp(zz))
+    assert np.array_equal(r, g.r) and np.array_equal(rp, g.rp) and np.array_equal(rpor[1:], g.rpor[1:])
+    zo = O.nucpot(g, z, parm)
+    assert np.array_equal(zz, zo)
+    for k in range(0, 3):
+        for (i, j) in [(0, 0), (0, 1), (1, 2), (2, 2)]:
+            y1 = np.zeros(n)
+            m1 = ref.ref_yzk(k, i + 1, j + 1, O.dp(y1))
+            y2, m2 = O.yzk(g, o, k, i, j)
+            assert m1 == m2 and np.array_equal(y1, y2)
+    for j in range(1, o.nw + 1):
+        assert ref.ref_rinti(j, j, 0) == O.rinti(g, o, zo, W.C_LIGHT, j, j, 0)
+    ta = np.exp(-1.3 * g.r[:n]) * g.r[:n] ** 2 * g.rp[:n]
+    assert ref.ref_quad(O.dp(np.ascontiguousarray(ta)), C.c_int(500)) == g.quad(ta, 500)
+
+
+def test_genintrk_table_bitexact(ref):
+    g, o, orb = _setup(ref, 6.0, W.subshells(4, 3), {1: 2, 2: 2}, W.fermi_parameters(12.0))
+    cap = 200000
+    kst = np.zeros(25, dtype=np.int32)
+    keys = np.zeros(cap, dtype=np.int32)
+    vals = np.zeros(cap)
+    nr = ref.ref_genintrk(O.i32p(kst), O.i32p(keys), O.dp(vals), C.c_int(cap))
+    ks, ko, vo, kv, lab = O.genintrk(g, o)
+    assert nr == len(ko)
+    assert np.array_equal(kst[: len(ks)], ks)
+    assert np.array_equal(keys[:nr], ko)
+    assert np.array_equal(vals[:nr], vo)
+
+
+def test_spicmv2_bitexact(ref):
+    csr = W.clustered_block(500, 3, fill=0.1)
+    B = np.asfortranarray(np.random.default_rng(0).standard_normal((500, 2)))
+    Cr = np.zeros((500, 2), order="F")
+    iendc = csr.row_end.astype(np.int32)
+    ref.ref_spicmv2(C.c_int(500), C.c_int(2), O.i32p(iendc), O.i32p(csr.col), O.dp(csr.val),
+                    C.c_int64(csr.nnz), O.dp(B), O.dp(Cr))
+    Co = O.spicmv2(csr.row_end, csr.col, csr.val, B)
+    assert np.array_equal(Cr, Co)
