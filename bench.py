@@ -1,0 +1,258 @@
+#!/usr/bin/env python3
+"""Benchmark: batched CI eigensolves on one MI355X.
+
+Workload (the batched configuration): 64 J/parity blocks, sizes log-uniform
+in [1e2, 5e4] CSFs, lower fill uniform in [1%, 10%] clustered in CSF-group
+sub-blocks, diagonal ~ N(-Z^2/2, 5^2) with Z = 26, off-diagonals ~ N(0,
+1e-4), all seeded; lowest 4 eigenpairs per block at 1e-9 relative.  Blocks
+are generated directly in HBM (grasp_amd.synth) and stay resident.
+
+One step = one full batched solve of all 64 blocks (split FP32 hi+lo
+storage, FP32 bulk SpMM with FP64 accumulation, block Davidson, FP64
+refinement).  value = eigenpairs per second over the timed steps.
+
+Also reported on the same line:
+  plain_fp64   the plain FP64 drop-in (MANEIG per block, reference criteria)
+               on the same blocks: eigenpairs/s and the speedup
+  accuracy     max relative eigenvalue error of the batched form vs the
+               plain FP64 form (total energies EVAL + EAV)
+  roofline     the symmetric SpMM kernel (panel-CSC) inside the timed
+               region: algorithmic bytes / kernel time from HIP events
+  serving      continuous batching: Poisson arrivals, p50/p99 latency
+  cpu_baseline the CPU restatement (oracle) of MANEIG on a bounded sample
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nblocks", type=int, default=64)
+    ap.add_argument("--nev", type=int, default=4)
+    ap.add_argument("--tol", type=float, default=1e-9)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--serve-rate", type=float, default=400.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the CI blocks shard by block across GPUs with no communication, "
+              "so a multi-GPU figure is the sum of independent one-GPU runs.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from grasp_amd import _lib as L
+    from grasp_amd import rci, synth
+
+    L.check(L.lib().gci_init(0))
+
+    def sync(_=0):
+        torch.cuda.synchronize()
+
+    spec = synth.batched_workload(args.nblocks, args.seed)
+    t0 = time.time()
+    blocks, eavs = [], []
+    for (n, f, s) in spec:
+        b, e = synth.synth_block(n, s, fill=f)
+        blocks.append(b)
+        eavs.append(e)
+    nnz = [b.nnz_offdiag for b in blocks]
+    log(f"generated {len(blocks)} blocks, sum n = {sum(b.n for b in blocks)}, "
+        f"sum nnz_L = {sum(nnz) + sum(b.n for b in blocks):.3e} in {time.time() - t0:.1f}s")
+    nev = [args.nev] * len(blocks)
+    neig = sum(nev)
+
+    # warmup (builds the panel-CSC layouts once)
+    for w in range(args.warmup):
+        t = time.time()
+        rci.batch_solve(blocks, nev, args.tol, eavs)
+        log(f"warmup {w}: {time.time() - t:.3f}s")
+
+    # timed region: K batched solves; kernel time of the SpMM via HIP events
+    L.check(L.lib().gci_profile_enable(1))
+    sync()
+    t_start = time.perf_counter()
+    results = None
+    for k in range(args.steps):
+        results = rci.batch_solve(blocks, nev, args.tol, eavs)
+    sync(0)
+    t_end = time.perf_counter()
+    stats = np.zeros(9)
+    L.check(L.lib().gci_profile_read(L.dp(stats)))
+    L.check(L.lib().gci_profile_enable(0))
+    elapsed = t_end - t_start
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = neig * args.steps / elapsed
+    evals, evecs, bstats = results
+    log(f"batched: {ms_per_step:.1f} ms/step, {value:.1f} eigenpairs/s, "
+        f"iterations max {max(s['nloops'] for s in bstats)}")
+
+    # roofline of the SpMM kernel (bulk operator dominates; FP64 refinement
+    # launches reported beside it)
+    launches, ms, byts = stats.reshape(3, 3).T
+    bulk = 1  # GCI_OP_SPLIT_BULK
+    roof = None
+    if launches[bulk] > 0:
+        avg_ms = ms[bulk] / launches[bulk]
+        avg_bytes = byts[bulk] / launches[bulk]
+        achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_pcsc (panel-CSC symmetric SpMM, bulk fp32 hi, m=4)",
+                "launches": int(launches[bulk]), "avg_ms": round(avg_ms, 4),
+                "alg_bytes_per_launch": round(avg_bytes),
+                "refine_fp64_achieved_GBs": round(byts[2] / (ms[2] * 1e-3) / 1e9, 1) if ms[2] > 0 else None}
+
+    out = {
+        "metric": "CI eigenpairs/sec + p50/p99 latency at 1e-9 rel; SpMM HBM GB/s vs FP64 plain",
+        "value": round(value, 2),
+        "unit": "eigenpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 hi+lo storage, FP32 bulk SpMM with FP64 accumulation, FP64 refinement",
+        "data": "synthetic (seeded, generated in HBM)",
+        "config": {"workload": "batched: 64 J/parity blocks, 1e2..5e4 CSFs log-uniform, fill 1-10% clustered, "
+                               "Z=26 diagonal, 4 eigenpairs/block at 1e-9 relative",
+                   "nblocks": len(blocks), "sum_n": int(sum(b.n for b in blocks)),
+                   "sum_nnz_lower": int(sum(nnz) + sum(b.n for b in blocks)), "nev": args.nev,
+                   "tol": args.tol, "seed": args.seed, "parallelism": "one GPU, blocks interleaved"},
+        "roofline": roof,
+    }
+
+    # plain FP64 drop-in on the same blocks (MANEIG per block)
+    if not args.no_plain:
+        plain_blocks = []
+        t = time.time()
+        for (n, f, s), e in zip(spec, eavs):
+            b, e2 = synth.synth_block(n, s, fill=f, precision=L.GCI_FP64)
+            assert abs(e2 - e) <= 1e-12 * abs(e)
+            plain_blocks.append(b)
+        sync(0)
+        t_plain0 = time.perf_counter()
+        plain = []
+        nloops = []
+        for b in plain_blocks:
+            ev, _, st = rci.maneig(b, args.nev)
+            plain.append(ev)
+            nloops.append(st["nloops"])
+        sync(0)
+        t_plain = time.perf_counter() - t_plain0
+        plain_rate = neig / t_plain
+        errs = []
+        for b in range(len(blocks)):
+            tot = plain[b] + eavs[b]
+            errs.append(np.max(np.abs(evals[b] - plain[b]) / np.abs(tot)))
+        out["plain_fp64"] = {"value": round(plain_rate, 2), "unit": "eigenpairs/s", "seconds": round(t_plain, 3),
+                             "speedup_batched": round(value / plain_rate, 2),
+                             "davidson_iterations_max": int(max(nloops))}
+        out["accuracy"] = {"max_rel_err_vs_fp64": float(f"{max(errs):.3e}"), "tol": args.tol,
+                           "all_within_tol": bool(max(errs) < args.tol)}
+        log(f"plain FP64: {t_plain:.2f}s, {plain_rate:.1f} eigenpairs/s; max rel err batched vs FP64 {max(errs):.2e}")
+        for b in plain_blocks:
+            b.free()
+
+    # serving: Poisson arrivals, sizes from the same distribution
+    if not args.no_serve:
+        rng = np.random.default_rng(args.seed + 1)
+        nreq = args.serve_requests
+        picks = rng.integers(0, len(spec), size=nreq)
+        nevs = rng.integers(1, 9, size=nreq)
+        tols = np.where(rng.random(nreq) < 0.5, 1e-8, 1e-10)
+        gaps = rng.exponential(1.0 / args.serve_rate, size=nreq)
+        srv = rci.Server(200 * (1 << 30), max_active=64)
+        pending = []
+        # pre-build the request matrices (device), then release them on the schedule
+        mats = []
+        for i in range(nreq):
+            n, f, s = spec[picks[i]]
+            b, e = synth.synth_block(n, s + 7919 * (i + 1), fill=f)
+            mats.append((b, e))
+        sync(0)
+        t0 = time.perf_counter()
+        tnext = t0
+        for i in range(nreq):
+            tnext += gaps[i]
+            while time.perf_counter() < tnext:
+                pass
+            b, e = mats[i]
+            pending.append((srv.submit(b, int(nevs[i]), float(tols[i]), e), int(nevs[i])))
+        srv.drain()
+        t_serve = time.perf_counter() - t0
+        lat = []
+        for rid, k in pending:
+            r = srv.poll(rid)
+            lat.append(r[2])
+        srv.close()
+        lat = np.array(lat)
+        out["serving"] = {"requests": nreq, "arrival_rate_per_s": args.serve_rate,
+                          "eigenpairs_per_s": round(float(np.sum(nevs)) / t_serve, 2),
+                          "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+                          "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+                          "hbm_budget_GiB": 200, "max_active": 64}
+        log(f"serving: {nreq} requests in {t_serve:.2f}s, p50 {np.percentile(lat, 50):.3f}s "
+            f"p99 {np.percentile(lat, 99):.3f}s")
+
+    # CPU baseline: the oracle's MANEIG (dense <= 2000, else INIEST2 + GDVD
+    # with a row-cyclic threaded SPICMV2) on the smallest blocks, ~15 s
+    if not args.no_cpu:
+        import oracle as O
+
+        threads = min(16, os.cpu_count() or 1)
+        order = np.argsort([b.n for b in blocks])
+        done_pairs, t_cpu, used, maxerr = 0, 0.0, [], 0.0
+        for b in order:
+            if t_cpu > args.cpu_seconds:
+                break
+            iendc, irow, emt = blocks[b].download()
+            emt = emt.copy()
+            emt[iendc[1:] - 1] += eavs[b]  # back to H as SETHAM writes it
+            t = time.perf_counter()
+            ev, _, nl, nm, rc = O.maneig(iendc, irow, emt, eavs[b], args.nev, nthreads=threads)
+            t_cpu += time.perf_counter() - t
+            done_pairs += args.nev
+            used.append(int(blocks[b].n))
+            maxerr = max(maxerr, float(np.max(np.abs(ev - evals[b]) / np.abs(ev + eavs[b]))))
+        out["cpu_baseline"] = {"value": round(done_pairs / t_cpu, 3), "unit": "eigenpairs/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"oracle MANEIG on the {len(used)} smallest blocks (n = {used[0]}..{used[-1]}), "
+                                         f"{args.nev} eigenpairs each, {t_cpu:.1f} s",
+                               "max_rel_err_gpu_vs_oracle": float(f"{maxerr:.3e}")}
+        log(f"cpu baseline: {done_pairs / t_cpu:.2f} eigenpairs/s on {len(used)} blocks")
+
+    for b in blocks:
+        b.free()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -64,6 +64,7 @@ EXPORTS = [
     "gci_hmat_upload", "gci_hmat_free", "gci_hmat_info", "gci_hmat_download", "gci_spmm",
     "gci_spmm_device", "gci_davidson", "gci_maneig", "gci_batch_solve", "gci_server_create",
     "gci_server_destroy", "gci_server_submit", "gci_server_poll", "gci_server_drain",
+    "gci_profile_enable", "gci_profile_read",
 ]
 
 

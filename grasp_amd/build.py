@@ -40,22 +40,35 @@ def _compile(src: str, obj: str):
     return obj
 
 
-def build(verbose: bool = True) -> str:
-    bdir = os.path.join(CSRC, "_build")
-    os.makedirs(bdir, exist_ok=True)
-    srcs = sources()
-    objs = [os.path.join(bdir, os.path.basename(s)[:-4] + ".o") for s in srcs]
-    jobs = min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
-    with cf.ThreadPoolExecutor(jobs) as ex:
-        list(ex.map(_compile, srcs, objs))
-    tmp = OUT + ".tmp"
-    cmd = [HIPCC, *LDFLAGS, *objs, "-o", tmp]
+SYNTH_DIR = os.path.join(CSRC, "synth")
+SYNTH_OUT = os.path.join(HERE, "libgcisynth.so")
+
+
+def _link(objs, out, extra=()):
+    tmp = out + ".tmp"
+    cmd = [HIPCC, *LDFLAGS, *objs, *extra, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
-    os.replace(tmp, OUT)
+    os.replace(tmp, out)
+
+
+def build(verbose: bool = True) -> str:
+    """Builds libgraspci.so (the engine) and libgcisynth.so (seeded
+    synthetic block generator used by bench.py and tests)."""
+    bdir = os.path.join(CSRC, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    srcs = sources()
+    ssrcs = sorted(os.path.join(SYNTH_DIR, f) for f in os.listdir(SYNTH_DIR) if f.endswith(".hip"))
+    objs = [os.path.join(bdir, os.path.basename(s)[:-4] + ".o") for s in srcs]
+    sobjs = [os.path.join(bdir, "synth_" + os.path.basename(s)[:-4] + ".o") for s in ssrcs]
+    jobs = max(1, min(8, os.cpu_count() or 1))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(_compile, srcs + ssrcs, objs + sobjs))
+    _link(objs, OUT)
+    _link(sobjs, SYNTH_OUT, [f"-L{HERE}", "-lgraspci", "-Wl,-rpath,$ORIGIN"])
     if verbose:
-        print(f"built {OUT}")
+        print(f"built {OUT} and {SYNTH_OUT}")
     return OUT
 
 

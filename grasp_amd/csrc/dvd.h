@@ -31,6 +31,14 @@ struct DvdProb {
     const double *val;
     const float *hi, *lo;
     const double *diagH;            // FP64 diagonal of H (shifted)
+    const int32_t *pseg_col;        // panel-CSC view used by the operator
+    const int64_t *pseg_start;
+    const uint16_t *plrow;
+    const double *pval;
+    const float *phi, *plo;
+    int64_t pnnz;                   // stored off-diagonal entries
+    void *xs;                       // operator staging: n x 8 (fp32 or fp64)
+    double *ys;                     // n x 8 fp64
     double *basis, *ab;             // n x lim each (column-major, ld = n)
     double *diag;                   // preconditioner diagonal (drifts as in GDVD)
     double *xc, *rc;                // n x nume candidate Ritz vectors / residuals
@@ -45,6 +53,7 @@ struct DvdProb {
     int refine_phase;               // 0 bulk operator, 1 FP64 (hi+lo) operator
     int refine_loops;
     int bulk_loops, bulk_nmv;       // iterations/products of the bulk phase
+    int brk_mask;                   // new columns that lost rank in the first GS pass
     // --- small arrays (1-based reference arrays stored 0-based)
     double eigval[MAXLIM];
     double oldval[MAXLIM + 1];
@@ -107,8 +116,15 @@ class DavidsonEngine {
     void launch_iteration();
     void launch_finalize();
     void launch_orth(int passes, int phase);
-    void launch_op(int phase);
+    void launch_op(int phase, int M);
+    void update_modes();
     void run_initdvd(const std::vector<int> &slots);
+    void drain_profile();
+    struct ProfEv {
+        int mode;
+        hipEvent_t e0, e1;
+    };
+    std::vector<ProfEv> prof_events_;
 
     int cap_;
     hipStream_t stream_;
@@ -121,12 +137,29 @@ class DavidsonEngine {
     DBuf<int64_t> chunk_row_;
     DBuf<double> partial_;
     DBuf<int> dphase_;
+    DBuf<SpmmJob> djobs_;
+    DBuf<SpmmItem> ditems_;
+    int64_t nitems_ = 0;
+    int modes_used_ = 0;           // bitmask of operator modes over active slots
+    int mwidth_iter_ = 1;          // launch width for iterations
     int64_t total_chunks_ = 0;
     std::vector<int> hprob_;
     std::vector<int64_t> hrow_;
     hipGraphExec_t gexec_ = nullptr;
     bool graph_valid_ = false;
 };
+
+// Operator profiling (gci_profile_*): per operator mode, launches, summed
+// kernel milliseconds (HIP events around the panel-CSC kernel) and summed
+// algorithmic bytes of the jobs each launch served.
+struct OpProfile {
+    bool on = false;
+    double launches[3] = {0, 0, 0};
+    double ms[3] = {0, 0, 0};
+    double bytes[3] = {0, 0, 0};
+};
+OpProfile &op_profile();
+double *op_bytes_device();  // device accumulator [3] written by k_make_jobs
 
 // Dense lowest eigenpairs on the device (rocSOLVER dsyevx): a is n x n
 // column-major (device), eval/evec device outputs for il..iu (1-based).

@@ -486,8 +486,13 @@ __global__ __launch_bounds__(256) void k_update(DvdProb *P, const int *chunk_pro
     }
 }
 
-// Cholesky of the nnew x nnew Gram matrix; rinv = R^{-1} (upper).
-__global__ void k_chol(DvdProb *P, const double *partial, int want_phase) {
+// Cholesky of the nnew x nnew Gram matrix; rinv = R^{-1} (upper).  A new
+// column that is numerically in the span of the basis and of the earlier
+// new columns (|w_j|^2 after projection below 1e-24 of its Gram entry, e.g.
+// A e_i for a CSF row with no couplings) is dropped from the product; in
+// the first pass it is then replaced by a pseudo-random direction that the
+// second pass orthonormalises.
+__global__ void k_chol(DvdProb *P, const double *partial, int want_phase, int pass) {
     DvdProb &p = P[blockIdx.x];
     if (p.phase != want_phase) return;
     __shared__ double G[MAXNUME * MAXNUME];
@@ -496,14 +501,16 @@ __global__ void k_chol(DvdProb *P, const double *partial, int want_phase) {
     __syncthreads();
     if (threadIdx.x != 0) return;
     double R[MAXNUME][MAXNUME];
+    int brk = 0;
     for (int i = 0; i < nn; ++i)
         for (int j = 0; j < nn; ++j) R[i][j] = 0.0;
     for (int j = 0; j < nn; ++j) {
         double s = G[j * nn + j];
         for (int q = 0; q < j; ++q) s -= R[q][j] * R[q][j];
-        if (!(s > 0.0)) {
-            p.ierr |= 1 << 20;  // orthogonalisation breakdown
-            s = 1e-300;
+        if (!(s > 1.0e-24 * fabs(G[j * nn + j])) || !(s > 1.0e-300)) {
+            brk |= 1 << j;
+            R[j][j] = 1.0;
+            continue;  // R[j][i > j] stay zero
         }
         R[j][j] = sqrt(s);
         for (int i = j + 1; i < nn; ++i) {
@@ -520,6 +527,36 @@ __global__ void k_chol(DvdProb *P, const double *partial, int want_phase) {
             double s = 0.0;
             for (int q = i + 1; q <= j; ++q) s += R[i][q] * p.rinv[q * nn + j];
             p.rinv[i * nn + j] = -s / R[i][i];
+        }
+    }
+    for (int j = 0; j < nn; ++j) {
+        if (!(brk & (1 << j))) continue;
+        for (int i = 0; i < nn; ++i) {
+            p.rinv[i * nn + j] = 0.0;
+            p.rinv[j * nn + i] = 0.0;
+        }
+    }
+    if (pass == 0) p.brk_mask = brk;
+    else if (brk) p.ierr |= 1 << 20;  // orthogonalisation failed twice
+}
+
+// pseudo-random replacement for dropped columns (after the first pass)
+__global__ __launch_bounds__(256) void k_fill_random(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
+                                                     int want_phase) {
+    DvdProb &p = P[chunk_prob[blockIdx.x]];
+    if (p.phase != want_phase || !p.brk_mask) return;
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    for (int j = 0; j < p.nnew; ++j) {
+        if (!(p.brk_mask & (1 << j))) continue;
+        double *W = p.basis + (int64_t)(p.kpass + j) * p.n;
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+            uint64_t x = (uint64_t)r * 0x9E3779B97F4A7C15ull ^ ((uint64_t)(p.nloops * 131 + j) << 32) ^ (uint64_t)p.slot;
+            x ^= x >> 33;
+            x *= 0xff51afd7ed558ccdull;
+            x ^= x >> 33;
+            x *= 0xc4ceb9fe1a85ec53ull;
+            x ^= x >> 33;
+            W[r] = (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
         }
     }
 }
@@ -598,49 +635,76 @@ __global__ void k_final_post(DvdProb *P, const double *partial) {
 }
 
 // ---------------------------------------------------- batched operator
-// y[:, j] = H x[:, j] for the NNEW new columns of every ITER problem
-// (x = basis[:, kpass..], y = ab[:, kpass..]); symmetric lower CSR, one
-// row per group of 16 lanes, atomic symmetric scatter.
-__global__ __launch_bounds__(256) void k_zero_new(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
-                                                  int want_phase) {
-    DvdProb &p = P[chunk_prob[blockIdx.x]];
-    if (p.phase != want_phase) return;
-    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
-    double *Y = p.ab + (int64_t)p.kpass * p.n;
-    for (int j = 0; j < p.nnew; ++j)
-        for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) Y[r + (int64_t)j * p.n] = 0.0;
+// AB[:, kpass..kpass+nnew) = H B[:, kpass..) for every problem in the stage,
+// through the panel-CSC SpMM (spmm.hip): jobs are made on the device from
+// the problem states, x is staged row-major (fp32 for the bulk operator),
+// and the FP64 diagonal is applied when the result is unpacked.
+__device__ inline int op_mode(const DvdProb &p) {
+    return p.refine_phase ? (p.mode == GCI_OP_FP64 ? GCI_OP_FP64 : GCI_OP_SPLIT_FULL) : p.mode;
 }
 
-template <int G>
-__global__ __launch_bounds__(256) void k_spmm_batch(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
-                                                    int want_phase) {
-    DvdProb &p = P[chunk_prob[blockIdx.x]];
-    if (p.phase != want_phase) return;
-    const int nn = p.nnew;
-    const int mode = p.refine_phase ? (p.mode == GCI_OP_FP64 ? GCI_OP_FP64 : GCI_OP_SPLIT_FULL) : p.mode;
-    const double *X = p.basis + (int64_t)p.kpass * p.n;
-    double *Y = p.ab + (int64_t)p.kpass * p.n;
-    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
-    const int lane = threadIdx.x % G;
-    for (int64_t r = r0 + threadIdx.x / G; r < r1; r += blockDim.x / G) {
-        const int64_t beg = p.rp[r], end = p.rp[r + 1];
-        for (int j = 0; j < nn; ++j) {
-            const double xr = X[r + (int64_t)j * p.n];
-            const double xrs = (mode == GCI_OP_SPLIT_BULK) ? (double)(float)xr : xr;
-            double acc = 0.0;
-            for (int64_t e = beg + lane; e < end; e += G) {
-                const int64_t c = p.col[e];
-                double a, xc = X[c + (int64_t)j * p.n];
-                if (mode == GCI_OP_FP64) a = p.val[e];
-                else if (mode == GCI_OP_SPLIT_BULK) { a = (double)p.hi[e]; xc = (double)(float)xc; }
-                else a = (double)p.hi[e] + (double)p.lo[e];
-                acc = fma(a, xc, acc);
-                unsafeAtomicAdd(&Y[c + (int64_t)j * p.n], a * xrs);
-            }
-            for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
-            if (lane == 0) unsafeAtomicAdd(&Y[r + (int64_t)j * p.n], fma(p.diagH[r], xr, acc));
-        }
+// columns [c0, c0 + 8) of the NNEW new vectors (the staging holds 8)
+__device__ inline int op_cols(const DvdProb &p, int c0) { return min(8, p.nnew - c0); }
+
+// algorithmic bytes of one operator application (m vectors): matrix values
+// and indices, row pointers, x and y (x as fp32 for the bulk operator)
+__device__ inline double op_alg_bytes(const DvdProb &p, int m, int mode) {
+    const double n = (double)p.n, nnz = (double)(p.pnnz + p.n);
+    if (mode == GCI_OP_SPLIT_BULK) return nnz * 8.0 + (n + 1) * 8.0 + n * m * 4.0 + n * m * 8.0;
+    return nnz * 12.0 + (n + 1) * 8.0 + 2.0 * n * m * 8.0;
+}
+
+__global__ void k_make_jobs(DvdProb *P, int nprob, SpmmJob *jobs, int want_phase, int c0, double *alg_bytes) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprob) return;
+    const DvdProb &p = P[i];
+    SpmmJob j{};
+    j.active = (p.phase == want_phase && op_cols(p, c0) > 0) ? 1 : 0;
+    if (j.active) {
+        j.seg_col = p.pseg_col;
+        j.seg_start = p.pseg_start;
+        j.lrow = p.plrow;
+        j.val = p.pval;
+        j.hi = p.phi;
+        j.lo = p.plo;
+        j.xs = p.xs;
+        j.ys = p.ys;
+        j.m = op_cols(p, c0);
+        j.mode = op_mode(p);
+        if (alg_bytes) atomicAdd(&alg_bytes[j.mode], op_alg_bytes(p, j.m, j.mode));
     }
+    jobs[i] = j;
+}
+
+__global__ __launch_bounds__(256) void k_pack_slots(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
+                                                    int want_phase, int M, int c0) {
+    DvdProb &p = P[chunk_prob[blockIdx.x]];
+    const int m = op_cols(p, c0);
+    if (p.phase != want_phase || m <= 0) return;
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    const double *X = p.basis + (int64_t)(p.kpass + c0) * p.n;
+    const bool f32 = op_mode(p) == GCI_OP_SPLIT_BULK;
+    for (int64_t q = r0 * M + threadIdx.x; q < r1 * M; q += blockDim.x) {
+        const int64_t i = q / M;
+        const int v = (int)(q % M);
+        const double x = v < m ? X[i + (int64_t)v * p.n] : 0.0;
+        if (f32) static_cast<float *>(p.xs)[q] = (float)x;
+        else static_cast<double *>(p.xs)[q] = x;
+        p.ys[q] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_unpack_slots(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
+                                                      int want_phase, int M, int c0) {
+    DvdProb &p = P[chunk_prob[blockIdx.x]];
+    const int m = min(op_cols(p, c0), M);
+    if (p.phase != want_phase || m <= 0) return;
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    const double *X = p.basis + (int64_t)(p.kpass + c0) * p.n;
+    double *Y = p.ab + (int64_t)(p.kpass + c0) * p.n;
+    for (int v = 0; v < m; ++v)
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x)
+            Y[i + (int64_t)v * p.n] = fma(p.diagH[i], X[i + (int64_t)v * p.n], p.ys[i * M + v]);
 }
 
 // ------------------------------------------------------------ INITDVD
@@ -732,6 +796,8 @@ DavidsonEngine::DavidsonEngine(int capacity, hipStream_t s) : cap_(capacity), st
     work_.resize(cap_);
     dprob_.alloc(cap_);
     dphase_.alloc(cap_);
+    djobs_.alloc(cap_);
+    ditems_.alloc(1);
     std::vector<DvdProb> init(cap_);
     for (int i = 0; i < cap_; ++i) {
         std::memset(&init[i], 0, sizeof(DvdProb));
@@ -751,7 +817,7 @@ DavidsonEngine::~DavidsonEngine() {
 size_t DavidsonEngine::workspace_bytes(const DvdSpec &s) {
     const int64_t n = s.h ? s.h->n : 0;
     const int lim = s.lim > 0 ? s.lim : (int)std::min<int64_t>(n, 2 * s.nume + 60);
-    return (size_t)n * (2 * lim + 1 + 2 * s.nume) * sizeof(double) + sizeof(DvdProb);
+    return (size_t)n * (2 * lim + 1 + 2 * s.nume + 16) * sizeof(double) + sizeof(DvdProb);
 }
 
 int DavidsonEngine::active() const {
@@ -787,6 +853,19 @@ void DavidsonEngine::rebuild_chunks() {
         GCI_HIP(hipMemcpyAsync(chunk_row_.p, hrow_.data(), total_chunks_ * sizeof(int64_t), hipMemcpyHostToDevice,
                                stream_));
     }
+    std::vector<SpmmItem> items;
+    for (int i = 0; i < cap_; ++i) {
+        if (phase_[i] == PH_EMPTY) continue;
+        for (SpmmItem it : spec_[i].h->pc.items) {
+            it.job = i;
+            items.push_back(it);
+        }
+    }
+    nitems_ = (int64_t)items.size();
+    if ((size_t)std::max<int64_t>(nitems_, 1) > ditems_.n) ditems_.alloc(std::max<int64_t>(nitems_, 1));
+    if (nitems_)
+        GCI_HIP(hipMemcpyAsync(ditems_.p, items.data(), nitems_ * sizeof(SpmmItem), hipMemcpyHostToDevice, stream_));
+    update_modes();
     for (int i = 0; i < cap_; ++i) {
         if (phase_[i] == PH_EMPTY) continue;
         // chunk0 and nchunks are adjacent fields of DvdProb
@@ -836,7 +915,8 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         s2.lim = lim;
         s2.maxiter = maxiter;
         spec_[slot] = s2;
-        work_[slot].alloc((size_t)n * (2 * lim + 1 + 2 * nume));
+        const_cast<HMat *>(sp.h)->pc.R == 0 ? ensure_pcsc(*const_cast<HMat *>(sp.h)) : (void)0;
+        work_[slot].alloc((size_t)n * (2 * lim + 1 + 2 * nume + 16));
         double *w = work_[slot].p;
         DvdProb p;
         std::memset(&p, 0, sizeof(p));
@@ -860,8 +940,17 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         p.diag = w + (size_t)n * 2 * lim;
         p.xc = p.diag + n;
         p.rc = p.xc + (size_t)n * nume;
+        p.xs = p.rc + (size_t)n * nume;
+        p.ys = static_cast<double *>(p.xs) + (size_t)n * 8;
+        p.pseg_col = sp.h->pc.seg_col.p;
+        p.pseg_start = sp.h->pc.seg_start.p;
+        p.plrow = sp.h->pc.lrow.p;
+        p.pval = sp.h->pc.val.p;
+        p.phi = sp.h->pc.hi.p;
+        p.plo = sp.h->pc.lo.p;
         p.tol_rel = sp.tol_rel;
         p.eshift = sp.eshift;
+        p.pnnz = sp.h->nnz;
         p.slot = slot;
         p.phase = PH_INIT;
         p.niv = niv[q];
@@ -902,10 +991,13 @@ void DavidsonEngine::run_initdvd(const std::vector<int> &slots) {
     std::vector<InitStep> steps(cap_, InitStep{0, 0, 0});
     std::vector<int> ist(cap_, 1), iend(cap_, 0), nume(cap_, 0);
     std::vector<DvdProb> cur(1);
+    int init_width = 1;
+    update_modes();
     for (int sl : slots) {
         GCI_HIP(hipMemcpy(cur.data(), &dprob_.p[sl], sizeof(DvdProb), hipMemcpyDeviceToHost));
         iend[sl] = cur[0].niv;
         nume[sl] = cur[0].nume;
+        init_width = std::max(init_width, std::max(cur[0].niv, cur[0].nume));
         steps[sl] = InitStep{1, 0, cur[0].niv};
     }
     auto push = [&]() {
@@ -914,7 +1006,7 @@ void DavidsonEngine::run_initdvd(const std::vector<int> &slots) {
         GCI_HIP(hipStreamSynchronize(stream_));
     };
     push();
-    launch_op(PH_INIT);
+    launch_op(PH_INIT, init_width);
     for (;;) {
         bool any = false;
         std::fill(steps.begin(), steps.end(), InitStep{0, 0, 0});
@@ -935,7 +1027,7 @@ void DavidsonEngine::run_initdvd(const std::vector<int> &slots) {
         launch(k_init_copy, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p,
                (const InitStep *)dsteps.p);
         launch_orth(2, PH_INIT);
-        launch_op(PH_INIT);
+        launch_op(PH_INIT, init_width);
     }
     for (int sl : slots) steps[sl] = InitStep{0, 0, iend[sl]};
     push();
@@ -957,16 +1049,54 @@ void DavidsonEngine::launch_orth(int passes, int phase) {
         launch(k_coef, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, phase);
         launch(k_update, G, 256, stream_, dprob_.p, cp, cr, phase);
         launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 1, phase);
-        launch(k_chol, cap_, 64, stream_, dprob_.p, (const double *)partial_.p, phase);
+        launch(k_chol, cap_, 64, stream_, dprob_.p, (const double *)partial_.p, phase, pass);
         launch(k_scale, G, 256, stream_, dprob_.p, cp, cr, phase);
+        if (pass == 0) launch(k_fill_random, G, 256, stream_, dprob_.p, cp, cr, phase);
     }
 }
 
-void DavidsonEngine::launch_op(int phase) {
+void DavidsonEngine::update_modes() {
+    int modes = 0, mw = 1;
+    for (int i = 0; i < cap_; ++i) {
+        if (phase_[i] == PH_EMPTY) continue;
+        const DvdSpec &sp = spec_[i];
+        if (sp.mode == GCI_OP_FP64) modes |= 1 << GCI_OP_FP64;
+        else if (refined_[i]) modes |= 1 << GCI_OP_SPLIT_FULL;
+        else modes |= 1 << sp.mode;
+        mw = std::max(mw, sp.mblock);
+    }
+    if (modes != modes_used_ || mw != mwidth_iter_) graph_valid_ = false;
+    modes_used_ = modes;
+    mwidth_iter_ = mw;
+}
+
+// maxcols: upper bound of NNEW over the problems in the stage
+void DavidsonEngine::launch_op(int phase, int maxcols) {
     const int G = (int)total_chunks_;
-    launch(k_zero_new, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p, phase);
-    launch(k_spmm_batch<16>, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p,
-           phase);
+    const int *cp = chunk_prob_.p;
+    const int64_t *cr = chunk_row_.p;
+    for (int c0 = 0; c0 < maxcols; c0 += 8) {
+        const int M = spmm_width(std::min(8, maxcols - c0));
+        OpProfile &pf = op_profile();
+        launch(k_make_jobs, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, djobs_.p, phase, c0,
+               pf.on ? op_bytes_device() : (double *)nullptr);
+        launch(k_pack_slots, G, 256, stream_, dprob_.p, cp, cr, phase, M, c0);
+        for (int mode = 0; mode < 3; ++mode) {
+            if (!(modes_used_ & (1 << mode))) continue;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (pf.on) {
+                GCI_HIP(hipEventCreate(&e0));
+                GCI_HIP(hipEventCreate(&e1));
+                GCI_HIP(hipEventRecord(e0, stream_));
+            }
+            spmm_items(ditems_.p, nitems_, djobs_.p, M, mode, stream_);
+            if (pf.on) {
+                GCI_HIP(hipEventRecord(e1, stream_));
+                prof_events_.push_back({mode, e0, e1});
+            }
+        }
+        launch(k_unpack_slots, G, 256, stream_, dprob_.p, cp, cr, phase, M, c0);
+    }
 }
 
 void DavidsonEngine::launch_finalize() {
@@ -990,15 +1120,30 @@ void DavidsonEngine::launch_iteration() {
     launch(k_newvec_decide, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, (const double *)partial_.p);
     launch(k_precond, G, 256, stream_, dprob_.p, cp, cr);
     launch_orth(2, PH_ITER);
-    launch_op(PH_ITER);
+    launch_op(PH_ITER, mwidth_iter_);
     launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 2, (int)PH_ITER);
     launch(k_adds, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, (int)PH_ITER, 1);
     launch_finalize();
 }
 
+void DavidsonEngine::drain_profile() {
+    if (prof_events_.empty()) return;
+    GCI_HIP(hipStreamSynchronize(stream_));
+    OpProfile &pf = op_profile();
+    for (auto &pe : prof_events_) {
+        float ms = 0.0f;
+        GCI_HIP(hipEventElapsedTime(&ms, pe.e0, pe.e1));
+        pf.ms[pe.mode] += ms;
+        pf.launches[pe.mode] += 1.0;
+        (void)hipEventDestroy(pe.e0);
+        (void)hipEventDestroy(pe.e1);
+    }
+    prof_events_.clear();
+}
+
 void DavidsonEngine::step(int iters) {
     if (total_chunks_ == 0) return;
-    if (use_graph) {
+    if (use_graph && !op_profile().on) {
         if (!graph_valid_) {
             if (gexec_) {
                 (void)hipGraphExecDestroy(gexec_);
@@ -1020,6 +1165,7 @@ void DavidsonEngine::step(int iters) {
 }
 
 std::vector<int> DavidsonEngine::poll() {
+    drain_profile();
     GCI_HIP(hipMemcpy2DAsync(dphase_.p, sizeof(int), &dprob_.p[0].phase, sizeof(DvdProb), sizeof(int), cap_,
                              hipMemcpyDeviceToDevice, stream_));
     std::vector<int> ph(cap_);
@@ -1045,6 +1191,7 @@ void DavidsonEngine::refine_restart(const std::vector<int> &slots) {
         refined_[sl] = 1;
         phase_[sl] = PH_INIT;
     }
+    update_modes();
     launch(k_copy_diag, (int)total_chunks_, 256, stream_, dprob_.p, (const int *)chunk_prob_.p,
            (const int64_t *)chunk_row_.p);
     GCI_HIP(hipStreamSynchronize(stream_));
@@ -1076,6 +1223,22 @@ void DavidsonEngine::release(int slot) {
     spec_[slot] = DvdSpec{};
     phase_[slot] = PH_EMPTY;
     rebuild_chunks();
+}
+
+// ------------------------------------------------------------ profile
+
+OpProfile &op_profile() {
+    static OpProfile pf;
+    return pf;
+}
+
+double *op_bytes_device() {
+    static DBuf<double> *buf = nullptr;
+    if (!buf) {
+        buf = new DBuf<double>(3);
+        GCI_HIP(hipMemset(buf->p, 0, 3 * sizeof(double)));
+    }
+    return buf->p;
 }
 
 // ------------------------------------------------- dense eigensolver

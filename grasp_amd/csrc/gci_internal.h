@@ -78,8 +78,36 @@ struct DBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Panel-CSC view of the strictly-lower off-diagonal part, the layout the
+// symmetric SpMM streams: rows are cut into panels of R rows; inside a panel
+// the entries are sorted by column, then row, so every column that the panel
+// touches is one contiguous segment.  A segment's scatter contribution
+// (H^T side) is reduced in registers and issued once; the panel's own rows
+// (H side) accumulate in LDS.
+struct SpmmItem {
+    int64_t seg0, seg1;  // segment range of one work item (a slice of a panel)
+    int64_t row0;        // first row of the panel
+    int32_t rows;        // panel height
+    int32_t job;         // owning job (filled per launch)
+};
+
+struct Pcsc {
+    int R = 0;                 // rows per panel (<= 65536)
+    int64_t nseg = 0, nnz = 0;
+    DBuf<int32_t> seg_col;     // nseg
+    DBuf<int64_t> seg_start;   // nseg + 1
+    DBuf<uint16_t> lrow;       // nnz, row - panel start
+    DBuf<double> val;          // nnz (FP64 values)
+    DBuf<float> hi, lo;        // nnz (split values)
+    std::vector<SpmmItem> items;  // host copy of the work items
+    size_t bytes() const {
+        return seg_col.bytes() + seg_start.bytes() + lrow.bytes() + val.bytes() + hi.bytes() + lo.bytes();
+    }
+};
+
 // Device-resident block Hamiltonian: off-diagonal lower triangle in CSR
-// (0-based columns, strictly below the diagonal, ascending), FP64 diagonal.
+// (0-based columns, strictly below the diagonal, ascending), FP64 diagonal,
+// and the panel-CSC copy the SpMM uses.
 struct HMat {
     int64_t n = 0;
     int64_t nnz = 0;          // off-diagonal entries
@@ -89,18 +117,36 @@ struct HMat {
     DBuf<double> val;         // nnz (FP64 form)
     DBuf<float> hi, lo;       // nnz (split form)
     DBuf<double> diag;        // n (shifted)
-    int group = 0;            // lanes per row chosen by the tuner (0 = not tuned)
+    Pcsc pc;
     hipStream_t stream = nullptr;
     size_t device_bytes() const {
-        return rp.bytes() + col.bytes() + val.bytes() + hi.bytes() + lo.bytes() + diag.bytes();
+        return rp.bytes() + col.bytes() + val.bytes() + hi.bytes() + lo.bytes() + diag.bytes() + pc.bytes();
     }
 };
 
-// SpMM launcher (spmm.hip): y = H x for m columns, device pointers.
-void spmm(const HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode,
-          hipStream_t s);
-// Pick the lanes-per-row group for h (autotuned once, cached on h).
-int spmm_group(HMat &h, int m, int mode);
+// One SpMM job: y = H x for m (<= 8) columns of a matrix, through
+// row-major staging buffers xs (fp32 for the bulk operator, else fp64) and
+// ys (fp64), both n x M with M the launch width.
+struct SpmmJob {
+    const int32_t *seg_col;
+    const int64_t *seg_start;
+    const uint16_t *lrow;
+    const double *val;
+    const float *hi, *lo;
+    const void *xs;
+    double *ys;
+    int m, mode, active, pad;
+};
+
+// Builds h.pc (idempotent).  R = rows per panel, target = entries per item.
+void ensure_pcsc(HMat &h, int R = 1024, int64_t target = 131072);
+// Launch width (1, 2, 4 or 8) for m columns.
+int spmm_width(int m);
+// The panel-CSC kernel over a list of work items whose jobs live in device
+// memory (jobs[item.job]); stream-ordered, no synchronisation.
+void spmm_items(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, int M, int mode, hipStream_t s);
+// Single-matrix SpMM on device pointers (column-major x, y).
+void spmm(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode, hipStream_t s);
 
 }  // namespace gci
 

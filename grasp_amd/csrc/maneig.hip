@@ -228,4 +228,28 @@ int gci_batch_solve(int nblocks, gci_hmat **blocks, const int *nev, const double
     });
 }
 
+int gci_profile_enable(int on) {
+    return guarded([&] {
+        OpProfile &pf = op_profile();
+        pf = OpProfile{};
+        pf.on = on != 0;
+        GCI_HIP(hipMemset(op_bytes_device(), 0, 3 * sizeof(double)));
+    });
+}
+
+int gci_profile_read(double *stats) {
+    return guarded([&] {
+        GCI_REQUIRE(stats, "gci_profile_read: null argument");
+        GCI_HIP(hipDeviceSynchronize());
+        OpProfile &pf = op_profile();
+        double b[3];
+        GCI_HIP(hipMemcpy(b, op_bytes_device(), sizeof(b), hipMemcpyDeviceToHost));
+        for (int m = 0; m < 3; ++m) {
+            stats[3 * m + 0] = pf.launches[m];
+            stats[3 * m + 1] = pf.ms[m];
+            stats[3 * m + 2] = b[m];
+        }
+    });
+}
+
 }  // extern "C"

@@ -2,153 +2,304 @@
 //
 // Replaces SPICMV2/DMERGE (lib9290/spicmv2.f90:40-56, dmerge.f90:36-40):
 // y = H x with H symmetric and only the strictly-lower off-diagonal part
-// stored (CSR, ascending columns) plus an FP64 diagonal.  Each stored entry
-// contributes twice: a gather into its own row (dot product, kept in
-// registers and reduced across the lanes of the row group) and a symmetric
-// scatter into the row named by its column (hardware FP64 atomic add).
+// stored, plus an FP64 diagonal.  Every stored entry H_ij (i > j) acts twice:
+// a gather into row i and the symmetric scatter into row j.
 //
-// Row groups of G lanes (G in {4, 8, 16, 32, 64}) stride over the entries of
-// one row; G is chosen per matrix by a short timing sweep (spmm_group).
-// Column-major x, y with leading dimensions; M vectors per pass.
-#include "gci_internal.h"
+// Layout (panel-CSC, built once per matrix on the device): rows are cut into
+// panels of R rows; inside a panel the entries are sorted by column, then
+// row, so each column the panel touches is one contiguous segment.  Per entry
+// the stream carries only the fp32/fp64 value and a 16-bit row offset.
+//
+// Kernel: one workgroup per work item (a slice of a panel's segments, sized
+// by entry count for load balance).  The panel's rows of x are staged in LDS
+// (fp32 for the bulk operator) and its row accumulators y_i live in LDS.
+// Groups of M lanes (lane v = vector v, M = launch width) walk a segment:
+// the gather y_i += H_ij x_j goes to LDS; the scatter sum s = sum_i H_ij x_i
+// stays in a register and leaves as one M-wide atomic add into y_j -- one
+// global atomic per segment instead of one per entry.  The panel's y rows
+// are flushed with atomics at the end.  x and y pass through row-major
+// staging so that both the staging loads and the atomics are line-sized.
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
-#include <chrono>
+
+#include "gci_internal.h"
 
 namespace gci {
 
 namespace {
 
-__global__ void k_zero(int64_t n, int m, double *__restrict__ y, int ldy) {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    int64_t total = n * m;
-    for (; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t v = i / n, r = i - v * n;
-        y[r + v * (int64_t)ldy] = 0.0;
+// ------------------------------------------------------ panel-CSC build
+
+__global__ void k_keys(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int R,
+                       uint64_t *__restrict__ keys, int64_t *__restrict__ idx) {
+    // one wave per row
+    const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const uint64_t panel = (uint64_t)(row / R), lr = (uint64_t)(row % R);
+    for (int64_t e = rp[row] + lane; e < rp[row + 1]; e += 64) {
+        keys[e] = (panel << 40) | ((uint64_t)col[e] << 16) | lr;
+        idx[e] = e;
     }
 }
+
+__global__ void k_seg_flags(int64_t nnz, const uint64_t *__restrict__ keys, int32_t *__restrict__ flag) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x)
+        flag[e] = (e == 0 || (keys[e] >> 16) != (keys[e - 1] >> 16)) ? 1 : 0;
+}
+
+__global__ void k_seg_fill(int64_t nnz, const uint64_t *__restrict__ keys, const int32_t *__restrict__ flag,
+                           const int64_t *__restrict__ segid, const int64_t *__restrict__ perm,
+                           const double *__restrict__ val, const float *__restrict__ hi, const float *__restrict__ lo,
+                           int32_t *__restrict__ seg_col, int64_t *__restrict__ seg_start,
+                           uint16_t *__restrict__ lrow, double *__restrict__ pval, float *__restrict__ phi,
+                           float *__restrict__ plo) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[e];
+        if (flag[e]) {
+            const int64_t s = segid[e];
+            seg_col[s] = (int32_t)((k >> 16) & 0xFFFFFFull);
+            seg_start[s] = e;
+        }
+        lrow[e] = (uint16_t)(k & 0xFFFFull);
+        const int64_t src = perm[e];
+        if (pval) pval[e] = val[src];
+        if (phi) {
+            phi[e] = hi[src];
+            plo[e] = lo[src];
+        }
+    }
+}
+
+// ------------------------------------------------------------ kernels
 
 template <int MODE>
-struct Val;
+struct XT;
 template <>
-struct Val<GCI_OP_FP64> {
-    const double *__restrict__ v;
-    __device__ double a(int64_t e) const { return v[e]; }
-    __device__ static double xin(double x) { return x; }
+struct XT<GCI_OP_FP64> {
+    using T = double;
 };
 template <>
-struct Val<GCI_OP_SPLIT_BULK> {
-    const float *__restrict__ hi;
-    __device__ double a(int64_t e) const { return (double)hi[e]; }
-    // fp32-rounded operand; the product of two fp32 values is exact in FP64
-    __device__ static double xin(double x) { return (double)(float)x; }
+struct XT<GCI_OP_SPLIT_BULK> {
+    using T = float;
 };
 template <>
-struct Val<GCI_OP_SPLIT_FULL> {
-    const float *__restrict__ hi;
-    const float *__restrict__ lo;
-    __device__ double a(int64_t e) const { return (double)hi[e] + (double)lo[e]; }
-    __device__ static double xin(double x) { return x; }
+struct XT<GCI_OP_SPLIT_FULL> {
+    using T = double;
 };
 
-template <int G, int M, int MODE>
-__global__ __launch_bounds__(256) void k_spmm_sym(int64_t n, const int64_t *__restrict__ rp,
-                                                  const int32_t *__restrict__ col, Val<MODE> val,
-                                                  const double *__restrict__ diag,
-                                                  const double *__restrict__ x, int ldx,
-                                                  double *__restrict__ y, int ldy) {
-    const int lane = threadIdx.x % G;
-    const int64_t group = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
-    const int64_t ngroups = (int64_t)gridDim.x * blockDim.x / G;
-    for (int64_t r = group; r < n; r += ngroups) {
-        double xr[M], acc[M];
-#pragma unroll
-        for (int v = 0; v < M; ++v) {
-            xr[v] = x[r + (int64_t)v * ldx];
-            acc[v] = 0.0;
-        }
-        const int64_t beg = rp[r], end = rp[r + 1];
-        for (int64_t e = beg + lane; e < end; e += G) {
-            const int64_t c = col[e];
-            const double a = val.a(e);
-#pragma unroll
-            for (int v = 0; v < M; ++v) {
-                acc[v] = fma(a, Val<MODE>::xin(x[c + (int64_t)v * ldx]), acc[v]);
-                unsafeAtomicAdd(&y[c + (int64_t)v * ldy], a * Val<MODE>::xin(xr[v]));
-            }
-        }
-#pragma unroll
-        for (int v = 0; v < M; ++v) {
-#pragma unroll
-            for (int off = G / 2; off > 0; off >>= 1) acc[v] += __shfl_xor(acc[v], off, G);
-        }
-        if (lane == 0) {
-            const double d = diag[r];
-#pragma unroll
-            for (int v = 0; v < M; ++v) unsafeAtomicAdd(&y[r + (int64_t)v * ldy], fma(d, xr[v], acc[v]));
-        }
-    }
-}
-
-template <int G, int M, int MODE>
-void launch_g(const HMat &h, const double *x, int ldx, double *y, int ldy, hipStream_t s) {
-    const int block = 256;
-    int64_t groups_needed = h.n;
-    int64_t blocks = (groups_needed * G + block - 1) / block;
-    blocks = std::min<int64_t>(blocks, 256 * 16);
-    if (blocks < 1) blocks = 1;
-    Val<MODE> vv;
-    if constexpr (MODE == GCI_OP_FP64) {
-        vv.v = h.val.p;
-    } else if constexpr (MODE == GCI_OP_SPLIT_BULK) {
-        vv.hi = h.hi.p;
-    } else {
-        vv.hi = h.hi.p;
-        vv.lo = h.lo.p;
-    }
-    hipLaunchKernelGGL((k_spmm_sym<G, M, MODE>), dim3((unsigned)blocks), dim3(block), 0, s, h.n, h.rp.p,
-                       h.col.p, vv, h.diag.p, x, ldx, y, ldy);
+template <int MODE>
+__device__ inline double entry(const SpmmJob &j, int64_t e) {
+    if constexpr (MODE == GCI_OP_FP64) return j.val[e];
+    else if constexpr (MODE == GCI_OP_SPLIT_BULK) return (double)j.hi[e];
+    else return (double)j.hi[e] + (double)j.lo[e];
 }
 
 template <int M, int MODE>
-void launch_m(const HMat &h, int G, const double *x, int ldx, double *y, int ldy, hipStream_t s) {
-    switch (G) {
-        case 4: launch_g<4, M, MODE>(h, x, ldx, y, ldy, s); break;
-        case 8: launch_g<8, M, MODE>(h, x, ldx, y, ldy, s); break;
-        case 16: launch_g<16, M, MODE>(h, x, ldx, y, ldy, s); break;
-        case 32: launch_g<32, M, MODE>(h, x, ldx, y, ldy, s); break;
-        default: launch_g<64, M, MODE>(h, x, ldx, y, ldy, s); break;
+__global__ __launch_bounds__(256) void k_pcsc(const SpmmItem *__restrict__ items, const SpmmJob *__restrict__ jobs) {
+    using T = typename XT<MODE>::T;
+    extern __shared__ __align__(16) char smem[];
+    const SpmmItem it = items[blockIdx.x];
+    const SpmmJob jb = jobs[it.job];
+    if (!jb.active || jb.mode != MODE) return;
+    const int rows = it.rows;
+    T *xs = reinterpret_cast<T *>(smem);
+    double *ys = reinterpret_cast<double *>(smem + (((size_t)rows * M * sizeof(T) + 15) & ~(size_t)15));
+    const T *gx = reinterpret_cast<const T *>(jb.xs);
+    const int t = threadIdx.x;
+    for (int i = t; i < rows * M; i += blockDim.x) {
+        xs[i] = gx[it.row0 * M + i];
+        ys[i] = 0.0;
     }
+    __syncthreads();
+    const int v = t % M;
+    const int g = t / M, ng = blockDim.x / M;
+    for (int64_t s = it.seg0 + g; s < it.seg1; s += ng) {
+        const int64_t c = jb.seg_col[s];
+        const int64_t e0 = jb.seg_start[s], e1 = jb.seg_start[s + 1];
+        const double xc = (double)gx[c * M + v];
+        double acc = 0.0;
+        int64_t e = e0;
+        for (; e + 4 <= e1; e += 4) {
+            int lr[4];
+            double a[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                lr[q] = jb.lrow[e + q];
+                a[q] = entry<MODE>(jb, e + q);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                acc = fma(a[q], (double)xs[lr[q] * M + v], acc);
+                atomicAdd(&ys[lr[q] * M + v], a[q] * xc);
+            }
+        }
+        for (; e < e1; ++e) {
+            const int lr = jb.lrow[e];
+            const double a = entry<MODE>(jb, e);
+            acc = fma(a, (double)xs[lr * M + v], acc);
+            atomicAdd(&ys[lr * M + v], a * xc);
+        }
+        if (v < jb.m) unsafeAtomicAdd(&jb.ys[c * M + v], acc);
+    }
+    __syncthreads();
+    for (int i = t; i < rows * M; i += blockDim.x) {
+        const double y = ys[i];
+        if (y != 0.0) unsafeAtomicAdd(&jb.ys[it.row0 * M + i], y);
+    }
+}
+
+// staging: xs[i*M + v] = x[i + v*ldx] (rounded to T), ys = 0
+template <class T>
+__global__ void k_pack(int64_t n, int m, int M, const double *__restrict__ x, int64_t ldx, T *__restrict__ xs,
+                       double *__restrict__ ys) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n * M; q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = q / M;
+        const int v = (int)(q % M);
+        xs[q] = v < m ? (T)x[i + v * ldx] : (T)0;
+        ys[q] = 0.0;
+    }
+}
+
+// y[i + v*ldy] = ys[i*M + v] + d_i x[i + v*ldx]  (diagonal in FP64)
+__global__ void k_unpack(int64_t n, int m, int M, const double *__restrict__ ys, const double *__restrict__ diag,
+                         const double *__restrict__ x, int64_t ldx, double *__restrict__ y, int64_t ldy) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n * m; q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = q % n;
+        const int v = (int)(q / n);
+        y[i + v * ldy] = fma(diag[i], x[i + v * ldx], ys[i * M + v]);
+    }
+}
+
+template <int M, int MODE>
+void launch_items(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, int R, hipStream_t s) {
+    using T = typename XT<MODE>::T;
+    const size_t shm = (((size_t)R * M * sizeof(T) + 15) & ~(size_t)15) + (size_t)R * M * sizeof(double);
+    static bool attr_set = false;  // one instantiation per (M, MODE)
+    if (!attr_set) {
+        GCI_HIP(hipFuncSetAttribute((const void *)k_pcsc<M, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)shm));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_pcsc<M, MODE>), dim3((unsigned)nitems), dim3(256), shm, s, d_items, d_jobs);
 }
 
 template <int MODE>
-void launch_mode(const HMat &h, int G, int m, const double *x, int ldx, double *y, int ldy,
-                 hipStream_t s) {
-    int done = 0;
-    while (done < m) {
-        int left = m - done;
-        const double *xs = x + (int64_t)done * ldx;
-        double *ys = y + (int64_t)done * ldy;
-        if (left >= 8) { launch_m<8, MODE>(h, G, xs, ldx, ys, ldy, s); done += 8; }
-        else if (left >= 4) { launch_m<4, MODE>(h, G, xs, ldx, ys, ldy, s); done += 4; }
-        else if (left >= 2) { launch_m<2, MODE>(h, G, xs, ldx, ys, ldy, s); done += 2; }
-        else { launch_m<1, MODE>(h, G, xs, ldx, ys, ldy, s); done += 1; }
+void launch_mode(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, int M, int R, hipStream_t s) {
+    switch (M) {
+        case 1: launch_items<1, MODE>(d_items, nitems, d_jobs, R, s); break;
+        case 2: launch_items<2, MODE>(d_items, nitems, d_jobs, R, s); break;
+        case 4: launch_items<4, MODE>(d_items, nitems, d_jobs, R, s); break;
+        default: launch_items<8, MODE>(d_items, nitems, d_jobs, R, s); break;
     }
 }
 
-int default_group(const HMat &h) {
-    double avg = h.n ? (double)h.nnz / (double)h.n : 0.0;
-    if (avg < 6) return 4;
-    if (avg < 12) return 8;
-    if (avg < 24) return 16;
-    if (avg < 48) return 32;
-    return 64;
-}
+int g_R = 0;  // panel height all matrices use (set by ensure_pcsc)
 
 }  // namespace
 
-void spmm(const HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode,
-          hipStream_t s) {
+int spmm_width(int m) { return m <= 1 ? 1 : m <= 2 ? 2 : m <= 4 ? 4 : 8; }
+
+void spmm_items(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, int M, int mode, hipStream_t s) {
+    if (nitems <= 0) return;
+    const int R = g_R > 0 ? g_R : 1024;
+    switch (mode) {
+        case GCI_OP_FP64: launch_mode<GCI_OP_FP64>(d_items, nitems, d_jobs, M, R, s); break;
+        case GCI_OP_SPLIT_BULK: launch_mode<GCI_OP_SPLIT_BULK>(d_items, nitems, d_jobs, M, R, s); break;
+        default: launch_mode<GCI_OP_SPLIT_FULL>(d_items, nitems, d_jobs, M, R, s); break;
+    }
+}
+
+void ensure_pcsc(HMat &h, int R, int64_t target) {
+    if (h.pc.R) return;
+    GCI_REQUIRE(R >= 1 && R <= 65536, "pcsc: panel height out of range");
+    GCI_REQUIRE(g_R == 0 || g_R == R, "pcsc: all matrices share one panel height");
+    g_R = R;
+    Pcsc &pc = h.pc;
+    const int64_t nnz = h.nnz, n = h.n;
+    hipStream_t s = h.stream;
+    pc.nnz = nnz;
+    if (nnz > 0) {
+        DBuf<uint64_t> k0(nnz), k1(nnz);
+        DBuf<int64_t> i0(nnz), i1(nnz);
+        const int64_t blocks = (n * 64 + 255) / 256;
+        hipLaunchKernelGGL(k_keys, dim3((unsigned)blocks), dim3(256), 0, s, n, h.rp.p, h.col.p, R, k0.p, i0.p);
+        GCI_HIP(hipGetLastError());
+        size_t tb = 0;
+        GCI_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0.p, k1.p, i0.p, i1.p, nnz, 0, 64, s));
+        {
+            DBuf<char> tmp(tb);
+            GCI_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0.p, k1.p, i0.p, i1.p, nnz, 0, 64, s));
+        }
+        k0.release();
+        i0.release();
+        DBuf<int32_t> flag(nnz);
+        DBuf<int64_t> segid(nnz + 1);
+        const int64_t eb = std::min<int64_t>((nnz + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_seg_flags, dim3((unsigned)eb), dim3(256), 0, s, nnz, k1.p, flag.p);
+        tb = 0;
+        GCI_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flag.p, segid.p, nnz, s));
+        {
+            DBuf<char> tmp(tb);
+            GCI_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, flag.p, segid.p, nnz, s));
+        }
+        int64_t last = 0;
+        int32_t lastf = 0;
+        GCI_HIP(hipMemcpyAsync(&last, segid.p + nnz - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        GCI_HIP(hipMemcpyAsync(&lastf, flag.p + nnz - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GCI_HIP(hipStreamSynchronize(s));
+        pc.nseg = last + lastf;
+        pc.seg_col.alloc(pc.nseg);
+        pc.seg_start.alloc(pc.nseg + 1);
+        pc.lrow.alloc(nnz);
+        if (h.val.p) pc.val.alloc(nnz);
+        if (h.precision == GCI_SPLIT32) {
+            pc.hi.alloc(nnz);
+            pc.lo.alloc(nnz);
+        }
+        hipLaunchKernelGGL(k_seg_fill, dim3((unsigned)eb), dim3(256), 0, s, nnz, k1.p, flag.p, segid.p, i1.p,
+                           h.val.p, h.hi.p, h.lo.p, pc.seg_col.p, pc.seg_start.p, pc.lrow.p, pc.val.p, pc.hi.p,
+                           pc.lo.p);
+        GCI_HIP(hipGetLastError());
+        GCI_HIP(hipMemcpyAsync(pc.seg_start.p + pc.nseg, &nnz, sizeof(int64_t), hipMemcpyHostToDevice, s));
+        GCI_HIP(hipStreamSynchronize(s));
+    }
+    // work items: panels cut into slices of about `target` entries
+    std::vector<int64_t> ss(pc.nseg + 1);
+    std::vector<int32_t> sc(pc.nseg);
+    if (pc.nseg) {
+        GCI_HIP(hipMemcpy(ss.data(), pc.seg_start.p, (pc.nseg + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+        GCI_HIP(hipMemcpy(sc.data(), pc.seg_col.p, pc.nseg * sizeof(int32_t), hipMemcpyDeviceToHost));
+    } else {
+        ss[0] = 0;
+    }
+    // segment -> panel: segments are sorted by (panel, column); recover the
+    // panel of each segment from its first entry's row via the CSR
+    std::vector<int64_t> rp(n + 1);
+    GCI_HIP(hipMemcpy(rp.data(), h.rp.p, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+    const int64_t npan = (n + R - 1) / R;
+    std::vector<int64_t> panel_nnz(npan, 0);
+    for (int64_t p = 0; p < npan; ++p) panel_nnz[p] = rp[std::min(n, (p + 1) * R)] - rp[p * R];
+    pc.items.clear();
+    int64_t seg = 0;
+    for (int64_t p = 0; p < npan; ++p) {
+        const int64_t row0 = p * R;
+        const int rows = (int)std::min<int64_t>(R, n - row0);
+        const int64_t end_e = ss[seg] + panel_nnz[p];  // entries of this panel are contiguous
+        int64_t s0 = seg;
+        while (seg < pc.nseg && ss[seg] < end_e) {
+            ++seg;
+            if (ss[seg] - ss[s0] >= target || seg == pc.nseg || ss[seg] >= end_e) {
+                pc.items.push_back(SpmmItem{s0, seg, row0, rows, 0});
+                s0 = seg;
+            }
+        }
+    }
+    GCI_REQUIRE(seg == pc.nseg, "pcsc: segment/panel bookkeeping mismatch");
+}
+
+void spmm(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode, hipStream_t s) {
     GCI_REQUIRE(m >= 1, "spmm: m < 1");
     GCI_REQUIRE(ldx >= h.n && ldy >= h.n, "spmm: leading dimension < n");
     if (mode != GCI_OP_FP64)
@@ -156,52 +307,44 @@ void spmm(const HMat &h, int m, const double *x, int ldx, double *y, int ldy, in
     else
         GCI_REQUIRE(h.val.p != nullptr, "spmm: FP64 operator on a matrix without FP64 values");
     if (h.n == 0) return;
-    const int G = h.group ? h.group : default_group(h);
-    int64_t zb = std::min<int64_t>((h.n * m + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_zero, dim3((unsigned)zb), dim3(256), 0, s, h.n, m, y, ldy);
-    switch (mode) {
-        case GCI_OP_FP64: launch_mode<GCI_OP_FP64>(h, G, m, x, ldx, y, ldy, s); break;
-        case GCI_OP_SPLIT_BULK: launch_mode<GCI_OP_SPLIT_BULK>(h, G, m, x, ldx, y, ldy, s); break;
-        case GCI_OP_SPLIT_FULL: launch_mode<GCI_OP_SPLIT_FULL>(h, G, m, x, ldx, y, ldy, s); break;
-        default: GCI_REQUIRE(false, "spmm: unknown mode");
-    }
-    GCI_HIP(hipGetLastError());
-}
-
-// Row-group autotuner: times each candidate G on the matrix itself (x = 1)
-// and keeps the fastest.  Runs once per matrix and operator width.
-int spmm_group(HMat &h, int m, int mode) {
-    if (h.group) return h.group;
-    if (h.n == 0 || h.nnz == 0) {
-        h.group = 4;
-        return h.group;
-    }
-    DBuf<double> x((size_t)h.n * m), y((size_t)h.n * m);
-    std::vector<double> ones((size_t)h.n * m, 1.0);
-    GCI_HIP(hipMemcpyAsync(x.p, ones.data(), x.bytes(), hipMemcpyHostToDevice, h.stream));
-    hipEvent_t e0, e1;
-    GCI_HIP(hipEventCreate(&e0));
-    GCI_HIP(hipEventCreate(&e1));
-    int best = default_group(h);
-    float best_ms = 1e30f;
-    for (int G : {4, 8, 16, 32, 64}) {
-        h.group = G;
-        spmm(h, m, x.p, (int)h.n, y.p, (int)h.n, mode, h.stream);  // warm
-        GCI_HIP(hipEventRecord(e0, h.stream));
-        for (int rep = 0; rep < 3; ++rep) spmm(h, m, x.p, (int)h.n, y.p, (int)h.n, mode, h.stream);
-        GCI_HIP(hipEventRecord(e1, h.stream));
-        GCI_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        GCI_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = G;
+    ensure_pcsc(h);
+    for (int c0 = 0; c0 < m; c0 += 8) {
+        const int mc = std::min(8, m - c0);
+        const int M = spmm_width(mc);
+        const size_t xb = mode == GCI_OP_SPLIT_BULK ? sizeof(float) : sizeof(double);
+        DBuf<char> xs((size_t)h.n * M * xb);
+        DBuf<double> ys((size_t)h.n * M);
+        const double *xc = x + (int64_t)c0 * ldx;
+        double *yc = y + (int64_t)c0 * ldy;
+        const int64_t pb = std::min<int64_t>((h.n * M + 255) / 256, 65536);
+        if (mode == GCI_OP_SPLIT_BULK)
+            hipLaunchKernelGGL(k_pack<float>, dim3((unsigned)pb), dim3(256), 0, s, h.n, mc, M, xc, (int64_t)ldx,
+                               (float *)xs.p, ys.p);
+        else
+            hipLaunchKernelGGL(k_pack<double>, dim3((unsigned)pb), dim3(256), 0, s, h.n, mc, M, xc, (int64_t)ldx,
+                               (double *)xs.p, ys.p);
+        if (!h.pc.items.empty()) {
+            SpmmJob jb{h.pc.seg_col.p, h.pc.seg_start.p, h.pc.lrow.p, h.pc.val.p, h.pc.hi.p, h.pc.lo.p, xs.p, ys.p,
+                       mc, mode, 1, 0};
+            DBuf<SpmmJob> dj(1);
+            DBuf<SpmmItem> di(h.pc.items.size());
+            GCI_HIP(hipMemcpyAsync(dj.p, &jb, sizeof(SpmmJob), hipMemcpyHostToDevice, s));
+            GCI_HIP(hipMemcpyAsync(di.p, h.pc.items.data(), h.pc.items.size() * sizeof(SpmmItem),
+                                   hipMemcpyHostToDevice, s));
+            spmm_items(di.p, (int64_t)h.pc.items.size(), dj.p, M, mode, s);
+            GCI_HIP(hipGetLastError());
+            const int64_t ub = std::min<int64_t>((h.n * mc + 255) / 256, 65536);
+            hipLaunchKernelGGL(k_unpack, dim3((unsigned)ub), dim3(256), 0, s, h.n, mc, M, (const double *)ys.p,
+                               (const double *)h.diag.p, xc, (int64_t)ldx, yc, (int64_t)ldy);
+            GCI_HIP(hipStreamSynchronize(s));  // staging buffers are released on return
+        } else {
+            const int64_t ub = std::min<int64_t>((h.n * mc + 255) / 256, 65536);
+            hipLaunchKernelGGL(k_unpack, dim3((unsigned)ub), dim3(256), 0, s, h.n, mc, M, (const double *)ys.p,
+                               (const double *)h.diag.p, xc, (int64_t)ldx, yc, (int64_t)ldy);
+            GCI_HIP(hipStreamSynchronize(s));
         }
+        GCI_HIP(hipGetLastError());
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    h.group = best;
-    return best;
 }
 
 }  // namespace gci

@@ -206,6 +206,17 @@ int gci_server_poll(gci_server *s, int64_t req, double *eval, double *evec, doub
 /* Block until every submitted request has finished. */
 int gci_server_drain(gci_server *s);
 
+/* --------------------------------------------- operator profiling
+ * No counterpart in the reference (which only times whole programs with
+ * STARTTIME/STOPTIME): per operator mode (GCI_OP_*), the number of symmetric
+ * SpMM kernel launches made by the Davidson engines, their summed kernel
+ * time in milliseconds (HIP events on the launching stream) and the summed
+ * algorithmic bytes of the blocks they served.  While enabled, engine
+ * iterations are launched directly instead of from a captured graph.
+ * stats: 9 doubles, [launches, ms, bytes] x 3 modes. */
+int gci_profile_enable(int on);
+int gci_profile_read(double *stats);
+
 #ifdef __cplusplus
 }
 #endif
