@@ -7,6 +7,7 @@ namespace gci {
 constexpr int MAXLIM = 80;    // largest LIM (2*nvec+60 for nvec <= 10)
 constexpr int MAXNUME = 16;   // largest number of wanted eigenpairs
 constexpr int CHUNK = 512;    // rows per tall-skinny work item
+constexpr int RESULT_STRIDE = 5 + 2 * MAXNUME;
 
 // Per-problem stage.  Kernels act only on problems in the stage they serve.
 enum : int {
@@ -105,6 +106,12 @@ class DavidsonEngine {
     bool needs_refine(int slot) const;
     void extract(int slot, double *eval, double *evec, double *resid, gci_dvd_stats *st);
     void release(int slot);
+    void release_all();
+    // compact per-slot results: [nloops, nmv, ierr, refined, bulk_loops,
+    // eigval(16), resn(16)] (RESULT_STRIDE doubles per slot)
+    void extract_all(std::vector<double> &res);
+    void copy_vectors(int slot, double *evec);  // async on the engine stream
+    void sync() { GCI_HIP(hipStreamSynchronize(stream_)); }
     int active() const;
     int capacity() const { return cap_; }
     int phase(int slot) const { return phase_[slot]; }
@@ -131,6 +138,7 @@ class DavidsonEngine {
     std::vector<DvdSpec> spec_;
     std::vector<int> phase_;               // last polled phases
     std::vector<char> refined_;
+    std::vector<int> niv_;
     std::vector<DBuf<double>> work_;
     DBuf<DvdProb> dprob_;
     DBuf<int> chunk_prob_;

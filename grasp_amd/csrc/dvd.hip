@@ -778,6 +778,81 @@ __global__ __launch_bounds__(256) void k_copy_diag(DvdProb *P, const int *chunk_
     for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) p.diag[r] = p.diagH[r];
 }
 
+// Unit initial vectors on the NIV smallest diagonal entries (ties by
+// index) for slots flagged in want[]: NIV argmin passes over the diagonal.
+__global__ __launch_bounds__(256) void k_init_unit(DvdProb *P, const int *want) {
+    DvdProb &p = P[blockIdx.x];
+    if (!want[blockIdx.x]) return;
+    __shared__ double bv[256];
+    __shared__ int64_t bi[256];
+    __shared__ int64_t picked[MAXLIM];
+    const int t = threadIdx.x;
+    for (int j = 0; j < p.niv; ++j) {
+        double best = 1.0e300;
+        int64_t bidx = -1;
+        for (int64_t i = t; i < p.n; i += blockDim.x) {
+            bool used = false;
+            for (int q = 0; q < j; ++q) used |= (picked[q] == i);
+            const double d = p.diagH[i];
+            if (!used && (d < best || (d == best && (bidx < 0 || i < bidx)))) {
+                best = d;
+                bidx = i;
+            }
+        }
+        bv[t] = best;
+        bi[t] = bidx;
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if (t < off) {
+                const double o = bv[t + off];
+                const int64_t oi = bi[t + off];
+                if (oi >= 0 && (bi[t] < 0 || o < bv[t] || (o == bv[t] && oi < bi[t]))) {
+                    bv[t] = o;
+                    bi[t] = oi;
+                }
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            picked[j] = bi[0];
+            p.basis[bi[0] + (int64_t)j * p.n] = 1.0;
+        }
+        __syncthreads();
+    }
+}
+
+struct ChunkMeta {
+    int64_t chunk0;
+    int nchunks;
+    int pad;
+};
+
+__global__ void k_set_chunks(DvdProb *P, int nprob, const ChunkMeta *meta) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprob || P[i].phase == PH_EMPTY) return;
+    P[i].chunk0 = meta[i].chunk0;
+    P[i].nchunks = meta[i].nchunks;
+}
+
+// results of every slot in one compact array: per slot RSTRIDE doubles:
+// [nloops, nmv, ierr, refine, bulk_loops, eigval(MAXNUME), resn(MAXNUME)]
+constexpr int RSTRIDE = RESULT_STRIDE;
+__global__ void k_gather_results(const DvdProb *P, int nprob, double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprob) return;
+    const DvdProb &p = P[i];
+    double *o = out + (size_t)i * RSTRIDE;
+    o[0] = p.nloops;
+    o[1] = p.nmv;
+    o[2] = p.ierr;
+    o[3] = p.refine_phase;
+    o[4] = p.bulk_loops;
+    for (int q = 0; q < MAXNUME; ++q) {
+        o[5 + q] = q < p.nume ? p.eigval[q] : 0.0;
+        o[5 + MAXNUME + q] = q < p.nume ? p.resn[q] : 0.0;
+    }
+}
+
 template <class K, class... Args>
 void launch(K kern, int grid, int block, hipStream_t s, Args... args) {
     if (grid <= 0) return;
@@ -793,6 +868,7 @@ DavidsonEngine::DavidsonEngine(int capacity, hipStream_t s) : cap_(capacity), st
     spec_.resize(cap_);
     phase_.assign(cap_, PH_EMPTY);
     refined_.assign(cap_, 0);
+    niv_.assign(cap_, 0);
     work_.resize(cap_);
     dprob_.alloc(cap_);
     dphase_.alloc(cap_);
@@ -866,24 +942,18 @@ void DavidsonEngine::rebuild_chunks() {
     if (nitems_)
         GCI_HIP(hipMemcpyAsync(ditems_.p, items.data(), nitems_ * sizeof(SpmmItem), hipMemcpyHostToDevice, stream_));
     update_modes();
-    for (int i = 0; i < cap_; ++i) {
-        if (phase_[i] == PH_EMPTY) continue;
-        // chunk0 and nchunks are adjacent fields of DvdProb
-        struct {
-            int64_t c0;
-            int nc;
-        } v{ch[i].first, ch[i].second};
-        GCI_HIP(hipMemcpyAsync(&dprob_.p[i].chunk0, &v.c0, sizeof(int64_t), hipMemcpyHostToDevice, stream_));
-        GCI_HIP(hipMemcpyAsync(&dprob_.p[i].nchunks, &v.nc, sizeof(int), hipMemcpyHostToDevice, stream_));
-        GCI_HIP(hipStreamSynchronize(stream_));
-    }
+    std::vector<ChunkMeta> meta(cap_);
+    for (int i = 0; i < cap_; ++i) meta[i] = ChunkMeta{ch[i].first, ch[i].second, 0};
+    DBuf<ChunkMeta> dmeta(cap_);
+    GCI_HIP(hipMemcpyAsync(dmeta.p, meta.data(), cap_ * sizeof(ChunkMeta), hipMemcpyHostToDevice, stream_));
+    launch(k_set_chunks, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, (const ChunkMeta *)dmeta.p);
     GCI_HIP(hipStreamSynchronize(stream_));
     graph_valid_ = false;
 }
 
 std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const std::vector<const double *> &init,
                                        const std::vector<int> &niv) {
-    std::vector<int> slots;
+    std::vector<int> slots, unit_init;
     for (size_t q = 0; q < specs.size(); ++q) {
         const DvdSpec &sp = specs[q];
         GCI_REQUIRE(sp.h && sp.h->n >= 2, "davidson: matrix order < 2");
@@ -916,7 +986,8 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         s2.maxiter = maxiter;
         spec_[slot] = s2;
         const_cast<HMat *>(sp.h)->pc.R == 0 ? ensure_pcsc(*const_cast<HMat *>(sp.h)) : (void)0;
-        work_[slot].alloc((size_t)n * (2 * lim + 1 + 2 * nume + 16));
+        const size_t need = (size_t)n * (2 * lim + 1 + 2 * nume + 16);
+        if (work_[slot].n < need) work_[slot].alloc(need);  // kept across releases for reuse
         double *w = work_[slot].p;
         DvdProb p;
         std::memset(&p, 0, sizeof(p));
@@ -954,28 +1025,26 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         p.slot = slot;
         p.phase = PH_INIT;
         p.niv = niv[q];
-        GCI_HIP(hipMemsetAsync(w, 0, work_[slot].bytes(), stream_));
+        GCI_HIP(hipMemsetAsync(w, 0, need * sizeof(double), stream_));
         GCI_HIP(hipMemcpyAsync(p.diag, sp.h->diag.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-        std::vector<double> iv;
-        const double *src = init[q];
-        if (!src) {
-            // unit vectors on the niv smallest diagonal elements
-            std::vector<double> d(n);
-            GCI_HIP(hipMemcpyAsync(d.data(), sp.h->diag.p, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
-            GCI_HIP(hipStreamSynchronize(stream_));
-            std::vector<int64_t> idx(n);
-            std::iota(idx.begin(), idx.end(), 0);
-            std::partial_sort(idx.begin(), idx.begin() + niv[q], idx.end(),
-                              [&](int64_t a, int64_t b) { return d[a] < d[b] || (d[a] == d[b] && a < b); });
-            iv.assign((size_t)n * niv[q], 0.0);
-            for (int j = 0; j < niv[q]; ++j) iv[(size_t)j * n + idx[j]] = 1.0;
-            src = iv.data();
-        }
-        GCI_HIP(hipMemcpyAsync(p.basis, src, (size_t)n * niv[q] * sizeof(double), hipMemcpyHostToDevice, stream_));
+        if (init[q])
+            GCI_HIP(hipMemcpyAsync(p.basis, init[q], (size_t)n * niv[q] * sizeof(double), hipMemcpyHostToDevice,
+                                   stream_));
+        else
+            unit_init.push_back(slot);
         GCI_HIP(hipMemcpyAsync(&dprob_.p[slot], &p, sizeof(DvdProb), hipMemcpyHostToDevice, stream_));
-        GCI_HIP(hipStreamSynchronize(stream_));
+        GCI_HIP(hipStreamSynchronize(stream_));  // p lives on this stack frame
+        niv_[slot] = niv[q];
         phase_[slot] = PH_INIT;
         refined_[slot] = 0;
+    }
+    if (!unit_init.empty()) {
+        std::vector<int> want(cap_, 0);
+        for (int sl : unit_init) want[sl] = 1;
+        DBuf<int> dw(cap_);
+        GCI_HIP(hipMemcpyAsync(dw.p, want.data(), cap_ * sizeof(int), hipMemcpyHostToDevice, stream_));
+        launch(k_init_unit, cap_, 256, stream_, dprob_.p, (const int *)dw.p);
+        GCI_HIP(hipStreamSynchronize(stream_));
     }
     rebuild_chunks();
     run_initdvd(slots);
@@ -990,15 +1059,13 @@ void DavidsonEngine::run_initdvd(const std::vector<int> &slots) {
     DBuf<InitStep> dsteps(cap_);
     std::vector<InitStep> steps(cap_, InitStep{0, 0, 0});
     std::vector<int> ist(cap_, 1), iend(cap_, 0), nume(cap_, 0);
-    std::vector<DvdProb> cur(1);
     int init_width = 1;
     update_modes();
     for (int sl : slots) {
-        GCI_HIP(hipMemcpy(cur.data(), &dprob_.p[sl], sizeof(DvdProb), hipMemcpyDeviceToHost));
-        iend[sl] = cur[0].niv;
-        nume[sl] = cur[0].nume;
-        init_width = std::max(init_width, std::max(cur[0].niv, cur[0].nume));
-        steps[sl] = InitStep{1, 0, cur[0].niv};
+        iend[sl] = niv_[sl];
+        nume[sl] = spec_[sl].nume;
+        init_width = std::max(init_width, std::max(niv_[sl], spec_[sl].nume));
+        steps[sl] = InitStep{1, 0, niv_[sl]};
     }
     auto push = [&]() {
         GCI_HIP(hipMemcpyAsync(dsteps.p, steps.data(), cap_ * sizeof(InitStep), hipMemcpyHostToDevice, stream_));
@@ -1190,6 +1257,7 @@ void DavidsonEngine::refine_restart(const std::vector<int> &slots) {
         launch(k_refine_begin, 1, 1, stream_, dprob_.p, sl, sp.tol_final, sp.critr_final > 0 ? sp.critr_final : sp.critr);
         refined_[sl] = 1;
         phase_[sl] = PH_INIT;
+        niv_[sl] = sp.nume;
     }
     update_modes();
     launch(k_copy_diag, (int)total_chunks_, 256, stream_, dprob_.p, (const int *)chunk_prob_.p,
@@ -1214,12 +1282,37 @@ void DavidsonEngine::extract(int slot, double *eval, double *evec, double *resid
     }
 }
 
+void DavidsonEngine::extract_all(std::vector<double> &res) {
+    DBuf<double> d((size_t)cap_ * RSTRIDE);
+    launch(k_gather_results, (cap_ + 63) / 64, 64, stream_, (const DvdProb *)dprob_.p, cap_, d.p);
+    res.resize((size_t)cap_ * RSTRIDE);
+    GCI_HIP(hipMemcpyAsync(res.data(), d.p, d.bytes(), hipMemcpyDeviceToHost, stream_));
+    GCI_HIP(hipStreamSynchronize(stream_));
+}
+
+void DavidsonEngine::copy_vectors(int slot, double *evec) {
+    const int64_t n = spec_[slot].h->n;
+    GCI_HIP(hipMemcpyAsync(evec, work_[slot].p, (size_t)n * spec_[slot].nume * sizeof(double),
+                           hipMemcpyDeviceToHost, stream_));
+}
+
+void DavidsonEngine::release_all() {
+    GCI_HIP(hipStreamSynchronize(stream_));
+    std::vector<int> empty_ph(cap_, PH_EMPTY);
+    GCI_HIP(hipMemcpy2D(&dprob_.p[0].phase, sizeof(DvdProb), empty_ph.data(), sizeof(int), sizeof(int), cap_,
+                        hipMemcpyHostToDevice));
+    for (int i = 0; i < cap_; ++i) {
+        spec_[i] = DvdSpec{};
+        phase_[i] = PH_EMPTY;
+    }
+    rebuild_chunks();
+}
+
 void DavidsonEngine::release(int slot) {
     GCI_REQUIRE(slot >= 0 && slot < cap_ && phase_[slot] != PH_EMPTY, "davidson: bad slot");
     GCI_HIP(hipStreamSynchronize(stream_));
     const int empty = PH_EMPTY;
     GCI_HIP(hipMemcpy(&dprob_.p[slot].phase, &empty, sizeof(int), hipMemcpyHostToDevice));
-    work_[slot].release();
     spec_[slot] = DvdSpec{};
     phase_[slot] = PH_EMPTY;
     rebuild_chunks();

@@ -2,6 +2,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <memory>
+#include <mutex>
 
 #include "dvd.h"
 
@@ -186,8 +188,16 @@ int gci_batch_solve(int nblocks, gci_hmat **blocks, const int *nev, const double
     return guarded([&] {
         GCI_REQUIRE(nblocks >= 1 && blocks && nev && tol && eval && eval_off, "gci_batch_solve: null argument");
         const auto t0 = std::chrono::steady_clock::now();
-        hipStream_t s = blocks[0]->stream;
-        DavidsonEngine eng(nblocks, s);
+        // one engine (and its workspace) is kept for the process and reused
+        // by later batches of up to the same number of blocks
+        static std::mutex mu;
+        static std::unique_ptr<DavidsonEngine> eng_cache;
+        static hipStream_t stream = nullptr;
+        std::lock_guard<std::mutex> lk(mu);
+        if (!stream) GCI_HIP(hipStreamCreate(&stream));
+        if (!eng_cache || eng_cache->capacity() < nblocks)
+            eng_cache = std::make_unique<DavidsonEngine>(std::max(nblocks, 64), stream);
+        DavidsonEngine &eng = *eng_cache;
         std::vector<DvdSpec> specs(nblocks);
         std::vector<const double *> init(nblocks, nullptr);
         std::vector<int> niv(nblocks);
@@ -212,19 +222,40 @@ int gci_batch_solve(int nblocks, gci_hmat **blocks, const int *nev, const double
             niv[b] = nev[b];
             guard = std::max(guard, 2 * sp.maxiter + 128);
         }
-        eng.admit(specs, init, niv);
-        run_to_completion(eng, guard);
-        int64_t eo = 0, vo = 0;
-        for (int b = 0; b < nblocks; ++b) {
-            eval_off[b] = eo;
-            if (evec_off) evec_off[b] = vo;
-            gci_dvd_stats st{};
-            eng.extract(b, eval + eo, evec ? evec + vo : nullptr, nullptr, &st);
-            st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            if (stats) stats[b] = st;
-            eo += nev[b];
-            vo += blocks[b]->n * nev[b];
+        std::vector<int> slots;
+        try {
+            slots = eng.admit(specs, init, niv);
+            run_to_completion(eng, guard);
+            std::vector<double> res;
+            eng.extract_all(res);
+            int64_t eo = 0, vo = 0;
+            for (int b = 0; b < nblocks; ++b) {
+                const double *r = res.data() + (size_t)slots[b] * RESULT_STRIDE;
+                eval_off[b] = eo;
+                if (evec_off) evec_off[b] = vo;
+                for (int q = 0; q < nev[b]; ++q) eval[eo + q] = r[5 + q];
+                if (evec) eng.copy_vectors(slots[b], evec + vo);
+                if (stats) {
+                    gci_dvd_stats st{};
+                    const bool refined = r[3] != 0.0;
+                    st.nloops = (int)(refined ? r[4] + r[0] : r[0]);
+                    st.nmv = (int)r[1];
+                    st.ierr = (int)r[2];
+                    st.refine_loops = refined ? (int)r[0] : 0;
+                    stats[b] = st;
+                }
+                eo += nev[b];
+                vo += blocks[b]->n * nev[b];
+            }
+            eng.sync();
+            const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (stats)
+                for (int b = 0; b < nblocks; ++b) stats[b].seconds = secs;
+        } catch (...) {
+            eng_cache.reset();  // state unknown after a failure
+            throw;
         }
+        eng.release_all();
     });
 }
 

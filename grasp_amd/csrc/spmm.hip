@@ -297,6 +297,7 @@ void ensure_pcsc(HMat &h, int R, int64_t target) {
         }
     }
     GCI_REQUIRE(seg == pc.nseg, "pcsc: segment/panel bookkeeping mismatch");
+    pc.R = R;  // built
 }
 
 void spmm(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode, hipStream_t s) {
