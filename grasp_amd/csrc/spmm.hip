@@ -89,16 +89,75 @@ struct XT<GCI_OP_SPLIT_FULL> {
     using T = double;
 };
 
+// Broadcast lane Q of each group of M lanes to the whole group: DPP quad
+// permutes for M <= 4 (a VALU move, no LDS traffic), ds_bpermute for M = 8.
+template <int M, int Q>
+__device__ inline int bcast(int x) {
+    if constexpr (M == 1) {
+        return x;
+    } else if constexpr (M == 2) {
+        constexpr int ctrl = Q == 0 ? 0xA0 : 0xF5;  // quad_perm [0,0,2,2] / [1,1,3,3]
+        return __builtin_amdgcn_mov_dpp(x, ctrl, 0xF, 0xF, false);
+    } else if constexpr (M == 4) {
+        constexpr int ctrl = Q | (Q << 2) | (Q << 4) | (Q << 6);  // quad_perm [Q,Q,Q,Q]
+        return __builtin_amdgcn_mov_dpp(x, ctrl, 0xF, 0xF, false);
+    } else {
+        return __shfl(x, (int)((threadIdx.x & 63) & ~7u) | Q, 64);
+    }
+}
+
+// One stored entry as it travels through the group: its row offset and its
+// value in the operator's precision (two 32-bit words for FP64 and hi+lo).
 template <int MODE>
-__device__ inline double entry(const SpmmJob &j, int64_t e) {
-    if constexpr (MODE == GCI_OP_FP64) return j.val[e];
-    else if constexpr (MODE == GCI_OP_SPLIT_BULK) return (double)j.hi[e];
-    else return (double)j.hi[e] + (double)j.lo[e];
+struct Ent {
+    int lr;
+    int w0, w1;
+    __device__ inline void load(const SpmmJob &j, int64_t e) {
+        lr = j.lrow[e];
+        if constexpr (MODE == GCI_OP_FP64) {
+            const double v = j.val[e];
+            w0 = __double2loint(v);
+            w1 = __double2hiint(v);
+        } else if constexpr (MODE == GCI_OP_SPLIT_BULK) {
+            w0 = __float_as_int(j.hi[e]);
+            w1 = 0;
+        } else {
+            w0 = __float_as_int(j.hi[e]);
+            w1 = __float_as_int(j.lo[e]);
+        }
+    }
+    __device__ inline double value() const {
+        if constexpr (MODE == GCI_OP_FP64) return __hiloint2double(w1, w0);
+        else if constexpr (MODE == GCI_OP_SPLIT_BULK) return (double)__int_as_float(w0);
+        else return (double)__int_as_float(w0) + (double)__int_as_float(w1);
+    }
+    template <int M, int Q>
+    __device__ inline Ent from_lane() const {
+        Ent r;
+        r.lr = bcast<M, Q>(lr);
+        r.w0 = bcast<M, Q>(w0);
+        if constexpr (MODE != GCI_OP_SPLIT_BULK) r.w1 = bcast<M, Q>(w1);
+        else r.w1 = 0;
+        return r;
+    }
+};
+
+template <int M, int MODE, int U, int Q>
+__device__ inline void consume(const Ent<MODE> (&en)[U], int u, const typename XT<MODE>::T *xs, double *ys,
+                               int v, double xc, double &acc) {
+    const Ent<MODE> q = en[u].template from_lane<M, Q>();
+    if (q.lr >= 0) {
+        const double a = q.value();
+        acc = fma(a, (double)xs[q.lr * M + v], acc);
+        atomicAdd(&ys[q.lr * M + v], a * xc);
+    }
+    if constexpr (Q + 1 < M) consume<M, MODE, U, Q + 1>(en, u, xs, ys, v, xc, acc);
 }
 
 template <int M, int MODE>
 __global__ __launch_bounds__(256) void k_pcsc(const SpmmItem *__restrict__ items, const SpmmJob *__restrict__ jobs) {
     using T = typename XT<MODE>::T;
+    constexpr int U = 4;  // entry batches each lane keeps in flight
     extern __shared__ __align__(16) char smem[];
     const SpmmItem it = items[blockIdx.x];
     const SpmmJob jb = jobs[it.job];
@@ -115,31 +174,34 @@ __global__ __launch_bounds__(256) void k_pcsc(const SpmmItem *__restrict__ items
     __syncthreads();
     const int v = t % M;
     const int g = t / M, ng = blockDim.x / M;
-    for (int64_t s = it.seg0 + g; s < it.seg1; s += ng) {
-        const int64_t c = jb.seg_col[s];
-        const int64_t e0 = jb.seg_start[s], e1 = jb.seg_start[s + 1];
+    // segment metadata is fetched one segment ahead
+    int64_t s = it.seg0 + g;
+    int64_t cn = 0, e0n = 0, e1n = 0;
+    if (s < it.seg1) {
+        cn = jb.seg_col[s];
+        e0n = jb.seg_start[s];
+        e1n = jb.seg_start[s + 1];
+    }
+    for (; s < it.seg1; s += ng) {
+        const int64_t c = cn, e0 = e0n, e1 = e1n;
+        const int64_t sn = s + ng;
+        if (sn < it.seg1) {
+            cn = jb.seg_col[sn];
+            e0n = jb.seg_start[sn];
+            e1n = jb.seg_start[sn + 1];
+        }
         const double xc = (double)gx[c * M + v];
         double acc = 0.0;
-        int64_t e = e0;
-        for (; e + 4 <= e1; e += 4) {
-            int lr[4];
-            double a[4];
+        for (int64_t e = e0; e < e1; e += U * M) {
+            Ent<MODE> en[U];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                lr[q] = jb.lrow[e + q];
-                a[q] = entry<MODE>(jb, e + q);
+            for (int u = 0; u < U; ++u) {
+                const int64_t idx = e + u * M + v;  // lanes of the group load consecutive entries
+                if (idx < e1) en[u].load(jb, idx);
+                else en[u] = Ent<MODE>{-1, 0, 0};
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                acc = fma(a[q], (double)xs[lr[q] * M + v], acc);
-                atomicAdd(&ys[lr[q] * M + v], a[q] * xc);
-            }
-        }
-        for (; e < e1; ++e) {
-            const int lr = jb.lrow[e];
-            const double a = entry<MODE>(jb, e);
-            acc = fma(a, (double)xs[lr * M + v], acc);
-            atomicAdd(&ys[lr * M + v], a * xc);
+            for (int u = 0; u < U; ++u) consume<M, MODE, U, 0>(en, u, xs, ys, v, xc, acc);
         }
         if (v < jb.m) unsafeAtomicAdd(&jb.ys[c * M + v], acc);
     }
