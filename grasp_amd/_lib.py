@@ -47,6 +47,11 @@ class GciPairLists(C.Structure):
                 ("nterms", C.c_int64)]
 
 
+class GciSpmmShape(C.Structure):
+    _fields_ = [("R", C.c_int), ("L", C.c_int), ("block", C.c_int), ("nseg", C.c_int), ("depth", C.c_int),
+                ("target", C.c_int64)]
+
+
 class GciDvdParams(C.Structure):
     _fields_ = [("nvec", C.c_int), ("lim", C.c_int), ("mblock", C.c_int), ("maxiter", C.c_int),
                 ("crite", C.c_double), ("critc", C.c_double), ("critr", C.c_double),
@@ -64,7 +69,8 @@ EXPORTS = [
     "gci_hmat_upload", "gci_hmat_free", "gci_hmat_info", "gci_hmat_download", "gci_spmm",
     "gci_spmm_device", "gci_davidson", "gci_maneig", "gci_batch_solve", "gci_server_create",
     "gci_server_destroy", "gci_server_submit", "gci_server_poll", "gci_server_drain",
-    "gci_profile_enable", "gci_profile_read",
+    "gci_profile_enable", "gci_profile_read", "gci_spmm_get_shape", "gci_spmm_set_shape", "gci_spmm_time",
+    "gci_spmm_autotune",
 ]
 
 

@@ -155,6 +155,7 @@ class DavidsonEngine {
     std::vector<int64_t> hrow_;
     hipGraphExec_t gexec_ = nullptr;
     bool graph_valid_ = false;
+    uint64_t shape_version_ = 0;  // SpMM shape the captured graph was built for
 };
 
 // Operator profiling (gci_profile_*): per operator mode, launches, summed

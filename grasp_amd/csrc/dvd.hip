@@ -643,8 +643,8 @@ __device__ inline int op_mode(const DvdProb &p) {
     return p.refine_phase ? (p.mode == GCI_OP_FP64 ? GCI_OP_FP64 : GCI_OP_SPLIT_FULL) : p.mode;
 }
 
-// columns [c0, c0 + 8) of the NNEW new vectors (the staging holds 8)
-__device__ inline int op_cols(const DvdProb &p, int c0) { return min(8, p.nnew - c0); }
+// columns [c0, c0 + cb) of the NNEW new vectors (cb <= 8, the staging width)
+__device__ inline int op_cols(const DvdProb &p, int c0, int cb) { return min(cb, p.nnew - c0); }
 
 // algorithmic bytes of one operator application (m vectors): matrix values
 // and indices, row pointers, x and y (x as fp32 for the bulk operator)
@@ -654,12 +654,13 @@ __device__ inline double op_alg_bytes(const DvdProb &p, int m, int mode) {
     return nnz * 12.0 + (n + 1) * 8.0 + 2.0 * n * m * 8.0;
 }
 
-__global__ void k_make_jobs(DvdProb *P, int nprob, SpmmJob *jobs, int want_phase, int c0, double *alg_bytes) {
+__global__ void k_make_jobs(DvdProb *P, int nprob, SpmmJob *jobs, int want_phase, int c0, int cb,
+                            double *alg_bytes) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nprob) return;
     const DvdProb &p = P[i];
     SpmmJob j{};
-    j.active = (p.phase == want_phase && op_cols(p, c0) > 0) ? 1 : 0;
+    j.active = (p.phase == want_phase && op_cols(p, c0, cb) > 0) ? 1 : 0;
     if (j.active) {
         j.seg_col = p.pseg_col;
         j.seg_start = p.pseg_start;
@@ -669,7 +670,7 @@ __global__ void k_make_jobs(DvdProb *P, int nprob, SpmmJob *jobs, int want_phase
         j.lo = p.plo;
         j.xs = p.xs;
         j.ys = p.ys;
-        j.m = op_cols(p, c0);
+        j.m = op_cols(p, c0, cb);
         j.mode = op_mode(p);
         if (alg_bytes) atomicAdd(&alg_bytes[j.mode], op_alg_bytes(p, j.m, j.mode));
     }
@@ -677,9 +678,9 @@ __global__ void k_make_jobs(DvdProb *P, int nprob, SpmmJob *jobs, int want_phase
 }
 
 __global__ __launch_bounds__(256) void k_pack_slots(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
-                                                    int want_phase, int M, int c0) {
+                                                    int want_phase, int M, int c0, int cb) {
     DvdProb &p = P[chunk_prob[blockIdx.x]];
-    const int m = op_cols(p, c0);
+    const int m = op_cols(p, c0, cb);
     if (p.phase != want_phase || m <= 0) return;
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
     const double *X = p.basis + (int64_t)(p.kpass + c0) * p.n;
@@ -695,9 +696,9 @@ __global__ __launch_bounds__(256) void k_pack_slots(DvdProb *P, const int *chunk
 }
 
 __global__ __launch_bounds__(256) void k_unpack_slots(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
-                                                      int want_phase, int M, int c0) {
+                                                      int want_phase, int M, int c0, int cb) {
     DvdProb &p = P[chunk_prob[blockIdx.x]];
-    const int m = min(op_cols(p, c0), M);
+    const int m = min(op_cols(p, c0, cb), M);
     if (p.phase != want_phase || m <= 0) return;
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
     const double *X = p.basis + (int64_t)(p.kpass + c0) * p.n;
@@ -932,6 +933,7 @@ void DavidsonEngine::rebuild_chunks() {
     std::vector<SpmmItem> items;
     for (int i = 0; i < cap_; ++i) {
         if (phase_[i] == PH_EMPTY) continue;
+        recut_items(*const_cast<HMat *>(spec_[i].h));  // items follow the launch shape in force
         for (SpmmItem it : spec_[i].h->pc.items) {
             it.job = i;
             items.push_back(it);
@@ -985,7 +987,7 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         s2.lim = lim;
         s2.maxiter = maxiter;
         spec_[slot] = s2;
-        const_cast<HMat *>(sp.h)->pc.R == 0 ? ensure_pcsc(*const_cast<HMat *>(sp.h)) : (void)0;
+        ensure_items(*const_cast<HMat *>(sp.h));
         const size_t need = (size_t)n * (2 * lim + 1 + 2 * nume + 16);
         if (work_[slot].n < need) work_[slot].alloc(need);  // kept across releases for reuse
         double *w = work_[slot].p;
@@ -1132,9 +1134,10 @@ void DavidsonEngine::update_modes() {
         else modes |= 1 << sp.mode;
         mw = std::max(mw, sp.mblock);
     }
-    if (modes != modes_used_ || mw != mwidth_iter_) graph_valid_ = false;
+    if (modes != modes_used_ || mw != mwidth_iter_ || spmm_shape_version() != shape_version_) graph_valid_ = false;
     modes_used_ = modes;
     mwidth_iter_ = mw;
+    shape_version_ = spmm_shape_version();
 }
 
 // maxcols: upper bound of NNEW over the problems in the stage
@@ -1142,12 +1145,16 @@ void DavidsonEngine::launch_op(int phase, int maxcols) {
     const int G = (int)total_chunks_;
     const int *cp = chunk_prob_.p;
     const int64_t *cr = chunk_row_.p;
-    for (int c0 = 0; c0 < maxcols; c0 += 8) {
-        const int M = spmm_width(std::min(8, maxcols - c0));
+    // column blocks as wide as every operator mode in use fits in LDS
+    int cb = 8;
+    for (int mode = 0; mode < 3; ++mode)
+        if (modes_used_ & (1 << mode)) cb = std::min(cb, spmm_max_width(mode));
+    for (int c0 = 0; c0 < maxcols; c0 += cb) {
+        const int M = spmm_width(std::min(cb, maxcols - c0));
         OpProfile &pf = op_profile();
-        launch(k_make_jobs, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, djobs_.p, phase, c0,
+        launch(k_make_jobs, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, djobs_.p, phase, c0, cb,
                pf.on ? op_bytes_device() : (double *)nullptr);
-        launch(k_pack_slots, G, 256, stream_, dprob_.p, cp, cr, phase, M, c0);
+        launch(k_pack_slots, G, 256, stream_, dprob_.p, cp, cr, phase, M, c0, cb);
         for (int mode = 0; mode < 3; ++mode) {
             if (!(modes_used_ & (1 << mode))) continue;
             hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1162,7 +1169,7 @@ void DavidsonEngine::launch_op(int phase, int maxcols) {
                 prof_events_.push_back({mode, e0, e1});
             }
         }
-        launch(k_unpack_slots, G, 256, stream_, dprob_.p, cp, cr, phase, M, c0);
+        launch(k_unpack_slots, G, 256, stream_, dprob_.p, cp, cr, phase, M, c0, cb);
     }
 }
 

@@ -91,17 +91,42 @@ struct SpmmItem {
     int32_t job;         // owning job (filled per launch)
 };
 
+struct SpmmJob;
+
+// Launch shape of the SpMM: panel height R (a power of two; the layout is
+// built for it), lanes per segment group L, workgroup size, entries per work
+// item, segments per LDS chunk, and prefetch depth.
+struct SpmmShape {
+    int R = 1024;
+    int L = 4;
+    int block = 512;
+    int64_t target = 65536;
+    int nseg = 512;   // segments per chunk (their tables are staged in LDS)
+    int depth = 3;    // batches each lane keeps in flight
+};
+
 struct Pcsc {
-    int R = 0;                 // rows per panel (<= 65536)
+    int R = 0;                 // rows per panel (0 = not built)
     int64_t nseg = 0, nnz = 0;
+    int64_t npad = 0;          // stored entries including segment padding
+    int64_t target = 0;        // entries per work item the items were cut for
     DBuf<int32_t> seg_col;     // nseg
-    DBuf<int64_t> seg_start;   // nseg + 1
-    DBuf<uint16_t> lrow;       // nnz, row - panel start
-    DBuf<double> val;          // nnz (FP64 values)
-    DBuf<float> hi, lo;        // nnz (split values)
+    DBuf<int64_t> seg_start;   // nseg + 1, multiples of 4
+    DBuf<uint16_t> lrow;       // npad, row - panel start
+    DBuf<double> val;          // npad (FP64 values)
+    DBuf<float> hi, lo;        // npad (split values)
+    std::vector<int64_t> seg_start_h;  // host copies for cutting work items
+    std::vector<int32_t> seg_panel_h;
     std::vector<SpmmItem> items;  // host copy of the work items
+    // single-matrix operator state: staging (n x stage_M), job table, items
+    DBuf<char> stage_x;
+    DBuf<double> stage_y;
+    int stage_M = 0;
+    DBuf<SpmmJob> d_jobs;
+    DBuf<SpmmItem> d_items;
     size_t bytes() const {
-        return seg_col.bytes() + seg_start.bytes() + lrow.bytes() + val.bytes() + hi.bytes() + lo.bytes();
+        return seg_col.bytes() + seg_start.bytes() + lrow.bytes() + val.bytes() + hi.bytes() + lo.bytes() +
+               stage_x.bytes() + stage_y.bytes();
     }
 };
 
@@ -138,8 +163,31 @@ struct SpmmJob {
     int m, mode, active, pad;
 };
 
-// Builds h.pc (idempotent).  R = rows per panel, target = entries per item.
-void ensure_pcsc(HMat &h, int R = 1024, int64_t target = 131072);
+// The launch shape in force (defaults, overridable once through the
+// GCI_SPMM_R / _L / _BLOCK / _TARGET / _NSEG / _DEPTH environment variables) and its
+// setter.
+SpmmShape spmm_shape();
+void set_spmm_shape(const SpmmShape &sh);
+bool valid_shape(const SpmmShape &sh);
+// Builds h.pc for the current shape (idempotent).
+void ensure_pcsc(HMat &h);
+// Lays the matrix out again if it was built for another panel height,
+// ensure_pcsc, then recut_items.
+void ensure_items(HMat &h);
+// Re-cuts the work items if the shape's entries per item changed since they
+// were cut (drops the device copy of the items); the panel height must match.
+void recut_items(HMat &h);
+// LDS bytes one workgroup of the kernel needs for M columns in `mode`, and
+// the per-workgroup LDS limit.
+size_t spmm_lds_bytes(const SpmmShape &sh, int M, int mode);
+size_t spmm_lds_max();
+// Widest launch width (8, 4, 2, 1) whose LDS fits the shape in `mode`.
+int spmm_max_width(int mode);
+// Incremented by every set_spmm_shape (engines re-capture their graphs).
+uint64_t spmm_shape_version();
+// Re-cuts h.pc's work items for about `target` entries per item, inside one
+// panel (host only).
+void build_items(HMat &h, int64_t target);
 // Launch width (1, 2, 4 or 8) for m columns.
 int spmm_width(int m);
 // The panel-CSC kernel over a list of work items whose jobs live in device

@@ -7,27 +7,42 @@
 //
 // Layout (panel-CSC, built once per matrix on the device): rows are cut into
 // panels of R rows; inside a panel the entries are sorted by column, then
-// row, so each column the panel touches is one contiguous segment.  Per entry
-// the stream carries only the fp32/fp64 value and a 16-bit row offset.
+// row, so each column the panel touches is one contiguous segment.  Segments
+// are padded to a multiple of 4 entries (pad entries: value 0, row of the
+// segment's last entry) so that every lane reads whole aligned quads.  Per
+// entry the stream carries only the value (fp32 hi, hi+lo or fp64) and a
+// 16-bit row offset; per segment its column and start.
 //
-// Kernel: one workgroup per work item (a slice of a panel's segments, sized
-// by entry count for load balance).  The panel's rows of x are staged in LDS
-// (fp32 for the bulk operator) and its row accumulators y_i live in LDS.
-// Groups of M lanes (lane v = vector v, M = launch width) walk a segment:
-// the gather y_i += H_ij x_j goes to LDS; the scatter sum s = sum_i H_ij x_i
-// stays in a register and leaves as one M-wide atomic add into y_j -- one
-// global atomic per segment instead of one per entry.  The panel's y rows
-// are flushed with atomics at the end.  x and y pass through row-major
-// staging so that both the staging loads and the atomics are line-sized.
+// Kernel: one workgroup per work item (a slice of one panel's segments).
+// The panel's rows of x are staged in LDS (fp32 for the bulk operator) and
+// its row accumulators y_i (fp64) live in LDS.  Groups of L lanes walk
+// segments; lane l of a group owns entries 4l..4l+3 of each 4L-entry batch
+// and applies each entry to all M vectors: one LDS vector read of x_i, M
+// FMAs into the lane's column partials, M LDS atomics into y_i.  At a
+// segment's end the group reduces its partials (DPP) and issues one M-wide
+// atomic add into y_j.  Loads of the next batch, the next segment's x_j and
+// the metadata two segments ahead are issued before the current batch is
+// consumed.  x and y pass through row-major staging buffers so that the
+// x_j reads and the y atomics are line-sized.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "gci_internal.h"
 
 namespace gci {
 
 namespace {
+
+#define GAS __attribute__((address_space(1)))
+
+constexpr size_t LDS_MAX = 160 * 1024;  // LDS per workgroup on gfx950
+
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef unsigned u2v __attribute__((ext_vector_type(2)));
 
 // ------------------------------------------------------ panel-CSC build
 
@@ -49,26 +64,57 @@ __global__ void k_seg_flags(int64_t nnz, const uint64_t *__restrict__ keys, int3
         flag[e] = (e == 0 || (keys[e] >> 16) != (keys[e - 1] >> 16)) ? 1 : 0;
 }
 
-__global__ void k_seg_fill(int64_t nnz, const uint64_t *__restrict__ keys, const int32_t *__restrict__ flag,
-                           const int64_t *__restrict__ segid, const int64_t *__restrict__ perm,
-                           const double *__restrict__ val, const float *__restrict__ hi, const float *__restrict__ lo,
-                           int32_t *__restrict__ seg_col, int64_t *__restrict__ seg_start,
-                           uint16_t *__restrict__ lrow, double *__restrict__ pval, float *__restrict__ phi,
-                           float *__restrict__ plo) {
+// segment s of entry e = (exclusive count of flags before e) + flag[e] - 1
+__device__ inline int64_t seg_of(const int32_t *flag, const int64_t *segid, int64_t e) {
+    return segid[e] + flag[e] - 1;
+}
+
+__global__ void k_seg_meta(int64_t nnz, const uint64_t *__restrict__ keys, const int32_t *__restrict__ flag,
+                           const int64_t *__restrict__ segid, int32_t *__restrict__ seg_col,
+                           int64_t *__restrict__ ustart, int32_t *__restrict__ seg_panel) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+        if (!flag[e]) continue;
         const uint64_t k = keys[e];
-        if (flag[e]) {
-            const int64_t s = segid[e];
-            seg_col[s] = (int32_t)((k >> 16) & 0xFFFFFFull);
-            seg_start[s] = e;
+        const int64_t s = segid[e];
+        seg_col[s] = (int32_t)((k >> 16) & 0xFFFFFFull);
+        seg_panel[s] = (int32_t)(k >> 40);
+        ustart[s] = e;
+    }
+}
+
+// padded length of every segment (multiple of 4); plen[nseg] = 0
+__global__ void k_seg_plen(int64_t nseg, int64_t nnz, const int64_t *__restrict__ ustart, int64_t *__restrict__ plen) {
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s <= nseg; s += (int64_t)gridDim.x * blockDim.x) {
+        if (s == nseg) {
+            plen[s] = 0;
+        } else {
+            const int64_t len = (s + 1 < nseg ? ustart[s + 1] : nnz) - ustart[s];
+            plen[s] = (len + 3) & ~(int64_t)3;
         }
-        lrow[e] = (uint16_t)(k & 0xFFFFull);
+    }
+}
+
+__global__ void k_seg_fill(int64_t nnz, int64_t nseg, const uint64_t *__restrict__ keys,
+                           const int32_t *__restrict__ flag, const int64_t *__restrict__ segid,
+                           const int64_t *__restrict__ ustart, const int64_t *__restrict__ pstart,
+                           const int64_t *__restrict__ perm, const double *__restrict__ val,
+                           const float *__restrict__ hi, const float *__restrict__ lo, uint16_t *__restrict__ lrow,
+                           double *__restrict__ pval, float *__restrict__ phi, float *__restrict__ plo) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = seg_of(flag, segid, e);
+        const int64_t pe = pstart[s] + (e - ustart[s]);
+        const uint16_t lr = (uint16_t)(keys[e] & 0xFFFFull);
+        lrow[pe] = lr;
         const int64_t src = perm[e];
-        if (pval) pval[e] = val[src];
+        if (pval) pval[pe] = val[src];
         if (phi) {
-            phi[e] = hi[src];
-            plo[e] = lo[src];
+            phi[pe] = hi[src];
+            plo[pe] = lo[src];
         }
+        // the segment's last entry also writes the pad rows (values stay 0)
+        const int64_t uend = s + 1 < nseg ? ustart[s + 1] : nnz;
+        if (e == uend - 1)
+            for (int64_t q = pe + 1; q < pstart[s + 1]; ++q) lrow[q] = lr;
     }
 }
 
@@ -89,126 +135,336 @@ struct XT<GCI_OP_SPLIT_FULL> {
     using T = double;
 };
 
-// Broadcast lane Q of each group of M lanes to the whole group: DPP quad
-// permutes for M <= 4 (a VALU move, no LDS traffic), ds_bpermute for M = 8.
-template <int M, int Q>
-__device__ inline int bcast(int x) {
-    if constexpr (M == 1) {
-        return x;
-    } else if constexpr (M == 2) {
-        constexpr int ctrl = Q == 0 ? 0xA0 : 0xF5;  // quad_perm [0,0,2,2] / [1,1,3,3]
-        return __builtin_amdgcn_mov_dpp(x, ctrl, 0xF, 0xF, false);
-    } else if constexpr (M == 4) {
-        constexpr int ctrl = Q | (Q << 2) | (Q << 4) | (Q << 6);  // quad_perm [Q,Q,Q,Q]
-        return __builtin_amdgcn_mov_dpp(x, ctrl, 0xF, 0xF, false);
+// Reinterpret a pointer as a pointer to vector type V in the same address
+// space (LDS/generic, or global).
+template <class V, class T>
+__device__ inline const V *vcast(const T *p) {
+    return reinterpret_cast<const V *>(p);
+}
+template <class V, class T>
+__device__ inline const GAS V *vcast(const GAS T *p) {
+    return reinterpret_cast<const GAS V *>(p);
+}
+
+// M consecutive values of type T (M * sizeof(T) bytes, naturally aligned)
+// converted to double; one vector access per 16 bytes.
+template <int M, class T, class P>
+__device__ inline void load_row(P p, double (&o)[M]) {
+    if constexpr (std::is_same<T, float>::value) {
+        if constexpr (M == 1) {
+            o[0] = (double)p[0];
+        } else if constexpr (M == 2) {
+            const f2v w = vcast<f2v>(p)[0];
+            o[0] = (double)w.x;
+            o[1] = (double)w.y;
+        } else {
+#pragma unroll
+            for (int q = 0; q < M / 4; ++q) {
+                const f4v w = vcast<f4v>(p)[q];
+                o[4 * q + 0] = (double)w.x;
+                o[4 * q + 1] = (double)w.y;
+                o[4 * q + 2] = (double)w.z;
+                o[4 * q + 3] = (double)w.w;
+            }
+        }
     } else {
-        return __shfl(x, (int)((threadIdx.x & 63) & ~7u) | Q, 64);
+        if constexpr (M == 1) {
+            o[0] = p[0];
+        } else {
+#pragma unroll
+            for (int q = 0; q < M / 2; ++q) {
+                const d2v w = vcast<d2v>(p)[q];
+                o[2 * q + 0] = w.x;
+                o[2 * q + 1] = w.y;
+            }
+        }
     }
 }
 
-// One stored entry as it travels through the group: its row offset and its
-// value in the operator's precision (two 32-bit words for FP64 and hi+lo).
+// Uniform (per work item) views of the matrix stream, as global-address-
+// space pointers offset to the item's first entry, so that every per-lane
+// address is a 32-bit offset from a scalar base.
+struct ItemPtrs {
+    const GAS u2v *rows;   // 4 row offsets (uint16) per quad
+    const GAS f4v *hi;     // 4 fp32 values per quad
+    const GAS f4v *lo;
+    const GAS d2v *val;    // 2 fp64 values per half quad
+};
+
+// Four consecutive entries of one lane.  settle() marks the registers as
+// read on every path (an empty asm statement with them as inputs), so the
+// wait for their load is placed before the branch that may skip them and
+// the wait state is the same on all paths.
 template <int MODE>
-struct Ent {
-    int lr;
-    int w0, w1;
-    __device__ inline void load(const SpmmJob &j, int64_t e) {
-        lr = j.lrow[e];
-        if constexpr (MODE == GCI_OP_FP64) {
-            const double v = j.val[e];
-            w0 = __double2loint(v);
-            w1 = __double2hiint(v);
-        } else if constexpr (MODE == GCI_OP_SPLIT_BULK) {
-            w0 = __float_as_int(j.hi[e]);
-            w1 = 0;
-        } else {
-            w0 = __float_as_int(j.hi[e]);
-            w1 = __float_as_int(j.lo[e]);
+struct Quad;
+
+template <>
+struct Quad<GCI_OP_SPLIT_BULK> {
+    u2v r;
+    f4v h;
+    __device__ inline void load(const ItemPtrs &p, uint32_t q) {
+        r = __builtin_nontemporal_load(p.rows + q);
+        h = __builtin_nontemporal_load(p.hi + q);
+    }
+    __device__ inline double value(int j) const { return (double)h[j]; }
+    __device__ inline void settle() const { asm volatile("" ::"v"(r), "v"(h)); }
+};
+
+template <>
+struct Quad<GCI_OP_SPLIT_FULL> {
+    u2v r;
+    f4v h, l;
+    __device__ inline void load(const ItemPtrs &p, uint32_t q) {
+        r = __builtin_nontemporal_load(p.rows + q);
+        h = __builtin_nontemporal_load(p.hi + q);
+        l = __builtin_nontemporal_load(p.lo + q);
+    }
+    __device__ inline double value(int j) const { return (double)h[j] + (double)l[j]; }
+    __device__ inline void settle() const { asm volatile("" ::"v"(r), "v"(h), "v"(l)); }
+};
+
+template <>
+struct Quad<GCI_OP_FP64> {
+    u2v r;
+    d2v a, b;
+    __device__ inline void load(const ItemPtrs &p, uint32_t q) {
+        r = __builtin_nontemporal_load(p.rows + q);
+        a = __builtin_nontemporal_load(p.val + 2 * q);
+        b = __builtin_nontemporal_load(p.val + 2 * q + 1);
+    }
+    __device__ inline double value(int j) const { return j < 2 ? a[j] : b[j - 2]; }
+    __device__ inline void settle() const { asm volatile("" ::"v"(r), "v"(a), "v"(b)); }
+};
+
+template <int Q>
+__device__ inline double dpp_double(double x) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), Q, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), Q, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the L lanes of a group (L a power of two <= 16), every lane
+// receiving the total; the pairing order is fixed, so the sum is the same in
+// every lane and on every run.
+template <int L>
+__device__ inline double group_sum(double x) {
+    if constexpr (L >= 2) x += dpp_double<0xB1>(x);  // quad_perm [1,0,3,2]
+    if constexpr (L >= 4) x += dpp_double<0x4E>(x);  // quad_perm [2,3,0,1]
+    if constexpr (L >= 8) x += __shfl_xor(x, 4, 64);
+    if constexpr (L >= 16) x += __shfl_xor(x, 8, 64);
+    return x;
+}
+
+template <int M, int MODE, int YS>
+__device__ inline void consume(const Quad<MODE> &q, const typename XT<MODE>::T *xs, double *ys,
+                               const double (&xc)[M], double (&acc)[M]) {
+    using T = typename XT<MODE>::T;
+    int lr[4];
+    lr[0] = (int)(q.r.x & 0xFFFFu);
+    lr[1] = (int)(q.r.x >> 16);
+    lr[2] = (int)(q.r.y & 0xFFFFu);
+    lr[3] = (int)(q.r.y >> 16);
+    double xv[4][M];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) load_row<M, T>(xs + lr[j] * M, xv[j]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const double a = q.value(j);
+        double *yr = ys + lr[j] * YS;
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            acc[v] = fma(a, xv[j][v], acc[v]);
+            __hip_atomic_fetch_add(yr + v, a * xc[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
-    __device__ inline double value() const {
-        if constexpr (MODE == GCI_OP_FP64) return __hiloint2double(w1, w0);
-        else if constexpr (MODE == GCI_OP_SPLIT_BULK) return (double)__int_as_float(w0);
-        else return (double)__int_as_float(w0) + (double)__int_as_float(w1);
+}
+
+// M doubles from registers to LDS, 16 bytes per access
+template <int M>
+__device__ inline void store_row(double *dst, const double (&v)[M]) {
+    if constexpr (M == 1) {
+        dst[0] = v[0];
+    } else {
+#pragma unroll
+        for (int q = 0; q < M / 2; ++q) reinterpret_cast<d2v *>(dst)[q] = d2v{v[2 * q], v[2 * q + 1]};
     }
-    template <int M, int Q>
-    __device__ inline Ent from_lane() const {
-        Ent r;
-        r.lr = bcast<M, Q>(lr);
-        r.w0 = bcast<M, Q>(w0);
-        if constexpr (MODE != GCI_OP_SPLIT_BULK) r.w1 = bcast<M, Q>(w1);
-        else r.w1 = 0;
-        return r;
+}
+
+// raw copy of one row of M values of type T, global -> LDS
+template <int M, class T>
+__device__ inline void copy_row(T *dst, const GAS T *src) {
+    if constexpr (M * sizeof(T) >= 16) {
+#pragma unroll
+        for (int q = 0; q < (int)(M * sizeof(T) / 16); ++q)
+            reinterpret_cast<f4v *>(dst)[q] = vcast<f4v>(src)[q];
+    } else if constexpr (M * sizeof(T) == 8) {
+        *reinterpret_cast<f2v *>(dst) = vcast<f2v>(src)[0];
+    } else {
+        *dst = *src;
+    }
+}
+
+// LDS carve-up of one workgroup (offsets in bytes, 64-byte aligned parts):
+// xs (R x M, T) | ys (R rows of M + 1 fp64: the pad word spreads rows over
+// the LDS banks) | xcol (NS x M, fp64: a segment's x_j row while it runs,
+// then its column sums) | sbeg (NS + 1, int) | scol (NS, int)
+struct LdsMap {
+    size_t ys, xcol, sbeg, scol, total;
+};
+template <int M, class T>
+__host__ __device__ constexpr LdsMap lds_map(int R, int NS) {
+    LdsMap m{};
+    m.ys = ((size_t)R * M * sizeof(T) + 63) & ~(size_t)63;
+    m.xcol = (m.ys + (size_t)R * (M + 1) * sizeof(double) + 63) & ~(size_t)63;
+    m.sbeg = m.xcol + (size_t)NS * M * sizeof(double);
+    m.scol = m.sbeg + (((size_t)(NS + 1) * sizeof(int) + 15) & ~(size_t)15);
+    m.total = m.scol + (size_t)NS * sizeof(int);
+    return m;
+}
+
+// A group's walk over its segments in 4L-entry batches: segment i (item-
+// chunk local), batch start k, segment end e1.
+struct Cursor {
+    int i, k, e1;
+    template <int L, int G>
+    __device__ inline bool advance(const int *sbeg, int nsc) {  // returns true at a segment end
+        if (k + 4 * L < e1) {
+            k += 4 * L;
+            return false;
+        }
+        i += G;
+        if (i < nsc) {
+            k = sbeg[i];
+            e1 = sbeg[i + 1];
+        }
+        return true;
     }
 };
 
-template <int M, int MODE, int U, int Q>
-__device__ inline void consume(const Ent<MODE> (&en)[U], int u, const typename XT<MODE>::T *xs, double *ys,
-                               int v, double xc, double &acc) {
-    const Ent<MODE> q = en[u].template from_lane<M, Q>();
-    if (q.lr >= 0) {
-        const double a = q.value();
-        acc = fma(a, (double)xs[q.lr * M + v], acc);
-        atomicAdd(&ys[q.lr * M + v], a * xc);
-    }
-    if constexpr (Q + 1 < M) consume<M, MODE, U, Q + 1>(en, u, xs, ys, v, xc, acc);
-}
-
-template <int M, int MODE>
-__global__ __launch_bounds__(256) void k_pcsc(const SpmmItem *__restrict__ items, const SpmmJob *__restrict__ jobs) {
+template <int M, int MODE, int L, int BS, int D>
+__global__ __launch_bounds__(BS) void k_pcsc(const SpmmItem *__restrict__ items, const SpmmJob *__restrict__ jobs,
+                                             int R, int NS) {
     using T = typename XT<MODE>::T;
-    constexpr int U = 4;  // entry batches each lane keeps in flight
-    extern __shared__ __align__(16) char smem[];
+    constexpr int G = BS / L;  // lane groups per workgroup
+    constexpr int YS = M + 1;  // ys row stride (doubles)
+    extern __shared__ __align__(64) char smem[];
     const SpmmItem it = items[blockIdx.x];
     const SpmmJob jb = jobs[it.job];
     if (!jb.active || jb.mode != MODE) return;
     const int rows = it.rows;
+    if (rows > R) return;  // never reached: items are cut for the shape's R
+    const LdsMap lm = lds_map<M, T>(R, NS);
     T *xs = reinterpret_cast<T *>(smem);
-    double *ys = reinterpret_cast<double *>(smem + (((size_t)rows * M * sizeof(T) + 15) & ~(size_t)15));
-    const T *gx = reinterpret_cast<const T *>(jb.xs);
+    double *ys = reinterpret_cast<double *>(smem + lm.ys);
+    double *xcol = reinterpret_cast<double *>(smem + lm.xcol);
+    int *sbeg = reinterpret_cast<int *>(smem + lm.sbeg);
+    int *scl = reinterpret_cast<int *>(smem + lm.scol);
+    const GAS T *gx = (const GAS T *)jb.xs;
+    const GAS int32_t *gcol = (const GAS int32_t *)jb.seg_col;
+    const GAS int64_t *gst = (const GAS int64_t *)jb.seg_start;
+    double *gy = jb.ys;
     const int t = threadIdx.x;
-    for (int i = t; i < rows * M; i += blockDim.x) {
-        xs[i] = gx[it.row0 * M + i];
-        ys[i] = 0.0;
-    }
-    __syncthreads();
-    const int v = t % M;
-    const int g = t / M, ng = blockDim.x / M;
-    // segment metadata is fetched one segment ahead
-    int64_t s = it.seg0 + g;
-    int64_t cn = 0, e0n = 0, e1n = 0;
-    if (s < it.seg1) {
-        cn = jb.seg_col[s];
-        e0n = jb.seg_start[s];
-        e1n = jb.seg_start[s + 1];
-    }
-    for (; s < it.seg1; s += ng) {
-        const int64_t c = cn, e0 = e0n, e1 = e1n;
-        const int64_t sn = s + ng;
-        if (sn < it.seg1) {
-            cn = jb.seg_col[sn];
-            e0n = jb.seg_start[sn];
-            e1n = jb.seg_start[sn + 1];
-        }
-        const double xc = (double)gx[c * M + v];
-        double acc = 0.0;
-        for (int64_t e = e0; e < e1; e += U * M) {
-            Ent<MODE> en[U];
+    const int l = t % L, g = t / L;
+    // the panel's x rows and zeroed y rows stay for the whole item
+    for (int r = t; r < rows; r += BS) {
+        copy_row<M, T>(xs + r * M, gx + (it.row0 + r) * M);
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int64_t idx = e + u * M + v;  // lanes of the group load consecutive entries
-                if (idx < e1) en[u].load(jb, idx);
-                else en[u] = Ent<MODE>{-1, 0, 0};
+        for (int v = 0; v < YS; ++v) ys[r * YS + v] = 0.0;
+    }
+    // the item's segments in chunks of at most NS (their tables fill the LDS)
+    for (int64_t cs = it.seg0; cs < it.seg1; cs += NS) {
+        const int nsc = (int)min((int64_t)NS, it.seg1 - cs);
+        const int64_t base = gst[cs];  // first entry of the chunk (a multiple of 4)
+        for (int i = t; i <= nsc; i += BS) {
+            sbeg[i] = (int)(gst[cs + i] - base);
+            if (i < nsc) {
+                const int c = gcol[cs + i];
+                scl[i] = c;
+                double xr[M];
+                load_row<M, T>(gx + (int64_t)c * M, xr);
+#pragma unroll
+                for (int v = 0; v < M; ++v) xcol[i * M + v] = xr[v];
             }
-#pragma unroll
-            for (int u = 0; u < U; ++u) consume<M, MODE, U, 0>(en, u, xs, ys, v, xc, acc);
         }
-        if (v < jb.m) unsafeAtomicAdd(&jb.ys[c * M + v], acc);
+        ItemPtrs P;
+        P.rows = (const GAS u2v *)(jb.lrow + base);
+        P.hi = (const GAS f4v *)(jb.hi + base);
+        P.lo = (const GAS f4v *)(jb.lo + base);
+        P.val = (const GAS d2v *)(jb.val + base);
+        __syncthreads();
+
+        // Main loop.  The load cursor runs D batches ahead of the consume
+        // cursor through a ring of D + 1 register sets, so D batches stay in
+        // flight while one is consumed.  Every step issues its load (clamped
+        // to the chunk's first quad once the group is done), so all paths
+        // carry the same loads and the same waits; the only global accesses
+        // in the loop are these loads.
+        Cursor cc{g, 0, 0}, lc{g, 0, 0};
+        if (g < nsc) {
+            cc.k = lc.k = sbeg[g];
+            cc.e1 = lc.e1 = sbeg[g + 1];
+        }
+        double xc[M], acc[M];
+#pragma unroll
+        for (int v = 0; v < M; ++v) acc[v] = 0.0;
+        if (g < nsc) load_row<M, double>(xcol + g * M, xc);
+        Quad<MODE> q[D + 1];
+        bool ok[D + 1];
+        auto issue = [&](int slot_dummy, Quad<MODE> &dst, bool &dok) {
+            (void)slot_dummy;
+            dok = lc.i < nsc && lc.k + 4 * l < lc.e1;
+            dst.load(P, dok ? (uint32_t)(lc.k + 4 * l) >> 2 : 0u);
+            if (lc.i < nsc) lc.advance<L, G>(sbeg, nsc);
+        };
+#pragma unroll
+        for (int d = 0; d < D; ++d) issue(d, q[d], ok[d]);
+        auto step = [&](Quad<MODE> &cur, bool cur_ok, Quad<MODE> &nxt, bool &nxt_ok) {
+            issue(0, nxt, nxt_ok);
+            cur.settle();
+            if (cur_ok) consume<M, MODE, YS>(cur, xs, ys, xc, acc);
+            if (cc.i < nsc) {
+                const int ci = cc.i;
+                if (cc.advance<L, G>(sbeg, nsc)) {
+                    // segment done: its column sums replace its x_j row
+#pragma unroll
+                    for (int v = 0; v < M; ++v) acc[v] = group_sum<L>(acc[v]);
+                    if (l == 0) store_row<M>(xcol + ci * M, acc);
+#pragma unroll
+                    for (int v = 0; v < M; ++v) acc[v] = 0.0;
+                    if (cc.i < nsc) load_row<M, double>(xcol + cc.i * M, xc);
+                }
+            }
+        };
+        while (cc.i < nsc) {
+            if constexpr (D == 1) {
+                step(q[0], ok[0], q[1], ok[1]);
+                step(q[1], ok[1], q[0], ok[0]);
+            } else if constexpr (D == 2) {
+                step(q[0], ok[0], q[2], ok[2]);
+                step(q[1], ok[1], q[0], ok[0]);
+                step(q[2], ok[2], q[1], ok[1]);
+            } else {
+                step(q[0], ok[0], q[3], ok[3]);
+                step(q[1], ok[1], q[0], ok[0]);
+                step(q[2], ok[2], q[1], ok[1]);
+                step(q[3], ok[3], q[2], ok[2]);
+            }
+        }
+#pragma unroll
+        for (int d = 0; d <= D; ++d) q[d].settle();
+        __syncthreads();
+        // the chunk's column sums, as coalesced atomics
+        for (int idx = t; idx < nsc * M; idx += BS) {
+            const double y = xcol[idx];
+            const int v = idx % M;
+            if (y != 0.0 && v < jb.m) unsafeAtomicAdd(&gy[(int64_t)scl[idx / M] * M + v], y);
+        }
+        __syncthreads();  // the next chunk overwrites the tables
     }
-    __syncthreads();
-    for (int i = t; i < rows * M; i += blockDim.x) {
-        const double y = ys[i];
-        if (y != 0.0) unsafeAtomicAdd(&jb.ys[it.row0 * M + i], y);
+    for (int idx = t; idx < rows * M; idx += BS) {
+        const int r = idx / M, v = idx % M;
+        const double y = ys[r * YS + v];
+        if (y != 0.0 && v < jb.m) unsafeAtomicAdd(&gy[(it.row0 + r) * M + v], y);
     }
 }
 
@@ -234,54 +490,179 @@ __global__ void k_unpack(int64_t n, int m, int M, const double *__restrict__ ys,
     }
 }
 
-template <int M, int MODE>
-void launch_items(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, int R, hipStream_t s) {
+// ------------------------------------------------------ launch shapes
+
+SpmmShape g_shape;         // shape all launches use
+bool g_shape_set = false;
+uint64_t g_shape_version = 1;
+
+template <int M, int MODE, int L, int BS, int D>
+void launch_depth(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh,
+                  hipStream_t s) {
     using T = typename XT<MODE>::T;
-    const size_t shm = (((size_t)R * M * sizeof(T) + 15) & ~(size_t)15) + (size_t)R * M * sizeof(double);
-    static bool attr_set = false;  // one instantiation per (M, MODE)
-    if (!attr_set) {
-        GCI_HIP(hipFuncSetAttribute((const void *)k_pcsc<M, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)shm));
-        attr_set = true;
+    const size_t shm = lds_map<M, T>(sh.R, sh.nseg).total;
+    GCI_REQUIRE(shm <= LDS_MAX, "spmm: panel height and segments per chunk exceed the LDS");
+    static size_t attr = 0;  // one instantiation per template tuple
+    if (shm > attr) {
+        GCI_HIP(hipFuncSetAttribute((const void *)k_pcsc<M, MODE, L, BS, D>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        attr = shm;
     }
-    hipLaunchKernelGGL((k_pcsc<M, MODE>), dim3((unsigned)nitems), dim3(256), shm, s, d_items, d_jobs);
+    hipLaunchKernelGGL((k_pcsc<M, MODE, L, BS, D>), dim3((unsigned)nitems), dim3(BS), shm, s, d_items, d_jobs,
+                       sh.R, sh.nseg);
+}
+
+template <int M, int MODE, int L, int BS>
+void launch_shape(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh,
+                  hipStream_t s) {
+    if (sh.depth == 2) launch_depth<M, MODE, L, BS, 2>(d_items, nitems, d_jobs, sh, s);
+    else launch_depth<M, MODE, L, BS, 3>(d_items, nitems, d_jobs, sh, s);
+}
+
+template <int M, int MODE, int L>
+void launch_bs(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh, hipStream_t s) {
+    if (sh.block == 256) launch_shape<M, MODE, L, 256>(d_items, nitems, d_jobs, sh, s);
+    else launch_shape<M, MODE, L, 512>(d_items, nitems, d_jobs, sh, s);
+}
+
+template <int M, int MODE>
+void launch_l(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh, hipStream_t s) {
+    if (sh.L == 4) launch_bs<M, MODE, 4>(d_items, nitems, d_jobs, sh, s);
+    else launch_bs<M, MODE, 8>(d_items, nitems, d_jobs, sh, s);
 }
 
 template <int MODE>
-void launch_mode(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, int M, int R, hipStream_t s) {
+void launch_mode(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, int M, const SpmmShape &sh,
+                 hipStream_t s) {
     switch (M) {
-        case 1: launch_items<1, MODE>(d_items, nitems, d_jobs, R, s); break;
-        case 2: launch_items<2, MODE>(d_items, nitems, d_jobs, R, s); break;
-        case 4: launch_items<4, MODE>(d_items, nitems, d_jobs, R, s); break;
-        default: launch_items<8, MODE>(d_items, nitems, d_jobs, R, s); break;
+        case 1: launch_l<1, MODE>(d_items, nitems, d_jobs, sh, s); break;
+        case 2: launch_l<2, MODE>(d_items, nitems, d_jobs, sh, s); break;
+        case 4: launch_l<4, MODE>(d_items, nitems, d_jobs, sh, s); break;
+        default: launch_l<8, MODE>(d_items, nitems, d_jobs, sh, s); break;
     }
 }
 
-int g_R = 0;  // panel height all matrices use (set by ensure_pcsc)
+int env_int(const char *name, int dflt) {
+    const char *v = std::getenv(name);
+    return v && *v ? std::atoi(v) : dflt;
+}
 
 }  // namespace
+
+bool valid_shape(const SpmmShape &sh) {
+    // the instantiated launch shapes: L in {4, 8}, 256 or 512 threads, depth 2 or 3
+    return sh.R >= 64 && sh.R <= 4096 && (sh.R & (sh.R - 1)) == 0 && (sh.L == 4 || sh.L == 8) &&
+           (sh.block == 256 || sh.block == 512) && sh.target >= 4096 && sh.nseg >= 64 && sh.nseg <= 4096 &&
+           (sh.depth == 2 || sh.depth == 3) && lds_map<1, double>(sh.R, sh.nseg).total <= LDS_MAX;
+}
+
+SpmmShape spmm_shape() {
+    if (!g_shape_set) {
+        SpmmShape sh;
+        sh.R = env_int("GCI_SPMM_R", sh.R);
+        sh.L = env_int("GCI_SPMM_L", sh.L);
+        sh.block = env_int("GCI_SPMM_BLOCK", sh.block);
+        sh.target = env_int("GCI_SPMM_TARGET", (int)sh.target);
+        sh.nseg = env_int("GCI_SPMM_NSEG", sh.nseg);
+        sh.depth = env_int("GCI_SPMM_DEPTH", sh.depth);
+        GCI_REQUIRE(valid_shape(sh), "spmm: invalid launch shape");
+        g_shape = sh;
+        g_shape_set = true;
+    }
+    return g_shape;
+}
+
+void set_spmm_shape(const SpmmShape &sh) {
+    GCI_REQUIRE(valid_shape(sh), "spmm: invalid launch shape");
+    g_shape = sh;
+    g_shape_set = true;
+    ++g_shape_version;
+}
+
+uint64_t spmm_shape_version() { return g_shape_version; }
+
+int spmm_max_width(int mode) {
+    const SpmmShape sh = spmm_shape();
+    for (int w = 8; w > 1; w /= 2)
+        if (spmm_lds_bytes(sh, w, mode) <= LDS_MAX) return w;
+    return 1;
+}
 
 int spmm_width(int m) { return m <= 1 ? 1 : m <= 2 ? 2 : m <= 4 ? 4 : 8; }
 
 void spmm_items(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, int M, int mode, hipStream_t s) {
     if (nitems <= 0) return;
-    const int R = g_R > 0 ? g_R : 1024;
+    const SpmmShape sh = spmm_shape();
     switch (mode) {
-        case GCI_OP_FP64: launch_mode<GCI_OP_FP64>(d_items, nitems, d_jobs, M, R, s); break;
-        case GCI_OP_SPLIT_BULK: launch_mode<GCI_OP_SPLIT_BULK>(d_items, nitems, d_jobs, M, R, s); break;
-        default: launch_mode<GCI_OP_SPLIT_FULL>(d_items, nitems, d_jobs, M, R, s); break;
+        case GCI_OP_FP64: launch_mode<GCI_OP_FP64>(d_items, nitems, d_jobs, M, sh, s); break;
+        case GCI_OP_SPLIT_BULK: launch_mode<GCI_OP_SPLIT_BULK>(d_items, nitems, d_jobs, M, sh, s); break;
+        default: launch_mode<GCI_OP_SPLIT_FULL>(d_items, nitems, d_jobs, M, sh, s); break;
     }
 }
 
-void ensure_pcsc(HMat &h, int R, int64_t target) {
+void build_items(HMat &h, int64_t target) {
+    Pcsc &pc = h.pc;
+    pc.items.clear();
+    if (pc.nseg == 0) return;
+    const int R = pc.R;
+    for (int64_t s0 = 0, s = 1; s <= pc.nseg; ++s) {
+        if (s == pc.nseg || pc.seg_panel_h[s] != pc.seg_panel_h[s0] ||
+            pc.seg_start_h[s] - pc.seg_start_h[s0] >= target) {
+            const int64_t row0 = (int64_t)pc.seg_panel_h[s0] * R;
+            pc.items.push_back(SpmmItem{s0, s, row0, (int32_t)std::min<int64_t>(R, h.n - row0), 0});
+            s0 = s;
+        }
+    }
+    pc.target = target;
+}
+
+void recut_items(HMat &h) {
+    const SpmmShape sh = spmm_shape();
+    GCI_REQUIRE(h.pc.R == sh.R, "spmm: panel height changed while the matrix is in use");
+    if (h.pc.target != sh.target) {
+        build_items(h, sh.target);
+        h.pc.d_items.release();
+    }
+}
+
+void ensure_items(HMat &h) {
+    if (h.pc.R != 0 && h.pc.R != spmm_shape().R) {
+        GCI_HIP(hipDeviceSynchronize());  // queued launches may still read the old layout
+        h.pc = Pcsc();
+    }
+    ensure_pcsc(h);
+    recut_items(h);
+}
+
+size_t spmm_lds_bytes(const SpmmShape &sh, int M, int mode) {
+    if (mode == GCI_OP_SPLIT_BULK) {
+        switch (M) {
+            case 1: return lds_map<1, float>(sh.R, sh.nseg).total;
+            case 2: return lds_map<2, float>(sh.R, sh.nseg).total;
+            case 4: return lds_map<4, float>(sh.R, sh.nseg).total;
+            default: return lds_map<8, float>(sh.R, sh.nseg).total;
+        }
+    }
+    switch (M) {
+        case 1: return lds_map<1, double>(sh.R, sh.nseg).total;
+        case 2: return lds_map<2, double>(sh.R, sh.nseg).total;
+        case 4: return lds_map<4, double>(sh.R, sh.nseg).total;
+        default: return lds_map<8, double>(sh.R, sh.nseg).total;
+    }
+}
+
+size_t spmm_lds_max() { return LDS_MAX; }
+
+void ensure_pcsc(HMat &h) {
     if (h.pc.R) return;
-    GCI_REQUIRE(R >= 1 && R <= 65536, "pcsc: panel height out of range");
-    GCI_REQUIRE(g_R == 0 || g_R == R, "pcsc: all matrices share one panel height");
-    g_R = R;
+    const SpmmShape sh = spmm_shape();
+    const int R = sh.R;
     Pcsc &pc = h.pc;
     const int64_t nnz = h.nnz, n = h.n;
     hipStream_t s = h.stream;
     pc.nnz = nnz;
+    pc.nseg = 0;
+    pc.npad = 0;
     if (nnz > 0) {
         DBuf<uint64_t> k0(nnz), k1(nnz);
         DBuf<int64_t> i0(nnz), i1(nnz);
@@ -297,7 +678,7 @@ void ensure_pcsc(HMat &h, int R, int64_t target) {
         k0.release();
         i0.release();
         DBuf<int32_t> flag(nnz);
-        DBuf<int64_t> segid(nnz + 1);
+        DBuf<int64_t> segid(nnz);
         const int64_t eb = std::min<int64_t>((nnz + 255) / 256, 65536);
         hipLaunchKernelGGL(k_seg_flags, dim3((unsigned)eb), dim3(256), 0, s, nnz, k1.p, flag.p);
         tb = 0;
@@ -311,55 +692,53 @@ void ensure_pcsc(HMat &h, int R, int64_t target) {
         GCI_HIP(hipMemcpyAsync(&last, segid.p + nnz - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         GCI_HIP(hipMemcpyAsync(&lastf, flag.p + nnz - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         GCI_HIP(hipStreamSynchronize(s));
-        pc.nseg = last + lastf;
-        pc.seg_col.alloc(pc.nseg);
-        pc.seg_start.alloc(pc.nseg + 1);
-        pc.lrow.alloc(nnz);
-        if (h.val.p) pc.val.alloc(nnz);
-        if (h.precision == GCI_SPLIT32) {
-            pc.hi.alloc(nnz);
-            pc.lo.alloc(nnz);
+        const int64_t nseg = last + lastf;
+        pc.nseg = nseg;
+        pc.seg_col.alloc(nseg);
+        DBuf<int64_t> ustart(nseg);
+        DBuf<int32_t> spanel(nseg);
+        hipLaunchKernelGGL(k_seg_meta, dim3((unsigned)eb), dim3(256), 0, s, nnz, k1.p, flag.p, segid.p,
+                           pc.seg_col.p, ustart.p, spanel.p);
+        GCI_HIP(hipGetLastError());
+        DBuf<int64_t> plen(nseg + 1);
+        const int64_t sb = std::min<int64_t>((nseg + 256) / 256, 65536);
+        hipLaunchKernelGGL(k_seg_plen, dim3((unsigned)sb), dim3(256), 0, s, nseg, nnz, ustart.p, plen.p);
+        GCI_HIP(hipGetLastError());
+        pc.seg_start.alloc(nseg + 1);
+        tb = 0;
+        GCI_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, plen.p, pc.seg_start.p, nseg + 1, s));
+        {
+            DBuf<char> tmp(tb);
+            GCI_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, plen.p, pc.seg_start.p, nseg + 1, s));
         }
-        hipLaunchKernelGGL(k_seg_fill, dim3((unsigned)eb), dim3(256), 0, s, nnz, k1.p, flag.p, segid.p, i1.p,
-                           h.val.p, h.hi.p, h.lo.p, pc.seg_col.p, pc.seg_start.p, pc.lrow.p, pc.val.p, pc.hi.p,
+        pc.seg_start_h.resize(nseg + 1);
+        pc.seg_panel_h.resize(nseg);
+        GCI_HIP(hipMemcpyAsync(pc.seg_start_h.data(), pc.seg_start.p, (nseg + 1) * sizeof(int64_t),
+                               hipMemcpyDeviceToHost, s));
+        GCI_HIP(hipMemcpyAsync(pc.seg_panel_h.data(), spanel.p, nseg * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GCI_HIP(hipStreamSynchronize(s));
+        const int64_t npad = pc.seg_start_h[nseg];
+        pc.npad = npad;
+        pc.lrow.alloc(npad);
+        if (h.val.p) pc.val.alloc(npad);
+        if (h.precision == GCI_SPLIT32) {
+            pc.hi.alloc(npad);
+            pc.lo.alloc(npad);
+        }
+        // pad entries keep value 0
+        if (pc.val.p) GCI_HIP(hipMemsetAsync(pc.val.p, 0, npad * sizeof(double), s));
+        if (pc.hi.p) {
+            GCI_HIP(hipMemsetAsync(pc.hi.p, 0, npad * sizeof(float), s));
+            GCI_HIP(hipMemsetAsync(pc.lo.p, 0, npad * sizeof(float), s));
+        }
+        hipLaunchKernelGGL(k_seg_fill, dim3((unsigned)eb), dim3(256), 0, s, nnz, nseg, k1.p, flag.p, segid.p,
+                           ustart.p, pc.seg_start.p, i1.p, h.val.p, h.hi.p, h.lo.p, pc.lrow.p, pc.val.p, pc.hi.p,
                            pc.lo.p);
         GCI_HIP(hipGetLastError());
-        GCI_HIP(hipMemcpyAsync(pc.seg_start.p + pc.nseg, &nnz, sizeof(int64_t), hipMemcpyHostToDevice, s));
         GCI_HIP(hipStreamSynchronize(s));
     }
-    // work items: panels cut into slices of about `target` entries
-    std::vector<int64_t> ss(pc.nseg + 1);
-    std::vector<int32_t> sc(pc.nseg);
-    if (pc.nseg) {
-        GCI_HIP(hipMemcpy(ss.data(), pc.seg_start.p, (pc.nseg + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
-        GCI_HIP(hipMemcpy(sc.data(), pc.seg_col.p, pc.nseg * sizeof(int32_t), hipMemcpyDeviceToHost));
-    } else {
-        ss[0] = 0;
-    }
-    // segment -> panel: segments are sorted by (panel, column); recover the
-    // panel of each segment from its first entry's row via the CSR
-    std::vector<int64_t> rp(n + 1);
-    GCI_HIP(hipMemcpy(rp.data(), h.rp.p, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
-    const int64_t npan = (n + R - 1) / R;
-    std::vector<int64_t> panel_nnz(npan, 0);
-    for (int64_t p = 0; p < npan; ++p) panel_nnz[p] = rp[std::min(n, (p + 1) * R)] - rp[p * R];
-    pc.items.clear();
-    int64_t seg = 0;
-    for (int64_t p = 0; p < npan; ++p) {
-        const int64_t row0 = p * R;
-        const int rows = (int)std::min<int64_t>(R, n - row0);
-        const int64_t end_e = ss[seg] + panel_nnz[p];  // entries of this panel are contiguous
-        int64_t s0 = seg;
-        while (seg < pc.nseg && ss[seg] < end_e) {
-            ++seg;
-            if (ss[seg] - ss[s0] >= target || seg == pc.nseg || ss[seg] >= end_e) {
-                pc.items.push_back(SpmmItem{s0, seg, row0, rows, 0});
-                s0 = seg;
-            }
-        }
-    }
-    GCI_REQUIRE(seg == pc.nseg, "pcsc: segment/panel bookkeeping mismatch");
-    pc.R = R;  // built
+    pc.R = R;
+    build_items(h, sh.target);
 }
 
 void spmm(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode, hipStream_t s) {
@@ -370,42 +749,52 @@ void spmm(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode
     else
         GCI_REQUIRE(h.val.p != nullptr, "spmm: FP64 operator on a matrix without FP64 values");
     if (h.n == 0) return;
-    ensure_pcsc(h);
-    for (int c0 = 0; c0 < m; c0 += 8) {
-        const int mc = std::min(8, m - c0);
+    ensure_items(h);
+    Pcsc &pc = h.pc;
+    // staging buffers, the job table (one descriptor per operator mode and
+    // column count) and the device copy of the work items stay with the
+    // matrix; they are (re)written only when they grow, after the stream
+    // has drained, so queued launches never see them change
+    const int W = spmm_max_width(mode);  // widest launch whose LDS fits the shape
+    const int Mmax = spmm_width(std::min(W, m));
+    if (pc.stage_M < Mmax) {
+        GCI_HIP(hipStreamSynchronize(s));
+        pc.stage_x.alloc((size_t)h.n * Mmax * sizeof(double));
+        pc.stage_y.alloc((size_t)h.n * Mmax);
+        pc.stage_M = Mmax;
+        std::vector<SpmmJob> jt(3 * 8);
+        for (int md = 0; md < 3; ++md)
+            for (int mc = 1; mc <= 8; ++mc)
+                jt[md * 8 + mc - 1] = SpmmJob{pc.seg_col.p, pc.seg_start.p, pc.lrow.p, pc.val.p, pc.hi.p, pc.lo.p,
+                                              pc.stage_x.p, pc.stage_y.p, mc, md, 1, 0};
+        pc.d_jobs.alloc(jt.size());
+        GCI_HIP(hipMemcpy(pc.d_jobs.p, jt.data(), jt.size() * sizeof(SpmmJob), hipMemcpyHostToDevice));
+    }
+    if (!pc.items.empty() && pc.d_items.n < pc.items.size()) {
+        GCI_HIP(hipStreamSynchronize(s));
+        pc.d_items.alloc(pc.items.size());
+        GCI_HIP(hipMemcpy(pc.d_items.p, pc.items.data(), pc.items.size() * sizeof(SpmmItem),
+                          hipMemcpyHostToDevice));
+    }
+    for (int c0 = 0; c0 < m; c0 += W) {
+        const int mc = std::min(W, m - c0);
         const int M = spmm_width(mc);
-        const size_t xb = mode == GCI_OP_SPLIT_BULK ? sizeof(float) : sizeof(double);
-        DBuf<char> xs((size_t)h.n * M * xb);
-        DBuf<double> ys((size_t)h.n * M);
         const double *xc = x + (int64_t)c0 * ldx;
         double *yc = y + (int64_t)c0 * ldy;
         const int64_t pb = std::min<int64_t>((h.n * M + 255) / 256, 65536);
         if (mode == GCI_OP_SPLIT_BULK)
             hipLaunchKernelGGL(k_pack<float>, dim3((unsigned)pb), dim3(256), 0, s, h.n, mc, M, xc, (int64_t)ldx,
-                               (float *)xs.p, ys.p);
+                               (float *)pc.stage_x.p, pc.stage_y.p);
         else
             hipLaunchKernelGGL(k_pack<double>, dim3((unsigned)pb), dim3(256), 0, s, h.n, mc, M, xc, (int64_t)ldx,
-                               (double *)xs.p, ys.p);
-        if (!h.pc.items.empty()) {
-            SpmmJob jb{h.pc.seg_col.p, h.pc.seg_start.p, h.pc.lrow.p, h.pc.val.p, h.pc.hi.p, h.pc.lo.p, xs.p, ys.p,
-                       mc, mode, 1, 0};
-            DBuf<SpmmJob> dj(1);
-            DBuf<SpmmItem> di(h.pc.items.size());
-            GCI_HIP(hipMemcpyAsync(dj.p, &jb, sizeof(SpmmJob), hipMemcpyHostToDevice, s));
-            GCI_HIP(hipMemcpyAsync(di.p, h.pc.items.data(), h.pc.items.size() * sizeof(SpmmItem),
-                                   hipMemcpyHostToDevice, s));
-            spmm_items(di.p, (int64_t)h.pc.items.size(), dj.p, M, mode, s);
-            GCI_HIP(hipGetLastError());
-            const int64_t ub = std::min<int64_t>((h.n * mc + 255) / 256, 65536);
-            hipLaunchKernelGGL(k_unpack, dim3((unsigned)ub), dim3(256), 0, s, h.n, mc, M, (const double *)ys.p,
-                               (const double *)h.diag.p, xc, (int64_t)ldx, yc, (int64_t)ldy);
-            GCI_HIP(hipStreamSynchronize(s));  // staging buffers are released on return
-        } else {
-            const int64_t ub = std::min<int64_t>((h.n * mc + 255) / 256, 65536);
-            hipLaunchKernelGGL(k_unpack, dim3((unsigned)ub), dim3(256), 0, s, h.n, mc, M, (const double *)ys.p,
-                               (const double *)h.diag.p, xc, (int64_t)ldx, yc, (int64_t)ldy);
-            GCI_HIP(hipStreamSynchronize(s));
-        }
+                               (double *)pc.stage_x.p, pc.stage_y.p);
+        GCI_HIP(hipGetLastError());
+        // items carry job 0; the table offset selects the descriptor
+        spmm_items(pc.d_items.p, (int64_t)pc.items.size(), pc.d_jobs.p + mode * 8 + mc - 1, M, mode, s);
+        GCI_HIP(hipGetLastError());
+        const int64_t ub = std::min<int64_t>((h.n * mc + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_unpack, dim3((unsigned)ub), dim3(256), 0, s, h.n, mc, M, (const double *)pc.stage_y.p,
+                           (const double *)h.diag.p, xc, (int64_t)ldx, yc, (int64_t)ldy);
         GCI_HIP(hipGetLastError());
     }
 }

@@ -244,6 +244,50 @@ def batch_solve(blocks, nev, tol, eav=None):
     return evals, evecs, stats
 
 
+_SHAPE_KEYS = ("R", "L", "block", "nseg", "depth", "target")
+
+
+def spmm_shape() -> dict:
+    """The SpMM launch shape in force: panel height R, lanes per column
+    segment L, workgroup size, segments per LDS chunk (nseg), batches in
+    flight per lane (depth) and entries per work item (target)."""
+    s = L.GciSpmmShape()
+    check(L.lib().gci_spmm_get_shape(C.byref(s)))
+    return {k: int(getattr(s, k)) for k in _SHAPE_KEYS}
+
+
+def set_spmm_shape(**kw) -> dict:
+    """Changes the given fields of the launch shape; returns the new shape."""
+    cur = spmm_shape()
+    unknown = set(kw) - set(_SHAPE_KEYS)
+    if unknown:
+        raise ValueError(f"unknown shape fields {sorted(unknown)}")
+    cur.update({k: int(v) for k, v in kw.items()})
+    s = L.GciSpmmShape(**cur)
+    check(L.lib().gci_spmm_set_shape(C.byref(s)))
+    return cur
+
+
+def spmm_time(blocks, m: int, mode: int = L.GCI_OP_SPLIT_BULK, reps: int = 5) -> float:
+    """Mean ms of one m-column operator application over all blocks."""
+    handles = (C.c_void_p * len(blocks))(*[b.handle for b in blocks])
+    ms = C.c_double()
+    check(L.lib().gci_spmm_time(handles, C.c_int(len(blocks)), C.c_int(m), C.c_int(mode), C.c_int(reps),
+                                C.byref(ms)))
+    return ms.value
+
+
+def autotune(blocks, m: int, mode: int = L.GCI_OP_SPLIT_BULK, reps: int = 3):
+    """Coordinate search of the launch shape on these blocks; leaves the
+    best shape in force.  Returns (shape dict, ms per pass)."""
+    handles = (C.c_void_p * len(blocks))(*[b.handle for b in blocks])
+    s = L.GciSpmmShape()
+    ms = C.c_double()
+    check(L.lib().gci_spmm_autotune(handles, C.c_int(len(blocks)), C.c_int(m), C.c_int(mode), C.c_int(reps),
+                                    C.byref(s), C.byref(ms)))
+    return {k: int(getattr(s, k)) for k in _SHAPE_KEYS}, ms.value
+
+
 class Server:
     """Continuous-batching CI solve server (gci_server_*)."""
 

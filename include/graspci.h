@@ -217,6 +217,34 @@ int gci_server_drain(gci_server *s);
 int gci_profile_enable(int on);
 int gci_profile_read(double *stats);
 
+/* --------------------------------------------- SpMM launch shape, autotuner
+ * No counterpart in the reference (SPICMV2 has no tunables).  The symmetric
+ * SpMM's launch shape: panel height R (rows whose x and y stay in LDS; the
+ * panel-CSC layout is built for it), lanes per column segment L (4, 8),
+ * workgroup size (256, 512), segments per LDS chunk `nseg`, batches each
+ * lane keeps in flight `depth` (2, 3), and about `target` entries per
+ * work item (a slice of one panel).  One shape is in force per process; a
+ * matrix laid out for another R is laid out again at its next use outside a
+ * Davidson engine.  Set or tune the shape before solves, not while requests
+ * are resident (an engine refuses a change of R under resident problems). */
+typedef struct gci_spmm_shape {
+    int R, L, block, nseg, depth;
+    int64_t target;
+} gci_spmm_shape;
+int gci_spmm_get_shape(gci_spmm_shape *out);
+int gci_spmm_set_shape(const gci_spmm_shape *shape);
+/* Mean milliseconds of the SpMM kernel launches of one m-column operator
+ * application (mode GCI_OP_*) to every matrix of the set at once -- all
+ * matrices' work items in one launch per column block, as the batched
+ * Davidson engine issues them -- on seeded device-resident operands; `reps`
+ * timed repetitions after one untimed application. */
+int gci_spmm_time(gci_hmat *const *mats, int nmats, int m, int mode, int reps, double *ms);
+/* Coordinate search over the shape parameters on that timing, starting from
+ * the shape in force; leaves the fastest shape in force and returns it and
+ * its time. */
+int gci_spmm_autotune(gci_hmat *const *mats, int nmats, int m, int mode, int reps, gci_spmm_shape *best,
+                      double *best_ms);
+
 #ifdef __cplusplus
 }
 #endif
