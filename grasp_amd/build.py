@@ -53,9 +53,46 @@ def _link(objs, out, extra=()):
     os.replace(tmp, out)
 
 
+FORT_DIR = os.path.join(HERE, "fortran")
+FORT_BUILD = os.path.join(FORT_DIR, "_build")
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran(verbose: bool = True) -> bool:
+    """Builds the Fortran host layer (ISO_C_BINDING module gci_bind, the
+    rci90-shaped wrappers gci_rci) into fortran/_build/libgcifort.so and the
+    example host program fortran/_build/gci_drive.  Returns False (and
+    builds nothing) when no Fortran compiler is installed."""
+    if not os.path.exists(FC):
+        if verbose:
+            print(f"build_fortran: {FC} not found; Fortran host layer not built")
+        return False
+    os.makedirs(FORT_BUILD, exist_ok=True)
+    fflags = ["-O2", "-fPIC", "-module-dir", FORT_BUILD]
+    link = [f"-L{HERE}", "-lgraspci", "-Wl,-rpath,$ORIGIN/../.."]
+    objs = []
+    for name in ("gci_bind", "gci_rci"):
+        obj = os.path.join(FORT_BUILD, name + ".o")
+        cmd = [FC, *fflags, "-c", os.path.join(FORT_DIR, name + ".f90"), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{FC} failed for {name}.f90:\n{r.stderr[-4000:]}")
+        objs.append(obj)
+    steps = [[FC, "-shared", *objs, *link, "-o", os.path.join(FORT_BUILD, "libgcifort.so")],
+             [FC, *fflags, os.path.join(FORT_DIR, "gci_drive.f90"), *objs, *link, "-o",
+              os.path.join(FORT_BUILD, "gci_drive")]]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"Fortran link failed:\n{r.stderr[-4000:]}")
+    if verbose:
+        print(f"built {FORT_BUILD}/libgcifort.so and gci_drive")
+    return True
+
+
 def build(verbose: bool = True) -> str:
-    """Builds libgraspci.so (the engine) and libgcisynth.so (seeded
-    synthetic block generator used by bench.py and tests)."""
+    """Builds libgraspci.so (the engine), libgcisynth.so (seeded synthetic
+    block generator used by bench.py and tests) and the Fortran host layer."""
     bdir = os.path.join(CSRC, "_build")
     os.makedirs(bdir, exist_ok=True)
     srcs = sources()
@@ -69,6 +106,7 @@ def build(verbose: bool = True) -> str:
     _link(sobjs, SYNTH_OUT, [f"-L{HERE}", "-lgraspci", "-Wl,-rpath,$ORIGIN"])
     if verbose:
         print(f"built {OUT} and {SYNTH_OUT}")
+    build_fortran(verbose)
     return OUT
 
 
