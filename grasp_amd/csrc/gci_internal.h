@@ -94,11 +94,11 @@ struct SpmmItem {
 struct SpmmJob;
 
 // Launch shape of the SpMM: panel height R (a power of two; the layout is
-// built for it), lanes per segment group L, workgroup size, entries per work
-// item, segments per LDS chunk, and prefetch depth.
+// built for it), lanes per segment group L (8 or 16), workgroup size,
+// entries per work item, segments per LDS chunk, and prefetch depth.
 struct SpmmShape {
     int R = 1024;
-    int L = 4;
+    int L = 16;       // lanes per segment group: L/M entry slots x M vectors
     int block = 512;
     int64_t target = 65536;
     int nseg = 512;   // segments per chunk (their tables are staged in LDS)

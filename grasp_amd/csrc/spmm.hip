@@ -94,6 +94,17 @@ __global__ void k_seg_plen(int64_t nseg, int64_t nnz, const int64_t *__restrict_
     }
 }
 
+// Position of a segment's p-th entry in storage.  Quad q (entries 4q..4q+3)
+// is stored rotated by q: slot t holds entry 4q + ((t + q) & 3).  The lane
+// that loads quad q is lane q mod L of its group, and the kernel applies
+// slot j of every lane's quad in the same instruction.  So one instruction's
+// lanes of a group touch entries 4l + ((j + l) & 3) -- rows 5 apart, not 4 --
+// which spreads their y rows over the LDS banks.
+__host__ __device__ inline int64_t quad_slot(int64_t p) {
+    const int64_t q = p >> 2, t = p & 3;
+    return (q << 2) | ((t - q) & 3);
+}
+
 __global__ void k_seg_fill(int64_t nnz, int64_t nseg, const uint64_t *__restrict__ keys,
                            const int32_t *__restrict__ flag, const int64_t *__restrict__ segid,
                            const int64_t *__restrict__ ustart, const int64_t *__restrict__ pstart,
@@ -102,7 +113,8 @@ __global__ void k_seg_fill(int64_t nnz, int64_t nseg, const uint64_t *__restrict
                            double *__restrict__ pval, float *__restrict__ phi, float *__restrict__ plo) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t s = seg_of(flag, segid, e);
-        const int64_t pe = pstart[s] + (e - ustart[s]);
+        const int64_t p = e - ustart[s];
+        const int64_t pe = pstart[s] + quad_slot(p);
         const uint16_t lr = (uint16_t)(keys[e] & 0xFFFFull);
         lrow[pe] = lr;
         const int64_t src = perm[e];
@@ -114,7 +126,7 @@ __global__ void k_seg_fill(int64_t nnz, int64_t nseg, const uint64_t *__restrict
         // the segment's last entry also writes the pad rows (values stay 0)
         const int64_t uend = s + 1 < nseg ? ustart[s + 1] : nnz;
         if (e == uend - 1)
-            for (int64_t q = pe + 1; q < pstart[s + 1]; ++q) lrow[q] = lr;
+            for (int64_t q = p + 1; q < pstart[s + 1] - pstart[s]; ++q) lrow[pstart[s] + quad_slot(q)] = lr;
     }
 }
 
@@ -236,57 +248,45 @@ struct Quad<GCI_OP_FP64> {
     __device__ inline void settle() const { asm volatile("" ::"v"(r), "v"(a), "v"(b)); }
 };
 
-template <int Q>
+template <int CTRL>
 __device__ inline double dpp_double(double x) {
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), Q, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), Q, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, false);
     return __hiloint2double(hi, lo);
 }
 
-// Sum over the L lanes of a group (L a power of two <= 16), every lane
-// receiving the total; the pairing order is fixed, so the sum is the same in
-// every lane and on every run.
-template <int L>
-__device__ inline double group_sum(double x) {
-    if constexpr (L >= 2) x += dpp_double<0xB1>(x);  // quad_perm [1,0,3,2]
-    if constexpr (L >= 4) x += dpp_double<0x4E>(x);  // quad_perm [2,3,0,1]
-    if constexpr (L >= 8) x += __shfl_xor(x, 4, 64);
-    if constexpr (L >= 16) x += __shfl_xor(x, 8, 64);
+// Sum over the lanes l, l + S, l + 2S, ... of a 16-lane DPP row (S = M,
+// the row holding E = 16/S such lanes): row rotations by S, 2S, ... (DPP
+// row_ror).  Every lane of the class gets the total; the pairing order is
+// fixed, so the sum is the same on every run.
+template <int S, int E>
+__device__ inline double slot_sum(double x) {
+    if constexpr (E >= 2) x += dpp_double<0x120 + S>(x);
+    if constexpr (E >= 4) x += dpp_double<0x120 + 2 * S>(x);
+    if constexpr (E >= 8) x += dpp_double<0x120 + 4 * S>(x);
+    if constexpr (E >= 16) x += dpp_double<0x120 + 8 * S>(x);
     return x;
 }
 
-template <int M, int MODE, int YS>
-__device__ inline void consume(const Quad<MODE> &q, const typename XT<MODE>::T *xs, double *ys,
-                               const double (&xc)[M], double (&acc)[M]) {
-    using T = typename XT<MODE>::T;
+// One lane's share of a quad: the four entries (rows lr, values a) of its
+// slot, each applied to the lane's vector v: y_i[v] += a x_c[v] through an
+// LDS atomic, and the column partial acc += a x_i[v].
+template <int M, int MODE>
+__device__ inline void consume(const Quad<MODE> &q, const typename XT<MODE>::T *xs, double *ys, int v, double xc,
+                               double &acc) {
     int lr[4];
     lr[0] = (int)(q.r.x & 0xFFFFu);
     lr[1] = (int)(q.r.x >> 16);
     lr[2] = (int)(q.r.y & 0xFFFFu);
     lr[3] = (int)(q.r.y >> 16);
-    double xv[4][M];
+    double xv[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) load_row<M, T>(xs + lr[j] * M, xv[j]);
+    for (int j = 0; j < 4; ++j) xv[j] = (double)xs[lr[j] * M + v];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const double a = q.value(j);
-        double *yr = ys + lr[j] * YS;
-#pragma unroll
-        for (int v = 0; v < M; ++v) {
-            acc[v] = fma(a, xv[j][v], acc[v]);
-            __hip_atomic_fetch_add(yr + v, a * xc[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-    }
-}
-
-// M doubles from registers to LDS, 16 bytes per access
-template <int M>
-__device__ inline void store_row(double *dst, const double (&v)[M]) {
-    if constexpr (M == 1) {
-        dst[0] = v[0];
-    } else {
-#pragma unroll
-        for (int q = 0; q < M / 2; ++q) reinterpret_cast<d2v *>(dst)[q] = d2v{v[2 * q], v[2 * q + 1]};
+        acc = fma(a, xv[j], acc);
+        __hip_atomic_fetch_add(ys + lr[j] * M + v, a * xc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 }
 
@@ -305,9 +305,8 @@ __device__ inline void copy_row(T *dst, const GAS T *src) {
 }
 
 // LDS carve-up of one workgroup (offsets in bytes, 64-byte aligned parts):
-// xs (R x M, T) | ys (R rows of M + 1 fp64: the pad word spreads rows over
-// the LDS banks) | xcol (NS x M, fp64: a segment's x_j row while it runs,
-// then its column sums) | sbeg (NS + 1, int) | scol (NS, int)
+// xs (R x M, T) | ys (R x M, fp64) | xcol (NS x M, fp64: a segment's x_j
+// row while it runs, then its column sums) | sbeg (NS + 1, int) | scol (NS)
 struct LdsMap {
     size_t ys, xcol, sbeg, scol, total;
 };
@@ -315,21 +314,21 @@ template <int M, class T>
 __host__ __device__ constexpr LdsMap lds_map(int R, int NS) {
     LdsMap m{};
     m.ys = ((size_t)R * M * sizeof(T) + 63) & ~(size_t)63;
-    m.xcol = (m.ys + (size_t)R * (M + 1) * sizeof(double) + 63) & ~(size_t)63;
+    m.xcol = (m.ys + (size_t)R * M * sizeof(double) + 63) & ~(size_t)63;
     m.sbeg = m.xcol + (size_t)NS * M * sizeof(double);
     m.scol = m.sbeg + (((size_t)(NS + 1) * sizeof(int) + 15) & ~(size_t)15);
     m.total = m.scol + (size_t)NS * sizeof(int);
     return m;
 }
 
-// A group's walk over its segments in 4L-entry batches: segment i (item-
-// chunk local), batch start k, segment end e1.
+// A group's walk over its segments in batches of E quads: segment i (chunk
+// local), batch start k, segment end e1.
 struct Cursor {
     int i, k, e1;
-    template <int L, int G>
+    template <int E, int G>
     __device__ inline bool advance(const int *sbeg, int nsc) {  // returns true at a segment end
-        if (k + 4 * L < e1) {
-            k += 4 * L;
+        if (k + 4 * E < e1) {
+            k += 4 * E;
             return false;
         }
         i += G;
@@ -341,12 +340,18 @@ struct Cursor {
     }
 };
 
+// The SpMM work-item kernel.  A group of L lanes (one DPP row or half of
+// one) holds E = L/M entry slots x M vectors: lane v + M*e loads quad e of
+// the group's batch (the M lanes of a slot load the same address) and
+// applies its four entries to vector v.  So one LDS instruction covers E
+// entries x M vectors, each entry's M lanes on one contiguous y row.
 template <int M, int MODE, int L, int BS, int D>
 __global__ __launch_bounds__(BS) void k_pcsc(const SpmmItem *__restrict__ items, const SpmmJob *__restrict__ jobs,
                                              int R, int NS) {
     using T = typename XT<MODE>::T;
+    static_assert(L % M == 0 && L <= 16, "a group is E slots x M vectors inside one DPP row");
+    constexpr int E = L / M;   // entry slots per group
     constexpr int G = BS / L;  // lane groups per workgroup
-    constexpr int YS = M + 1;  // ys row stride (doubles)
     extern __shared__ __align__(64) char smem[];
     const SpmmItem it = items[blockIdx.x];
     const SpmmJob jb = jobs[it.job];
@@ -364,12 +369,12 @@ __global__ __launch_bounds__(BS) void k_pcsc(const SpmmItem *__restrict__ items,
     const GAS int64_t *gst = (const GAS int64_t *)jb.seg_start;
     double *gy = jb.ys;
     const int t = threadIdx.x;
-    const int l = t % L, g = t / L;
+    const int g = t / L, r = t % L, e = r / M, v = r % M;
     // the panel's x rows and zeroed y rows stay for the whole item
-    for (int r = t; r < rows; r += BS) {
-        copy_row<M, T>(xs + r * M, gx + (it.row0 + r) * M);
+    for (int rr = t; rr < rows; rr += BS) {
+        copy_row<M, T>(xs + rr * M, gx + (it.row0 + rr) * M);
 #pragma unroll
-        for (int v = 0; v < YS; ++v) ys[r * YS + v] = 0.0;
+        for (int u = 0; u < M; ++u) ys[rr * M + u] = 0.0;
     }
     // the item's segments in chunks of at most NS (their tables fill the LDS)
     for (int64_t cs = it.seg0; cs < it.seg1; cs += NS) {
@@ -383,7 +388,7 @@ __global__ __launch_bounds__(BS) void k_pcsc(const SpmmItem *__restrict__ items,
                 double xr[M];
                 load_row<M, T>(gx + (int64_t)c * M, xr);
 #pragma unroll
-                for (int v = 0; v < M; ++v) xcol[i * M + v] = xr[v];
+                for (int u = 0; u < M; ++u) xcol[i * M + u] = xr[u];
             }
         }
         ItemPtrs P;
@@ -404,42 +409,33 @@ __global__ __launch_bounds__(BS) void k_pcsc(const SpmmItem *__restrict__ items,
             cc.k = lc.k = sbeg[g];
             cc.e1 = lc.e1 = sbeg[g + 1];
         }
-        double xc[M], acc[M];
-#pragma unroll
-        for (int v = 0; v < M; ++v) acc[v] = 0.0;
-        if (g < nsc) load_row<M, double>(xcol + g * M, xc);
+        double acc = 0.0, xc = g < nsc ? xcol[g * M + v] : 0.0;
         Quad<MODE> q[D + 1];
         bool ok[D + 1];
-        auto issue = [&](int slot_dummy, Quad<MODE> &dst, bool &dok) {
-            (void)slot_dummy;
-            dok = lc.i < nsc && lc.k + 4 * l < lc.e1;
-            dst.load(P, dok ? (uint32_t)(lc.k + 4 * l) >> 2 : 0u);
-            if (lc.i < nsc) lc.advance<L, G>(sbeg, nsc);
+        auto issue = [&](Quad<MODE> &dst, bool &dok) {
+            dok = lc.i < nsc && lc.k + 4 * e < lc.e1;
+            dst.load(P, dok ? (uint32_t)(lc.k + 4 * e) >> 2 : 0u);
+            if (lc.i < nsc) lc.advance<E, G>(sbeg, nsc);
         };
 #pragma unroll
-        for (int d = 0; d < D; ++d) issue(d, q[d], ok[d]);
+        for (int d = 0; d < D; ++d) issue(q[d], ok[d]);
         auto step = [&](Quad<MODE> &cur, bool cur_ok, Quad<MODE> &nxt, bool &nxt_ok) {
-            issue(0, nxt, nxt_ok);
+            issue(nxt, nxt_ok);
             cur.settle();
-            if (cur_ok) consume<M, MODE, YS>(cur, xs, ys, xc, acc);
+            if (cur_ok) consume<M, MODE>(cur, xs, ys, v, xc, acc);
             if (cc.i < nsc) {
                 const int ci = cc.i;
-                if (cc.advance<L, G>(sbeg, nsc)) {
+                if (cc.advance<E, G>(sbeg, nsc)) {
                     // segment done: its column sums replace its x_j row
-#pragma unroll
-                    for (int v = 0; v < M; ++v) acc[v] = group_sum<L>(acc[v]);
-                    if (l == 0) store_row<M>(xcol + ci * M, acc);
-#pragma unroll
-                    for (int v = 0; v < M; ++v) acc[v] = 0.0;
-                    if (cc.i < nsc) load_row<M, double>(xcol + cc.i * M, xc);
+                    acc = slot_sum<M, E>(acc);
+                    if (e == 0) xcol[ci * M + v] = acc;
+                    acc = 0.0;
+                    if (cc.i < nsc) xc = xcol[cc.i * M + v];
                 }
             }
         };
         while (cc.i < nsc) {
-            if constexpr (D == 1) {
-                step(q[0], ok[0], q[1], ok[1]);
-                step(q[1], ok[1], q[0], ok[0]);
-            } else if constexpr (D == 2) {
+            if constexpr (D == 2) {
                 step(q[0], ok[0], q[2], ok[2]);
                 step(q[1], ok[1], q[0], ok[0]);
                 step(q[2], ok[2], q[1], ok[1]);
@@ -456,15 +452,14 @@ __global__ __launch_bounds__(BS) void k_pcsc(const SpmmItem *__restrict__ items,
         // the chunk's column sums, as coalesced atomics
         for (int idx = t; idx < nsc * M; idx += BS) {
             const double y = xcol[idx];
-            const int v = idx % M;
-            if (y != 0.0 && v < jb.m) unsafeAtomicAdd(&gy[(int64_t)scl[idx / M] * M + v], y);
+            const int u = idx % M;
+            if (y != 0.0 && u < jb.m) unsafeAtomicAdd(&gy[(int64_t)scl[idx / M] * M + u], y);
         }
         __syncthreads();  // the next chunk overwrites the tables
     }
     for (int idx = t; idx < rows * M; idx += BS) {
-        const int r = idx / M, v = idx % M;
-        const double y = ys[r * YS + v];
-        if (y != 0.0 && v < jb.m) unsafeAtomicAdd(&gy[(it.row0 + r) * M + v], y);
+        const double y = ys[idx];
+        if (y != 0.0 && idx % M < jb.m) unsafeAtomicAdd(&gy[it.row0 * M + idx], y);
     }
 }
 
@@ -527,8 +522,8 @@ void launch_bs(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, c
 
 template <int M, int MODE>
 void launch_l(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh, hipStream_t s) {
-    if (sh.L == 4) launch_bs<M, MODE, 4>(d_items, nitems, d_jobs, sh, s);
-    else launch_bs<M, MODE, 8>(d_items, nitems, d_jobs, sh, s);
+    if (sh.L == 8) launch_bs<M, MODE, 8>(d_items, nitems, d_jobs, sh, s);
+    else launch_bs<M, MODE, 16>(d_items, nitems, d_jobs, sh, s);
 }
 
 template <int MODE>
@@ -550,8 +545,8 @@ int env_int(const char *name, int dflt) {
 }  // namespace
 
 bool valid_shape(const SpmmShape &sh) {
-    // the instantiated launch shapes: L in {4, 8}, 256 or 512 threads, depth 2 or 3
-    return sh.R >= 64 && sh.R <= 4096 && (sh.R & (sh.R - 1)) == 0 && (sh.L == 4 || sh.L == 8) &&
+    // the instantiated launch shapes: L in {8, 16}, 256 or 512 threads, depth 2 or 3
+    return sh.R >= 64 && sh.R <= 4096 && (sh.R & (sh.R - 1)) == 0 && (sh.L == 8 || sh.L == 16) &&
            (sh.block == 256 || sh.block == 512) && sh.target >= 4096 && sh.nseg >= 64 && sh.nseg <= 4096 &&
            (sh.depth == 2 || sh.depth == 3) && lds_map<1, double>(sh.R, sh.nseg).total <= LDS_MAX;
 }

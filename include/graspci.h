@@ -220,7 +220,7 @@ int gci_profile_read(double *stats);
 /* --------------------------------------------- SpMM launch shape, autotuner
  * No counterpart in the reference (SPICMV2 has no tunables).  The symmetric
  * SpMM's launch shape: panel height R (rows whose x and y stay in LDS; the
- * panel-CSC layout is built for it), lanes per column segment L (4, 8),
+ * panel-CSC layout is built for it), lanes per column segment L (8, 16),
  * workgroup size (256, 512), segments per LDS chunk `nseg`, batches each
  * lane keeps in flight `depth` (2, 3), and about `target` entries per
  * work item (a slice of one panel).  One shape is in force per process; a
