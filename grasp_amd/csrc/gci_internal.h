@@ -102,7 +102,7 @@ struct SpmmShape {
     int block = 512;
     int64_t target = 65536;
     int nseg = 512;   // segments per chunk (their tables are staged in LDS)
-    int depth = 3;    // batches each lane keeps in flight
+    int depth = 4;    // batches each lane keeps in flight
 };
 
 struct Pcsc {

@@ -193,20 +193,34 @@ __device__ inline void load_row(P p, double (&o)[M]) {
     }
 }
 
-// Uniform (per work item) views of the matrix stream, as global-address-
-// space pointers offset to the item's first entry, so that every per-lane
-// address is a 32-bit offset from a scalar base.
-struct ItemPtrs {
-    const GAS u2v *rows;   // 4 row offsets (uint16) per quad
-    const GAS f4v *hi;     // 4 fp32 values per quad
-    const GAS f4v *lo;
-    const GAS d2v *val;    // 2 fp64 values per half quad
+// The matrix stream of one chunk as buffer resources (scalar registers):
+// every per-lane access is a 32-bit byte offset, and an offset past the
+// chunk reads zeros without touching memory (the hardware range check), so
+// a lane with nothing to load needs no clamped address.
+typedef int i2v __attribute__((ext_vector_type(2)));
+typedef int i4v __attribute__((ext_vector_type(4)));
+constexpr int RSRC_WORD3 = 0x00020000;  // raw (untyped) buffer, range-checked
+constexpr uint32_t OFF_NONE = 0x7FFFFFF0u;  // an entry offset past any chunk
+
+struct ChunkRsrc {
+    __amdgpu_buffer_rsrc_t rows, hi, lo, val;
 };
 
-// Four consecutive entries of one lane.  settle() marks the registers as
-// read on every path (an empty asm statement with them as inputs), so the
-// wait for their load is placed before the branch that may skip them and
-// the wait state is the same on all paths.
+// The base and size are uniform over the workgroup; readfirstlane states it,
+// so the descriptor lives in scalar registers.
+__device__ inline __amdgpu_buffer_rsrc_t rsrc(const void *p, uint32_t bytes) {
+    const uint64_t a = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    void *u = (void *)(((uint64_t)hi << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(u, (short)0, (int)__builtin_amdgcn_readfirstlane(bytes), RSRC_WORD3);
+}
+
+// Four consecutive entries of one lane (entry offset `e` from the chunk
+// start, a multiple of 4).  settle() marks the registers as read on every
+// path (an empty asm statement with them as inputs), so the wait for their
+// load is placed before the branch that may skip them and the wait state is
+// the same on all paths.
 template <int MODE>
 struct Quad;
 
@@ -214,9 +228,9 @@ template <>
 struct Quad<GCI_OP_SPLIT_BULK> {
     u2v r;
     f4v h;
-    __device__ inline void load(const ItemPtrs &p, uint32_t q) {
-        r = __builtin_nontemporal_load(p.rows + q);
-        h = __builtin_nontemporal_load(p.hi + q);
+    __device__ inline void load(const ChunkRsrc &p, uint32_t e) {
+        r = __builtin_bit_cast(u2v, __builtin_amdgcn_raw_buffer_load_b64(p.rows, e * 2u, 0, 0));
+        h = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(p.hi, e * 4u, 0, 0));
     }
     __device__ inline double value(int j) const { return (double)h[j]; }
     __device__ inline void settle() const { asm volatile("" ::"v"(r), "v"(h)); }
@@ -226,10 +240,10 @@ template <>
 struct Quad<GCI_OP_SPLIT_FULL> {
     u2v r;
     f4v h, l;
-    __device__ inline void load(const ItemPtrs &p, uint32_t q) {
-        r = __builtin_nontemporal_load(p.rows + q);
-        h = __builtin_nontemporal_load(p.hi + q);
-        l = __builtin_nontemporal_load(p.lo + q);
+    __device__ inline void load(const ChunkRsrc &p, uint32_t e) {
+        r = __builtin_bit_cast(u2v, __builtin_amdgcn_raw_buffer_load_b64(p.rows, e * 2u, 0, 0));
+        h = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(p.hi, e * 4u, 0, 0));
+        l = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(p.lo, e * 4u, 0, 0));
     }
     __device__ inline double value(int j) const { return (double)h[j] + (double)l[j]; }
     __device__ inline void settle() const { asm volatile("" ::"v"(r), "v"(h), "v"(l)); }
@@ -239,10 +253,10 @@ template <>
 struct Quad<GCI_OP_FP64> {
     u2v r;
     d2v a, b;
-    __device__ inline void load(const ItemPtrs &p, uint32_t q) {
-        r = __builtin_nontemporal_load(p.rows + q);
-        a = __builtin_nontemporal_load(p.val + 2 * q);
-        b = __builtin_nontemporal_load(p.val + 2 * q + 1);
+    __device__ inline void load(const ChunkRsrc &p, uint32_t e) {
+        r = __builtin_bit_cast(u2v, __builtin_amdgcn_raw_buffer_load_b64(p.rows, e * 2u, 0, 0));
+        a = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(p.val, e * 8u, 0, 0));
+        b = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(p.val, e * 8u + 16u, 0, 0));
     }
     __device__ inline double value(int j) const { return j < 2 ? a[j] : b[j - 2]; }
     __device__ inline void settle() const { asm volatile("" ::"v"(r), "v"(a), "v"(b)); }
@@ -340,6 +354,15 @@ struct Cursor {
     }
 };
 
+// One turn of the prefetch ring: D + 1 steps, step J consuming register set
+// J while the load D batches ahead goes into set (J + D) mod (D + 1).  The
+// sets are indexed by constants only, so they stay in registers.
+template <int J, int D, class Step, class Q>
+__device__ inline void ring_steps(Step &step, Q (&q)[D + 1], bool (&ok)[D + 1]) {
+    step(q[J], ok[J], q[(J + D) % (D + 1)], ok[(J + D) % (D + 1)]);
+    if constexpr (J < D) ring_steps<J + 1, D>(step, q, ok);
+}
+
 // The SpMM work-item kernel.  A group of L lanes (one DPP row or half of
 // one) holds E = L/M entry slots x M vectors: lane v + M*e loads quad e of
 // the group's batch (the M lanes of a slot load the same address) and
@@ -391,19 +414,20 @@ __global__ __launch_bounds__(BS) void k_pcsc(const SpmmItem *__restrict__ items,
                 for (int u = 0; u < M; ++u) xcol[i * M + u] = xr[u];
             }
         }
-        ItemPtrs P;
-        P.rows = (const GAS u2v *)(jb.lrow + base);
-        P.hi = (const GAS f4v *)(jb.hi + base);
-        P.lo = (const GAS f4v *)(jb.lo + base);
-        P.val = (const GAS d2v *)(jb.val + base);
+        const uint32_t cn = (uint32_t)(gst[cs + nsc] - base);  // entries in the chunk
+        ChunkRsrc P;
+        P.rows = rsrc(jb.lrow + base, cn * 2u);
+        P.hi = rsrc(jb.hi + base, cn * 4u);
+        P.lo = rsrc(jb.lo + base, cn * 4u);
+        P.val = rsrc(jb.val + base, cn * 8u);
         __syncthreads();
 
         // Main loop.  The load cursor runs D batches ahead of the consume
         // cursor through a ring of D + 1 register sets, so D batches stay in
-        // flight while one is consumed.  Every step issues its load (clamped
-        // to the chunk's first quad once the group is done), so all paths
-        // carry the same loads and the same waits; the only global accesses
-        // in the loop are these loads.
+        // flight while one is consumed.  Every step issues its load (out of
+        // range, reading zeros, once the group is done), so all paths carry
+        // the same loads and the same waits; the only global accesses in
+        // the loop are these loads.
         Cursor cc{g, 0, 0}, lc{g, 0, 0};
         if (g < nsc) {
             cc.k = lc.k = sbeg[g];
@@ -414,7 +438,7 @@ __global__ __launch_bounds__(BS) void k_pcsc(const SpmmItem *__restrict__ items,
         bool ok[D + 1];
         auto issue = [&](Quad<MODE> &dst, bool &dok) {
             dok = lc.i < nsc && lc.k + 4 * e < lc.e1;
-            dst.load(P, dok ? (uint32_t)(lc.k + 4 * e) >> 2 : 0u);
+            dst.load(P, dok ? (uint32_t)(lc.k + 4 * e) : OFF_NONE);
             if (lc.i < nsc) lc.advance<E, G>(sbeg, nsc);
         };
 #pragma unroll
@@ -434,18 +458,7 @@ __global__ __launch_bounds__(BS) void k_pcsc(const SpmmItem *__restrict__ items,
                 }
             }
         };
-        while (cc.i < nsc) {
-            if constexpr (D == 2) {
-                step(q[0], ok[0], q[2], ok[2]);
-                step(q[1], ok[1], q[0], ok[0]);
-                step(q[2], ok[2], q[1], ok[1]);
-            } else {
-                step(q[0], ok[0], q[3], ok[3]);
-                step(q[1], ok[1], q[0], ok[0]);
-                step(q[2], ok[2], q[1], ok[1]);
-                step(q[3], ok[3], q[2], ok[2]);
-            }
-        }
+        while (cc.i < nsc) ring_steps<0, D>(step, q, ok);
 #pragma unroll
         for (int d = 0; d <= D; ++d) q[d].settle();
         __syncthreads();
@@ -510,8 +523,11 @@ void launch_depth(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs
 template <int M, int MODE, int L, int BS>
 void launch_shape(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh,
                   hipStream_t s) {
-    if (sh.depth == 2) launch_depth<M, MODE, L, BS, 2>(d_items, nitems, d_jobs, sh, s);
-    else launch_depth<M, MODE, L, BS, 3>(d_items, nitems, d_jobs, sh, s);
+    switch (sh.depth) {
+        case 3: launch_depth<M, MODE, L, BS, 3>(d_items, nitems, d_jobs, sh, s); break;
+        case 4: launch_depth<M, MODE, L, BS, 4>(d_items, nitems, d_jobs, sh, s); break;
+        default: launch_depth<M, MODE, L, BS, 6>(d_items, nitems, d_jobs, sh, s); break;
+    }
 }
 
 template <int M, int MODE, int L>
@@ -545,10 +561,10 @@ int env_int(const char *name, int dflt) {
 }  // namespace
 
 bool valid_shape(const SpmmShape &sh) {
-    // the instantiated launch shapes: L in {8, 16}, 256 or 512 threads, depth 2 or 3
+    // the instantiated launch shapes: L in {8, 16}, 256 or 512 threads, depth 3, 4 or 6
     return sh.R >= 64 && sh.R <= 4096 && (sh.R & (sh.R - 1)) == 0 && (sh.L == 8 || sh.L == 16) &&
            (sh.block == 256 || sh.block == 512) && sh.target >= 4096 && sh.nseg >= 64 && sh.nseg <= 4096 &&
-           (sh.depth == 2 || sh.depth == 3) && lds_map<1, double>(sh.R, sh.nseg).total <= LDS_MAX;
+           (sh.depth == 3 || sh.depth == 4 || sh.depth == 6) && lds_map<1, double>(sh.R, sh.nseg).total <= LDS_MAX;
 }
 
 SpmmShape spmm_shape() {

@@ -181,7 +181,7 @@ int gci_spmm_autotune(gci_hmat *const *mats, int nmats, int m, int mode, int rep
         set_spmm_shape(cur);
         double tbest = ts.time(reps);
         const std::vector<int> Ls{8, 16}, blocks{256, 512}, nsegs{256, 512, 1024, 2048},
-            Rs{256, 512, 1024, 2048, 4096}, depths{2, 3};
+            Rs{256, 512, 1024, 2048, 4096}, depths{3, 4, 6};
         const std::vector<int64_t> targets{65536, 131072, 262144, 524288};
         auto try_shape = [&](const SpmmShape &cand) {
             if (!fits(cand, m, mode)) return;

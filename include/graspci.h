@@ -222,7 +222,7 @@ int gci_profile_read(double *stats);
  * SpMM's launch shape: panel height R (rows whose x and y stay in LDS; the
  * panel-CSC layout is built for it), lanes per column segment L (8, 16),
  * workgroup size (256, 512), segments per LDS chunk `nseg`, batches each
- * lane keeps in flight `depth` (2, 3), and about `target` entries per
+ * lane keeps in flight `depth` (3, 4, 6), and about `target` entries per
  * work item (a slice of one panel).  One shape is in force per process; a
  * matrix laid out for another R is laid out again at its next use outside a
  * Davidson engine.  Set or tune the shape before solves, not while requests
