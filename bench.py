@@ -42,6 +42,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_pcsc_batched64_traffic.json")
+
+
+def _traffic():
+    """HBM bytes per bulk SpMM launch on this workload, from the FETCH_SIZE and
+    WRITE_SIZE counter passes (scaled by the copy-kernel calibration) recorded
+    in profiles/; None when no measurement is recorded."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return int(json.load(f)["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +71,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run only a plain elementwise kernel that reads and writes 1 GiB (the scale for the "
+                         "FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -65,6 +82,17 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.calibrate:
+        n = 1 << 28  # float32: 1 GiB read + 1 GiB written per launch
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "torch.neg float32", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                          "launches": 3}), flush=True)
+        return
 
     from grasp_amd import _lib as L
     from grasp_amd import rci, synth
@@ -122,7 +150,7 @@ def main():
         avg_bytes = byts[bulk] / launches[bulk]
         achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(),
                 "kernel": "k_pcsc (panel-CSC symmetric SpMM, bulk fp32 hi, m=4)",
                 "launches": int(launches[bulk]), "avg_ms": round(avg_ms, 4),
                 "alg_bytes_per_launch": round(avg_bytes),
