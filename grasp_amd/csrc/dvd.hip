@@ -387,6 +387,13 @@ __global__ __launch_bounds__(256) void k_precond(DvdProb *P, const int *chunk_pr
 //      1 = (basis[:,kpass:+nnew], itself)                 (Cholesky-QR)
 //      2 = (basis[:,0:kpass+nnew], ab[:,kpass:+nnew])     (ADDS)
 //      3 = (ab[:,0:nume] - eigval*basis[:,0:nume]) norms  (exit residuals)
+// Sel 0-2 are tall-skinny A^T B products on the FP64 matrix cores: the
+// chunk is staged through LDS 64 rows at a time and each wave accumulates
+// 16 x 16 output tiles (16 columns of A x up to 16 of B) with
+// v_mfma_f64_16x16x4f64, 4 rows per instruction, rows in ascending order,
+// so every output is summed in the same order on every run.
+typedef double d4v __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
                                               double *partial, int sel, int want_phase) {
     DvdProb &p = P[chunk_prob[blockIdx.x]];
@@ -401,22 +408,65 @@ __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob,
     else if (sel == 2) { A = p.basis; ka = k + nn; B = p.ab + (int64_t)k * p.n; kb = nn; }
     else { A = p.basis; ka = p.nume; B = p.ab; kb = p.nume; }
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    if (sel != 3) {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int nti = (ka + 15) >> 4;  // 16-column tiles of A (<= 5); B fits one tile (kb <= 16)
+        d4v acc[2];
+        acc[0] = d4v{0.0, 0.0, 0.0, 0.0};
+        acc[1] = acc[0];
+        for (int64_t rt = r0; rt < r1; rt += 64) {
+            const int rows = (int)min<int64_t>(64, r1 - rt);
+            for (int idx = threadIdx.x; idx < 64 * ka; idx += blockDim.x) {
+                const int rr = idx & 63, i = idx >> 6;
+                ta[rr][i] = rr < rows ? A[rt + rr + (int64_t)i * p.n] : 0.0;
+            }
+            for (int idx = threadIdx.x; idx < 64 * kb; idx += blockDim.x) {
+                const int rr = idx & 63, j = idx >> 6;
+                tb[rr][j] = rr < rows ? B[rt + rr + (int64_t)j * p.n] : 0.0;
+            }
+            __syncthreads();
+            // operands: lane l holds A^T[i = l%16][r = l/16] and B[r = l/16][j = l%16]
+            const int j = lane & 15;
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int it = wave + 4 * t;
+                if (it < nti) {
+                    const int i = it * 16 + (lane & 15);
+#pragma unroll 4
+                    for (int ks = 0; ks < 16; ++ks) {
+                        const int r = ks * 4 + (lane >> 4);
+                        const double av = i < ka ? ta[r][i] : 0.0;
+                        const double bv = j < kb ? tb[r][j] : 0.0;
+                        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // result: lane l, element q holds D[l/16 + 4q][l%16]
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int it = wave + 4 * t;
+            if (it >= nti) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = it * 16 + (lane >> 4) + 4 * q, jj = lane & 15;
+                if (i < ka && jj < kb) partial[(int64_t)blockIdx.x * PSTRIDE + i * kb + jj] = acc[t][q];
+            }
+        }
+        return;
+    }
+    // sel 3: squared column norms of AB - lambda B (diagonal only), FP64 VALU
     const int nout = ka * kb;
     constexpr int NQ = (PSTRIDE + 255) / 256;
-    double acc[NQ];  // outputs t, t+256, ...
+    double accv[NQ];  // outputs t, t+256, ...
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    for (int q = 0; q < NQ; ++q) accv[q] = 0.0;
     for (int64_t rt = r0; rt < r1; rt += 64) {
         const int rows = (int)min<int64_t>(64, r1 - rt);
-        for (int idx = threadIdx.x; idx < rows * ka; idx += blockDim.x) {
-            const int rr = idx % rows, i = idx / rows;
-            ta[rr][i] = A[rt + rr + (int64_t)i * p.n];
-        }
         for (int idx = threadIdx.x; idx < rows * kb; idx += blockDim.x) {
             const int rr = idx % rows, j = idx / rows;
-            double v = B[rt + rr + (int64_t)j * p.n];
-            if (sel == 3) v = fma(-p.eigval[j], A[rt + rr + (int64_t)j * p.n], v);
-            tb[rr][j] = v;
+            tb[rr][j] = fma(-p.eigval[j], A[rt + rr + (int64_t)j * p.n], B[rt + rr + (int64_t)j * p.n]);
         }
         __syncthreads();
 #pragma unroll
@@ -424,14 +474,10 @@ __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob,
             const int o = threadIdx.x + q * 256;
             if (o < nout) {
                 const int i = o / kb, j = o % kb;
-                double s = acc[q];
-                if (sel == 3) {
-                    if (i == j)
-                        for (int rr = 0; rr < rows; ++rr) s = fma(tb[rr][j], tb[rr][j], s);
-                } else {
-                    for (int rr = 0; rr < rows; ++rr) s = fma(ta[rr][i], tb[rr][j], s);
-                }
-                acc[q] = s;
+                double sum = accv[q];
+                if (i == j)
+                    for (int rr = 0; rr < rows; ++rr) sum = fma(tb[rr][j], tb[rr][j], sum);
+                accv[q] = sum;
             }
         }
         __syncthreads();
@@ -439,7 +485,7 @@ __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob,
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int o = threadIdx.x + q * 256;
-        if (o < nout) partial[(int64_t)blockIdx.x * PSTRIDE + o] = acc[q];
+        if (o < nout) partial[(int64_t)blockIdx.x * PSTRIDE + o] = accv[q];
     }
 }
 
@@ -1275,6 +1321,7 @@ void DavidsonEngine::refine_restart(const std::vector<int> &slots) {
 
 void DavidsonEngine::extract(int slot, double *eval, double *evec, double *resid, gci_dvd_stats *st) {
     DvdProb p;
+    GCI_HIP(hipStreamSynchronize(stream_));  // the engine's stream does not sync with the legacy one
     GCI_HIP(hipMemcpy(&p, &dprob_.p[slot], sizeof(DvdProb), hipMemcpyDeviceToHost));
     for (int q = 0; q < p.nume; ++q) {
         if (eval) eval[q] = p.eigval[q];

@@ -119,7 +119,18 @@ void solve_one(HMat &h, const DvdSpec &sp, const double *init, int niv, double *
         init = iv.data();
         niv = sp.nume;
     }
-    DavidsonEngine eng(1, h.stream);
+    // the engine gets its own non-blocking stream (it captures graphs on it);
+    // the matrix's data were completed on the matrix's stream
+    hipStream_t es = nullptr;
+    GCI_HIP(hipStreamCreateWithFlags(&es, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    } guard{es};
+    DavidsonEngine eng(1, es);
     std::vector<int> slots = eng.admit({sp}, {init}, {niv});
     run_to_completion(eng, 2 * sp.maxiter + 128);
     eng.extract(slots[0], eval, evec, resid, st);
@@ -156,6 +167,7 @@ int gci_maneig(gci_hmat *h, int nvec, double *eval, double *evec, gci_dvd_stats 
         gci_dvd_stats local{};
         if (h->n == 1) {
             // trivial case (maneig.f90:136-156)
+            GCI_HIP(hipStreamSynchronize(h->stream));
             GCI_HIP(hipMemcpy(eval, h->diag.p, sizeof(double), hipMemcpyDeviceToHost));
             evec[0] = 1.0;
         } else if (h->n <= 2000) {
@@ -194,7 +206,9 @@ int gci_batch_solve(int nblocks, gci_hmat **blocks, const int *nev, const double
         static std::unique_ptr<DavidsonEngine> eng_cache;
         static hipStream_t stream = nullptr;
         std::lock_guard<std::mutex> lk(mu);
-        if (!stream) GCI_HIP(hipStreamCreate(&stream));
+        // non-blocking: the engine captures graphs on it, and a capture must
+        // not be joined by other threads' legacy-stream calls
+        if (!stream) GCI_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         if (!eng_cache || eng_cache->capacity() < nblocks)
             eng_cache = std::make_unique<DavidsonEngine>(std::max(nblocks, 64), stream);
         DavidsonEngine &eng = *eng_cache;
@@ -261,10 +275,12 @@ int gci_batch_solve(int nblocks, gci_hmat **blocks, const int *nev, const double
 
 int gci_profile_enable(int on) {
     return guarded([&] {
+        GCI_HIP(hipDeviceSynchronize());  // engines run on non-blocking streams
         OpProfile &pf = op_profile();
         pf = OpProfile{};
         pf.on = on != 0;
         GCI_HIP(hipMemset(op_bytes_device(), 0, 3 * sizeof(double)));
+        GCI_HIP(hipDeviceSynchronize());
     });
 }
 

@@ -182,7 +182,9 @@ int gci_server_create(size_t hbm_budget_bytes, int max_active, gci_server **out)
         auto s = std::make_unique<gci_server>();
         s->budget = hbm_budget_bytes;
         s->max_active = max_active;
-        GCI_HIP(hipStreamCreate(&s->stream));
+        // non-blocking: the worker captures graphs on it while the caller's
+        // thread uploads matrices through the legacy stream
+        GCI_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         s->eng = std::make_unique<DavidsonEngine>(max_active, s->stream);
         gci_server *raw = s.release();
         raw->worker = std::thread(worker_loop, raw);
