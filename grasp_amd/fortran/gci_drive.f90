@@ -84,7 +84,7 @@ program gci_drive
       write (v, '(2i12)') nint, j2max
       write (v, '(i12)') kstart(0:j2max + 1)
       do i = 1, nint
-         write (v, '(i12, es25.17e3)') indteirk(i), valteirk(i)
+         write (v, '(i12, 1x, es25.17e3)') indteirk(i), valteirk(i)
       end do
    end if
    close (v)
