@@ -42,7 +42,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_pcsc_batched64_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sell_batched64_traffic.json")
 
 
 def _traffic():
@@ -217,7 +217,7 @@ def main():
         achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(),
-                "kernel": "k_pcsc (panel-CSC symmetric SpMM, bulk fp32 hi, m=4)",
+                "kernel": "k_sell (panel-slice symmetric SpMM, bulk fp32 hi, m=4)",
                 "launches": int(launches[bulk]), "avg_ms": round(avg_ms, 4),
                 "alg_bytes_per_launch": round(avg_bytes),
                 "refine_fp64_achieved_GBs": round(byts[2] / (ms[2] * 1e-3) / 1e9, 1) if ms[2] > 0 else None}
@@ -294,18 +294,20 @@ def main():
         sync(0)
         t0 = time.perf_counter()
         tnext = t0
+        late = []  # how long after its scheduled arrival each request was submitted
         for i in range(nreq):
             tnext += gaps[i]
             while time.perf_counter() < tnext:
                 pass
+            late.append(time.perf_counter() - tnext)
             b, e = mats[i]
             pending.append((srv.submit(b, int(nevs[i]), float(tols[i]), e), int(nevs[i])))
         srv.drain()
         t_serve = time.perf_counter() - t0
-        lat = []
-        for rid, k in pending:
+        lat = []  # scheduled arrival to completion
+        for (rid, k), d in zip(pending, late):
             r = srv.poll(rid)
-            lat.append(r[2])
+            lat.append(r[2] + d)
         srv.close()
         lat = np.array(lat)
         out["serving"] = {"requests": nreq, "arrival_rate_per_s": args.serve_rate,

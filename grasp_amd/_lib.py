@@ -48,7 +48,7 @@ class GciPairLists(C.Structure):
 
 
 class GciSpmmShape(C.Structure):
-    _fields_ = [("R", C.c_int), ("L", C.c_int), ("block", C.c_int), ("nseg", C.c_int), ("depth", C.c_int),
+    _fields_ = [("R", C.c_int), ("block", C.c_int), ("nslice", C.c_int), ("depth", C.c_int),
                 ("target", C.c_int64)]
 
 
@@ -71,6 +71,7 @@ EXPORTS = [
     "gci_server_destroy", "gci_server_submit", "gci_server_poll", "gci_server_drain",
     "gci_profile_enable", "gci_profile_read", "gci_spmm_get_shape", "gci_spmm_set_shape", "gci_spmm_time",
     "gci_spmm_autotune",
+    "gci_spmm_layout",
 ]
 
 

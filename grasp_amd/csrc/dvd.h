@@ -32,8 +32,8 @@ struct DvdProb {
     const double *val;
     const float *hi, *lo;
     const double *diagH;            // FP64 diagonal of H (shifted)
-    const int32_t *pseg_col;        // panel-CSC view used by the operator
-    const int64_t *pseg_start;
+    const int32_t *pscol;           // panel-slice view used by the operator
+    const int64_t *psstep;
     const uint16_t *plrow;
     const double *pval;
     const float *phi, *plo;

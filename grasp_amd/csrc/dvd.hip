@@ -708,8 +708,9 @@ __global__ void k_make_jobs(DvdProb *P, int nprob, SpmmJob *jobs, int want_phase
     SpmmJob j{};
     j.active = (p.phase == want_phase && op_cols(p, c0, cb) > 0) ? 1 : 0;
     if (j.active) {
-        j.seg_col = p.pseg_col;
-        j.seg_start = p.pseg_start;
+        j.scol = p.pscol;
+        j.sstep = p.psstep;
+        j.n = p.n;
         j.lrow = p.plrow;
         j.val = p.pval;
         j.hi = p.phi;
@@ -1061,8 +1062,8 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         p.rc = p.xc + (size_t)n * nume;
         p.xs = p.rc + (size_t)n * nume;
         p.ys = static_cast<double *>(p.xs) + (size_t)n * 8;
-        p.pseg_col = sp.h->pc.seg_col.p;
-        p.pseg_start = sp.h->pc.seg_start.p;
+        p.pscol = sp.h->pc.scol.p;
+        p.psstep = sp.h->pc.sstep.p;
         p.plrow = sp.h->pc.lrow.p;
         p.pval = sp.h->pc.val.p;
         p.phi = sp.h->pc.hi.p;

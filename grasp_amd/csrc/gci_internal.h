@@ -78,46 +78,80 @@ struct DBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
-// Panel-CSC view of the strictly-lower off-diagonal part, the layout the
-// symmetric SpMM streams: rows are cut into panels of R rows; inside a panel
-// the entries are sorted by column, then row, so every column that the panel
-// touches is one contiguous segment.  A segment's scatter contribution
-// (H^T side) is reduced in registers and issued once; the panel's own rows
-// (H side) accumulate in LDS.
+// Keeps up to a few GiB of freed stream-ordered memory in the device's
+// default pool (once per device), so temporaries are reused without returning
+// to the driver.
+void retain_pool();
+
+// Stream-ordered temporary device buffer (hipMallocAsync / hipFreeAsync on
+// one stream): allocation and release queue behind the stream's work and
+// never synchronise the device, unlike hipFree.  For build temporaries that
+// only that stream touches.
+template <class T>
+struct SBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipStream_t s = nullptr;
+    SBuf(size_t count, hipStream_t stream) : s(stream) {
+        if (count == 0) count = 1;
+        retain_pool();
+        if (hipMallocAsync(reinterpret_cast<void **>(&p), count * sizeof(T), s) != hipSuccess) {
+            p = nullptr;
+            throw Error{GCI_ERR_NOMEM, "hipMallocAsync of " + std::to_string(count * sizeof(T)) + " bytes failed"};
+        }
+        n = count;
+    }
+    SBuf(const SBuf &) = delete;
+    SBuf &operator=(const SBuf &) = delete;
+    ~SBuf() {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+};
+
+// Panel-slice view of the strictly-lower off-diagonal part, the layout the
+// symmetric SpMM streams.  Rows are cut into panels of R rows; inside a panel
+// the entries of one column form a segment.  A panel's segments, longest
+// first, are dealt in groups of 64 (one per wavefront lane) into slices, and
+// a slice is stored step-major: step k holds entries 4k..4k+3 of each of its
+// 64 segments (256 entries), padded with zero-valued entries to the slice's
+// longest segment.  The steps of all slices form one stream; a slice is a run
+// of steps with the same 64 columns.
 struct SpmmItem {
-    int64_t seg0, seg1;  // segment range of one work item (a slice of a panel)
-    int64_t row0;        // first row of the panel
-    int32_t rows;        // panel height
-    int32_t job;         // owning job (filled per launch)
+    int64_t step0, step1;    // step range of one work item (inside one panel)
+    int64_t slice0, slice1;  // slices overlapping it
+    int64_t row0;            // first row of the panel
+    int32_t rows;            // panel height
+    int32_t job;             // owning job (filled per launch)
 };
 
 struct SpmmJob;
 
-// Launch shape of the SpMM: panel height R (a power of two; the layout is
-// built for it), lanes per segment group L (8 or 16), workgroup size,
-// entries per work item, segments per LDS chunk, and prefetch depth.
+// Launch shape of the SpMM: panel height R (1024 or 2048; the layout is
+// built for it), workgroup size, entries per work item, slices per chunk
+// (the most whose tables a launch stages in LDS at once; fewer when the
+// launch width leaves less LDS), and prefetch depth (steps each wavefront
+// keeps in flight).
 struct SpmmShape {
     int R = 1024;
-    int L = 16;       // lanes per segment group: L/M entry slots x M vectors
     int block = 512;
-    int64_t target = 65536;
-    int nseg = 512;   // segments per chunk (their tables are staged in LDS)
-    int depth = 4;    // batches each lane keeps in flight
+    int64_t target = 131072;
+    int nslice = 32;
+    int depth = 4;
 };
 
 struct Pcsc {
-    int R = 0;                 // rows per panel (0 = not built)
-    int64_t nseg = 0, nnz = 0;
-    int64_t npad = 0;          // stored entries including segment padding
-    int64_t target = 0;        // entries per work item the items were cut for
-    DBuf<int32_t> seg_col;     // nseg
-    DBuf<int64_t> seg_start;   // nseg + 1, multiples of 4
-    DBuf<uint16_t> lrow;       // npad, row - panel start
-    DBuf<double> val;          // npad (FP64 values)
-    DBuf<float> hi, lo;        // npad (split values)
-    std::vector<int64_t> seg_start_h;  // host copies for cutting work items
-    std::vector<int32_t> seg_panel_h;
-    std::vector<SpmmItem> items;  // host copy of the work items
+    int R = 0;                  // rows per panel (0 = not built)
+    int64_t nseg = 0, nnz = 0;  // segments, stored (unpadded) entries
+    int64_t nslice = 0, nstep = 0;
+    int64_t target = 0;         // entries per work item the items were cut for
+    DBuf<int32_t> scol;         // nslice x 64 columns (-1: empty lane)
+    DBuf<int64_t> sstep;        // nslice + 1 first steps
+    DBuf<uint16_t> lrow;        // nstep x 256: (row - panel start) x 8
+    DBuf<double> val;           // nstep x 256 (FP64 values, pairs of lanes' halves)
+    DBuf<float> hi, lo;         // nstep x 256 (split values)
+    std::vector<int64_t> sstep_h;   // host copies for cutting work items
+    std::vector<int32_t> spanel_h;  // panel of each slice
+    std::vector<SpmmItem> items;    // host copy of the work items
     // single-matrix operator state: staging (n x stage_M), job table, items
     DBuf<char> stage_x;
     DBuf<double> stage_y;
@@ -125,14 +159,14 @@ struct Pcsc {
     DBuf<SpmmJob> d_jobs;
     DBuf<SpmmItem> d_items;
     size_t bytes() const {
-        return seg_col.bytes() + seg_start.bytes() + lrow.bytes() + val.bytes() + hi.bytes() + lo.bytes() +
+        return scol.bytes() + sstep.bytes() + lrow.bytes() + val.bytes() + hi.bytes() + lo.bytes() +
                stage_x.bytes() + stage_y.bytes();
     }
 };
 
 // Device-resident block Hamiltonian: off-diagonal lower triangle in CSR
 // (0-based columns, strictly below the diagonal, ascending), FP64 diagonal,
-// and the panel-CSC copy the SpMM uses.
+// and the panel-slice copy the SpMM uses.
 struct HMat {
     int64_t n = 0;
     int64_t nnz = 0;          // off-diagonal entries
@@ -149,23 +183,24 @@ struct HMat {
     }
 };
 
-// One SpMM job: y = H x for m (<= 8) columns of a matrix, through
-// row-major staging buffers xs (fp32 for the bulk operator, else fp64) and
-// ys (fp64), both n x M with M the launch width.
+// One SpMM job: y = H x for m (<= 8) columns of a matrix of n rows,
+// through row-major staging buffers xs (fp32 for the bulk operator, else
+// fp64) and ys (fp64), both n x M with M the launch width.
 struct SpmmJob {
-    const int32_t *seg_col;
-    const int64_t *seg_start;
+    const int32_t *scol;
+    const int64_t *sstep;
     const uint16_t *lrow;
     const double *val;
     const float *hi, *lo;
     const void *xs;
     double *ys;
+    int64_t n;
     int m, mode, active, pad;
 };
 
 // The launch shape in force (defaults, overridable once through the
-// GCI_SPMM_R / _L / _BLOCK / _TARGET / _NSEG / _DEPTH environment variables) and its
-// setter.
+// GCI_SPMM_R / _BLOCK / _TARGET / _NSLICE / _DEPTH environment variables) and
+// its setter.
 SpmmShape spmm_shape();
 void set_spmm_shape(const SpmmShape &sh);
 bool valid_shape(const SpmmShape &sh);
@@ -190,7 +225,7 @@ uint64_t spmm_shape_version();
 void build_items(HMat &h, int64_t target);
 // Launch width (1, 2, 4 or 8) for m columns.
 int spmm_width(int m);
-// The panel-CSC kernel over a list of work items whose jobs live in device
+// The panel-slice kernel over a list of work items whose jobs live in device
 // memory (jobs[item.job]); stream-ordered, no synchronisation.
 void spmm_items(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, int M, int mode, hipStream_t s);
 // Single-matrix SpMM on device pointers (column-major x, y).

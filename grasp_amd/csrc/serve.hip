@@ -216,6 +216,10 @@ int gci_server_submit(gci_server *s, gci_hmat *h, int nev, double tol, double ea
         GCI_REQUIRE(nev >= 1 && nev <= MAXNUME && h->n >= 2 && 2 * nev + 60 <= MAXLIM + 60,
                     "gci_server_submit: bad nev or block order");
         GCI_REQUIRE(tol > 0.0, "gci_server_submit: tolerance must be positive");
+        const Clock::time_point t0 = Clock::now();
+        // the SpMM layout is built here, on the caller's thread and the
+        // matrix's stream, so the worker's engine steps never wait for it
+        ensure_items(*h);
         auto r = std::make_unique<Request>();
         r->h = h;
         r->own = true;
@@ -223,7 +227,7 @@ int gci_server_submit(gci_server *s, gci_hmat *h, int nev, double tol, double ea
         r->tol = tol;
         r->eav = eav;
         r->bytes = h->device_bytes() + DavidsonEngine::workspace_bytes(spec_for(*r));
-        r->t_submit = Clock::now();
+        r->t_submit = t0;
         std::lock_guard<std::mutex> lk(s->mu);
         GCI_REQUIRE(s->fatal.empty(), "gci_server_submit: server failed: " + s->fatal);
         if (r->bytes > s->budget) {

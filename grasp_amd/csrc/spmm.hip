@@ -3,27 +3,34 @@
 // Replaces SPICMV2/DMERGE (lib9290/spicmv2.f90:40-56, dmerge.f90:36-40):
 // y = H x with H symmetric and only the strictly-lower off-diagonal part
 // stored, plus an FP64 diagonal.  Every stored entry H_ij (i > j) acts twice:
-// a gather into row i and the symmetric scatter into row j.
+// y_i += H_ij x_j, and the symmetric y_j += H_ij x_i.
 //
-// Layout (panel-CSC, built once per matrix on the device): rows are cut into
-// panels of R rows; inside a panel the entries are sorted by column, then
-// row, so each column the panel touches is one contiguous segment.  Segments
-// are padded to a multiple of 4 entries (pad entries: value 0, row of the
-// segment's last entry) so that every lane reads whole aligned quads.  Per
-// entry the stream carries only the value (fp32 hi, hi+lo or fp64) and a
-// 16-bit row offset; per segment its column and start.
+// Layout (panel slices, built once per matrix on the device).  Rows are cut
+// into panels of R rows; inside a panel the entries of one column form a
+// segment.  A panel's segments are sorted by length, longest first, and dealt
+// in groups of 64 -- one per wavefront lane -- into slices.  A slice is stored
+// step-major: step k holds entries 4k..4k+3 of each of its 64 segments, so a
+// step is 256 entries that a wavefront reads with one 16-byte load per lane
+// (fp32 values) and one 8-byte load (row offsets).  A slice has as many steps
+// as its longest segment needs; the other lanes are padded with zero-valued
+// entries.  Per entry the stream carries the value (fp32 hi, hi+lo, or fp64)
+// and a 16-bit row code, the LDS byte offset (row - panel start) x 8; per
+// slice, its 64 columns and first step.
 //
-// Kernel: one workgroup per work item (a slice of one panel's segments).
-// The panel's rows of x are staged in LDS (fp32 for the bulk operator) and
-// its row accumulators y_i (fp64) live in LDS.  Groups of L lanes walk
-// segments; lane l of a group owns entries 4l..4l+3 of each 4L-entry batch
-// and applies each entry to all M vectors: one LDS vector read of x_i, M
-// FMAs into the lane's column partials, M LDS atomics into y_i.  At a
-// segment's end the group reduces its partials (DPP) and issues one M-wide
-// atomic add into y_j.  Loads of the next batch, the next segment's x_j and
-// the metadata two segments ahead are issued before the current batch is
-// consumed.  x and y pass through row-major staging buffers so that the
-// x_j reads and the y atomics are line-sized.
+// Inside a segment the entries are ordered by (row mod 32, row) and lane l's
+// segment begins at its first entry of row class >= l mod 32.  Lanes of one
+// slice have segments of nearly equal length, so at every step their rows
+// fall in nearly distinct classes: the 64 LDS addresses of one row gather or
+// scatter instruction spread over the banks.
+//
+// Kernel: one workgroup per work item (a run of one panel's steps).  The
+// panel's rows of x (fp32 for the bulk operator, fp64 otherwise) and its row
+// accumulators y_i (fp64) live in LDS, vector-major.  The item's steps are
+// split evenly over the wavefronts, and a wavefront walks its steps with the
+// next `depth` steps' loads in flight.  Lane l holds its segment's x_j and
+// column partials sum_i H_ij x_i in registers; per entry it reads x_i from
+// LDS and adds H_ij x_j into y_i with an LDS atomic.  At a slice end the
+// column partials leave as global atomics, at the item end the panel's rows.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -38,13 +45,20 @@ namespace {
 #define GAS __attribute__((address_space(1)))
 
 constexpr size_t LDS_MAX = 160 * 1024;  // LDS per workgroup on gfx950
+constexpr int SW = 64;                  // lanes (segments) per slice
+constexpr int SE = SW * 4;              // entries per step
 
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d2v __attribute__((ext_vector_type(2)));
 typedef unsigned u2v __attribute__((ext_vector_type(2)));
 
-// ------------------------------------------------------ panel-CSC build
+// ------------------------------------------------------ panel-slice build
+
+// Row code of the sort key: orders a segment's rows by (row mod 32, row).
+// Valid for rows below 65536.
+__host__ __device__ inline uint32_t row_code(uint32_t lr) { return ((lr & 31u) << 11) | (lr >> 5); }
+__host__ __device__ inline uint32_t code_row(uint32_t c) { return ((c & 0x7FFu) << 5) | (c >> 11); }
 
 __global__ void k_keys(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int R,
                        uint64_t *__restrict__ keys, int64_t *__restrict__ idx) {
@@ -54,7 +68,7 @@ __global__ void k_keys(int64_t n, const int64_t *__restrict__ rp, const int32_t 
     if (row >= n) return;
     const uint64_t panel = (uint64_t)(row / R), lr = (uint64_t)(row % R);
     for (int64_t e = rp[row] + lane; e < rp[row + 1]; e += 64) {
-        keys[e] = (panel << 40) | ((uint64_t)col[e] << 16) | lr;
+        keys[e] = (panel << 40) | ((uint64_t)col[e] << 16) | row_code((uint32_t)lr);
         idx[e] = e;
     }
 }
@@ -62,11 +76,6 @@ __global__ void k_keys(int64_t n, const int64_t *__restrict__ rp, const int32_t 
 __global__ void k_seg_flags(int64_t nnz, const uint64_t *__restrict__ keys, int32_t *__restrict__ flag) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x)
         flag[e] = (e == 0 || (keys[e] >> 16) != (keys[e - 1] >> 16)) ? 1 : 0;
-}
-
-// segment s of entry e = (exclusive count of flags before e) + flag[e] - 1
-__device__ inline int64_t seg_of(const int32_t *flag, const int64_t *segid, int64_t e) {
-    return segid[e] + flag[e] - 1;
 }
 
 __global__ void k_seg_meta(int64_t nnz, const uint64_t *__restrict__ keys, const int32_t *__restrict__ flag,
@@ -82,51 +91,252 @@ __global__ void k_seg_meta(int64_t nnz, const uint64_t *__restrict__ keys, const
     }
 }
 
-// padded length of every segment (multiple of 4); plen[nseg] = 0
-__global__ void k_seg_plen(int64_t nseg, int64_t nnz, const int64_t *__restrict__ ustart, int64_t *__restrict__ plen) {
-    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s <= nseg; s += (int64_t)gridDim.x * blockDim.x) {
-        if (s == nseg) {
-            plen[s] = 0;
-        } else {
-            const int64_t len = (s + 1 < nseg ? ustart[s + 1] : nnz) - ustart[s];
-            plen[s] = (len + 3) & ~(int64_t)3;
+// segment s of entry e = (exclusive count of flags before e) + flag[e] - 1
+__device__ inline int64_t seg_of(const int32_t *flag, const int64_t *segid, int64_t e) {
+    return segid[e] + flag[e] - 1;
+}
+
+constexpr int SLACK = 4;  // positions a slice's schedule may add to its longest segment
+// positions the schedule of a slice whose longest segment has `len` entries
+// can use (a multiple of 4 at most len + SLACK + 3)
+__host__ __device__ inline int sched_cap(int len) { return len + SLACK + 3; }
+
+// per segment: its length and the key that sorts a panel's segments longest
+// first (ties by column)
+__global__ void k_seg_order(int64_t nseg, int64_t nnz, const int64_t *__restrict__ ustart,
+                            const int32_t *__restrict__ seg_panel, int32_t *__restrict__ ulen,
+                            uint64_t *__restrict__ okey, int64_t *__restrict__ oval) {
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nseg; s += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t len = (s + 1 < nseg ? ustart[s + 1] : nnz) - ustart[s];
+        ulen[s] = (int32_t)len;
+        okey[s] = ((uint64_t)seg_panel[s] << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)len);
+        oval[s] = s;
+    }
+}
+
+// per panel p (and p = npanel): its first segment (segments are sorted by
+// panel: a binary search) and its slice count, ceil(segments / 64)
+__global__ void k_panel_slices(int64_t npanel, int64_t nseg, const int32_t *__restrict__ seg_panel,
+                               int64_t *__restrict__ pfirst, int64_t *__restrict__ pslices) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p <= npanel;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        auto first = [&](int64_t q) {  // first segment of panel q or later
+            int64_t lo = 0, hi = nseg;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) / 2;
+                if (seg_panel[mid] < q) lo = mid + 1;
+                else hi = mid;
+            }
+            return lo;
+        };
+        const int64_t f = first(p);
+        pfirst[p] = f;
+        pslices[p] = p < npanel ? (first(p + 1) - f + SW - 1) / SW : 0;
+    }
+}
+
+// Deals the sorted segments into slices: the segment of rank r inside its
+// panel goes to slice (panel's first slice) + r / 64, lane r % 64.  Lane 0
+// holds the slice's longest segment and sizes its schedule buffer.
+__global__ void k_seg_place(int64_t nseg, const int64_t *__restrict__ order, const int32_t *__restrict__ seg_panel,
+                            const int32_t *__restrict__ seg_col, const int32_t *__restrict__ ulen,
+                            const int64_t *__restrict__ pfirst, const int64_t *__restrict__ pslice0,
+                            int32_t *__restrict__ scol, int64_t *__restrict__ slane_seg,
+                            int32_t *__restrict__ slice_panel, int64_t *__restrict__ seg_slice,
+                            int32_t *__restrict__ seg_lane, int64_t *__restrict__ slice_pcap) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nseg; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = order[r];
+        const int p = seg_panel[s];
+        const int64_t q = r - pfirst[p];
+        const int64_t sl = pslice0[p] + q / SW;
+        const int lane = (int)(q % SW);
+        scol[sl * SW + lane] = seg_col[s];
+        slane_seg[sl * SW + lane] = s;
+        seg_slice[s] = sl;
+        seg_lane[s] = lane;
+        if (lane == 0) {
+            slice_panel[sl] = p;
+            slice_pcap[sl] = (int64_t)SW * sched_cap(ulen[s]);
         }
     }
 }
 
-// Position of a segment's p-th entry in storage.  Quad q (entries 4q..4q+3)
-// is stored rotated by q: slot t holds entry 4q + ((t + q) & 3).  The lane
-// that loads quad q is lane q mod L of its group, and the kernel applies
-// slot j of every lane's quad in the same instruction.  So one instruction's
-// lanes of a group touch entries 4l + ((j + l) & 3) -- rows 5 apart, not 4 --
-// which spreads their y rows over the LDS banks.
-__host__ __device__ inline int64_t quad_slot(int64_t p) {
-    const int64_t q = p >> 2, t = p & 3;
-    return (q << 2) | ((t - q) & 3);
+// Stored position of entry t (0..3) of lane `lane` in step `step`: row codes
+// and fp32 values lane-major (4 per lane); fp64 values as two halves of the
+// step, entries 0-1 of all lanes, then entries 2-3, so each half is one
+// 16-byte load per lane.
+__host__ __device__ inline int64_t slot_pos(int64_t step, int lane, int t) { return step * SE + lane * 4 + t; }
+__host__ __device__ inline int64_t val_pos(int64_t step, int lane, int t) {
+    return step * SE + (t >> 1) * (SE / 2) + lane * 2 + (t & 1);
 }
 
-__global__ void k_seg_fill(int64_t nnz, int64_t nseg, const uint64_t *__restrict__ keys,
-                           const int32_t *__restrict__ flag, const int64_t *__restrict__ segid,
-                           const int64_t *__restrict__ ustart, const int64_t *__restrict__ pstart,
-                           const int64_t *__restrict__ perm, const double *__restrict__ val,
-                           const float *__restrict__ hi, const float *__restrict__ lo, uint16_t *__restrict__ lrow,
-                           double *__restrict__ pval, float *__restrict__ phi, float *__restrict__ plo) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s = seg_of(flag, segid, e);
-        const int64_t p = e - ustart[s];
-        const int64_t pe = pstart[s] + quad_slot(p);
-        const uint16_t lr = (uint16_t)(keys[e] & 0xFFFFull);
-        lrow[pe] = lr;
-        const int64_t src = perm[e];
-        if (pval) pval[pe] = val[src];
-        if (phi) {
-            phi[pe] = hi[src];
-            plo[pe] = lo[src];
+// Bank schedule of one slice.  An fp64 LDS access of row r falls on the bank
+// pair r mod 32 (the row class), and a 64-lane access is served as two
+// 32-lane halves.  One step position p is one LDS instruction over the 64
+// lanes; it is conflict-free when no two lanes of a half share a class.  Per
+// position, every lane with entries left asks for the class it has most
+// entries of; in each half a class goes to the asker with most entries left
+// (ties to the lower lane), and lanes turned away ask for their next class
+// still free, for up to NROUND rounds.  A lane still without a place whose
+// entries left would run past the slice's target length (its longest
+// segment + SLACK positions) takes its best class anyway, sharing a bank pair;
+// the other lanes without a place, and lanes whose segment is done, take pad
+// entries (value 0) of the classes left free.  So the slice grows by at most
+// SLACK positions, and most instructions are conflict-free.  The schedule
+// records each entry's position in its slice and the class of every pad; the
+// stream is written from these once the slices' first steps are known.
+constexpr int NCLS = 32;
+constexpr int NROUND = 4;
+
+__global__ __launch_bounds__(64) void k_sched(const int64_t *__restrict__ slane_seg, const int64_t *__restrict__ ustart,
+                                              const int32_t *__restrict__ ulen, const uint64_t *__restrict__ keys,
+                                              const int32_t *__restrict__ slice_panel, int R, int64_t n,
+                                              const int64_t *__restrict__ padoff, int64_t *__restrict__ slice_steps,
+                                              int32_t *__restrict__ qpos, uint8_t *__restrict__ padcls,
+                                              int *__restrict__ overflow) {
+    __shared__ uint16_t nxt[NCLS][SW];  // next entry (index in the segment) per class
+    __shared__ uint32_t best[SW];       // per half and class: the largest asking key this position
+    __shared__ uint32_t tmask[2];       // per half: classes taken this position
+    const int64_t sl = blockIdx.x;
+    const int lane = threadIdx.x, half = lane & 32, hl = lane & 31, hi2 = lane >> 5;
+    const uint32_t below = (hl == 0) ? 0u : (0xFFFFFFFFu >> (32 - hl));  // lanes below this one in its half
+    const int64_t s = slane_seg[sl * SW + lane];
+    const int len = s >= 0 ? ulen[s] : 0;
+    const int64_t e0 = s >= 0 ? ustart[s] : 0;
+    // entries left per class, in registers (all accesses below are unrolled)
+    int cnt[NCLS];
+#pragma unroll
+    for (int c = 0; c < NCLS; ++c) cnt[c] = 0;
+    for (int i = 0; i < len; ++i) {
+        const int ci = (int)((keys[e0 + i] & 0xFFFFull) >> 11);  // entries are sorted by class
+#pragma unroll
+        for (int c = 0; c < NCLS; ++c) cnt[c] += (c == ci);
+    }
+    {
+        int acc = 0;
+#pragma unroll
+        for (int c = 0; c < NCLS; ++c) {
+            nxt[c][lane] = (uint16_t)acc;
+            acc += cnt[c];
         }
-        // the segment's last entry also writes the pad rows (values stay 0)
-        const int64_t uend = s + 1 < nseg ? ustart[s + 1] : nnz;
-        if (e == uend - 1)
-            for (int64_t q = p + 1; q < pstart[s + 1] - pstart[s]; ++q) lrow[pstart[s] + quad_slot(q)] = lr;
+    }
+    const int64_t prow = min((int64_t)R, n - (int64_t)slice_panel[sl] * R);
+    const int ncls = (int)min((int64_t)NCLS, prow);  // classes present in the panel
+    const uint32_t present = ncls >= 32 ? 0xFFFFFFFFu : ((1u << ncls) - 1u);
+    int rem = len;
+    int lmax = len;  // the slice's longest segment
+    for (int o = 32; o > 0; o >>= 1) lmax = max(lmax, __shfl_xor(lmax, o));
+    const int target = lmax + SLACK, cap = sched_cap(lmax);
+    uint8_t *pads = padcls + padoff[sl] + (int64_t)lane * cap;  // this lane's pad classes
+    // the class with entries left, not in `excl`, with most entries; ties to
+    // the first in the lane's rotated order starting at class lane + p
+    auto pick = [&](uint32_t excl, int p) {
+        int bk = -1, bc = -1;
+#pragma unroll
+        for (int c = 0; c < NCLS; ++c) {
+            const int k = ((excl >> c) & 1u) || cnt[c] == 0 ? -1 : (cnt[c] << 5) | (31 - ((c - lane - p) & 31));
+            if (k > bk) {
+                bk = k;
+                bc = c;
+            }
+        }
+        return bc;
+    };
+    int p = 0;
+    // positions until every lane is done, rounded up to whole steps (at most cap)
+    for (; p < cap && (__ballot(rem > 0) != 0 || (p & 3) != 0); ++p) {
+        best[lane] = 0u;
+        if (lane < 2) tmask[lane] = ~present;
+        int choice = -1;
+        int cand = rem > 0 ? pick(0u, p) : -1;
+        __syncthreads();
+        // Rounds: each asking lane offers its key (entries left, then lower
+        // lane first) to its class in its half; the largest key takes it.  A
+        // class asked for is always taken, so later rounds ask only for
+        // classes not yet taken and the keys left in best[] never mislead.
+        for (int r = 0; r < NROUND; ++r) {
+            const bool asking = cand >= 0 && choice < 0;
+            if (__ballot(asking) == 0) break;
+            const uint32_t key = ((uint32_t)rem << 6) | (uint32_t)(63 - lane);
+            if (asking) atomicMax(&best[half + cand], key);
+            __syncthreads();
+            if (asking && best[half + cand] == key) {
+                choice = cand;
+                atomicOr(&tmask[hi2], 1u << cand);
+            }
+            __syncthreads();
+            if (cand >= 0 && choice < 0) cand = pick(tmask[hi2], p);
+            __syncthreads();
+        }
+        // critical lanes: their best class, free if one is, else shared
+        const uint32_t tm = tmask[hi2];
+        if (choice < 0 && rem > 0 && rem >= target - p) {
+            const int cf = pick(tm, p);
+            choice = cf >= 0 ? cf : pick(0u, p);
+        }
+        __syncthreads();
+        if (choice >= 0) atomicOr(&tmask[hi2], 1u << choice);
+        __syncthreads();
+        // pads: the lanes of a half without an entry, in lane order, take its
+        // free classes in class order
+        const uint64_t padm = __ballot(choice < 0);
+        if (choice < 0) {
+            uint32_t f = ~tmask[hi2];
+            const int rank = __popc((uint32_t)(padm >> half) & below);
+            int padc = (lane + p) % ncls;  // no class left: any class
+            if (rank < __popc(f)) {
+                for (int k = 0; k < rank; ++k) f &= f - 1;
+                padc = __ffs((int)f) - 1;
+            }
+            pads[p] = (uint8_t)padc;
+        } else {
+            const int idx = nxt[choice][lane];
+            nxt[choice][lane] = (uint16_t)(idx + 1);
+#pragma unroll
+            for (int c = 0; c < NCLS; ++c) cnt[c] -= (c == choice);
+            --rem;
+            qpos[e0 + idx] = p;
+        }
+        __syncthreads();  // best[] and tmask[] are reset for the next position
+    }
+    if (lane == 0) slice_steps[sl] = p / 4;
+    if (rem > 0) atomicAdd(overflow, 1);  // never: critical lanes finish by the target length
+}
+
+// The stream's entries at their scheduled positions: row codes and values.
+__global__ void k_sell_fill(int64_t nnz, const uint64_t *__restrict__ keys, const int32_t *__restrict__ flag,
+                            const int64_t *__restrict__ segid, const int64_t *__restrict__ seg_slice,
+                            const int32_t *__restrict__ seg_lane, const int32_t *__restrict__ qpos,
+                            const int64_t *__restrict__ sstep, const int64_t *__restrict__ perm,
+                            const double *__restrict__ val, const float *__restrict__ hi,
+                            const float *__restrict__ lo, uint16_t *__restrict__ lrow, double *__restrict__ pval,
+                            float *__restrict__ phi, float *__restrict__ plo) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t sg = seg_of(flag, segid, e);
+        const int q = qpos[e], lane = seg_lane[sg], t = q & 3;
+        const int64_t step = sstep[seg_slice[sg]] + q / 4;
+        lrow[slot_pos(step, lane, t)] = (uint16_t)(code_row((uint32_t)(keys[e] & 0xFFFFull)) * 8u);
+        const int64_t src = perm[e];
+        if (pval) pval[val_pos(step, lane, t)] = val[src];
+        if (phi) {
+            phi[slot_pos(step, lane, t)] = hi[src];
+            plo[slot_pos(step, lane, t)] = lo[src];
+        }
+    }
+}
+
+// The pad entries (values stay 0): rows of the scheduled classes.
+__global__ void k_sell_pads(int64_t nslice, const int64_t *__restrict__ sstep, const int64_t *__restrict__ padoff,
+                            const uint8_t *__restrict__ padcls, uint16_t *__restrict__ lrow) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nslice * SW;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t sl = g / SW;
+        const int lane = (int)(g % SW);
+        const int cap = (int)((padoff[sl + 1] - padoff[sl]) / SW);
+        const uint8_t *pads = padcls + padoff[sl] + (int64_t)lane * cap;
+        const int npos = (int)(4 * (sstep[sl + 1] - sstep[sl]));
+        for (int p = 0; p < npos; ++p)
+            if (pads[p] != 0xFF) lrow[slot_pos(sstep[sl] + p / 4, lane, p & 3)] = (uint16_t)(pads[p] * 8u);
     }
 }
 
@@ -147,67 +357,12 @@ struct XT<GCI_OP_SPLIT_FULL> {
     using T = double;
 };
 
-// Reinterpret a pointer as a pointer to vector type V in the same address
-// space (LDS/generic, or global).
-template <class V, class T>
-__device__ inline const V *vcast(const T *p) {
-    return reinterpret_cast<const V *>(p);
-}
-template <class V, class T>
-__device__ inline const GAS V *vcast(const GAS T *p) {
-    return reinterpret_cast<const GAS V *>(p);
-}
+constexpr int RSRC_WORD3 = 0x00020000;       // raw (untyped) buffer, range-checked
 
-// M consecutive values of type T (M * sizeof(T) bytes, naturally aligned)
-// converted to double; one vector access per 16 bytes.
-template <int M, class T, class P>
-__device__ inline void load_row(P p, double (&o)[M]) {
-    if constexpr (std::is_same<T, float>::value) {
-        if constexpr (M == 1) {
-            o[0] = (double)p[0];
-        } else if constexpr (M == 2) {
-            const f2v w = vcast<f2v>(p)[0];
-            o[0] = (double)w.x;
-            o[1] = (double)w.y;
-        } else {
-#pragma unroll
-            for (int q = 0; q < M / 4; ++q) {
-                const f4v w = vcast<f4v>(p)[q];
-                o[4 * q + 0] = (double)w.x;
-                o[4 * q + 1] = (double)w.y;
-                o[4 * q + 2] = (double)w.z;
-                o[4 * q + 3] = (double)w.w;
-            }
-        }
-    } else {
-        if constexpr (M == 1) {
-            o[0] = p[0];
-        } else {
-#pragma unroll
-            for (int q = 0; q < M / 2; ++q) {
-                const d2v w = vcast<d2v>(p)[q];
-                o[2 * q + 0] = w.x;
-                o[2 * q + 1] = w.y;
-            }
-        }
-    }
-}
-
-// The matrix stream of one chunk as buffer resources (scalar registers):
-// every per-lane access is a 32-bit byte offset, and an offset past the
-// chunk reads zeros without touching memory (the hardware range check), so
-// a lane with nothing to load needs no clamped address.
-typedef int i2v __attribute__((ext_vector_type(2)));
-typedef int i4v __attribute__((ext_vector_type(4)));
-constexpr int RSRC_WORD3 = 0x00020000;  // raw (untyped) buffer, range-checked
-constexpr uint32_t OFF_NONE = 0x7FFFFFF0u;  // an entry offset past any chunk
-
-struct ChunkRsrc {
-    __amdgpu_buffer_rsrc_t rows, hi, lo, val;
-};
-
-// The base and size are uniform over the workgroup; readfirstlane states it,
-// so the descriptor lives in scalar registers.
+// A buffer resource (scalar registers) over `bytes` bytes at p.  Every lane
+// access is then a 32-bit byte offset, and an offset past the end reads
+// zeros without touching memory (the hardware range check).  The base and
+// size are uniform over the wavefront; readfirstlane states it.
 __device__ inline __amdgpu_buffer_rsrc_t rsrc(const void *p, uint32_t bytes) {
     const uint64_t a = (uint64_t)p;
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
@@ -216,11 +371,21 @@ __device__ inline __amdgpu_buffer_rsrc_t rsrc(const void *p, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(u, (short)0, (int)__builtin_amdgcn_readfirstlane(bytes), RSRC_WORD3);
 }
 
-// Four consecutive entries of one lane (entry offset `e` from the chunk
-// start, a multiple of 4).  settle() marks the registers as read on every
+// a wavefront-uniform 64-bit value, stated as such (scalar registers)
+__device__ inline int64_t uniform64(int64_t x) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+struct StreamRsrc {
+    __amdgpu_buffer_rsrc_t rows, hi, lo, val;
+};
+
+// One lane's quad of one step (entry offset `e` from the wavefront's first
+// entry, a multiple of 4).  settle() marks the registers as read on every
 // path (an empty asm statement with them as inputs), so the wait for their
-// load is placed before the branch that may skip them and the wait state is
-// the same on all paths.
+// load is placed before any branch and is the same on all paths.
 template <int MODE>
 struct Quad;
 
@@ -228,7 +393,7 @@ template <>
 struct Quad<GCI_OP_SPLIT_BULK> {
     u2v r;
     f4v h;
-    __device__ inline void load(const ChunkRsrc &p, uint32_t e) {
+    __device__ inline void load(const StreamRsrc &p, uint32_t e) {
         r = __builtin_bit_cast(u2v, __builtin_amdgcn_raw_buffer_load_b64(p.rows, e * 2u, 0, 0));
         h = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(p.hi, e * 4u, 0, 0));
     }
@@ -240,7 +405,7 @@ template <>
 struct Quad<GCI_OP_SPLIT_FULL> {
     u2v r;
     f4v h, l;
-    __device__ inline void load(const ChunkRsrc &p, uint32_t e) {
+    __device__ inline void load(const StreamRsrc &p, uint32_t e) {
         r = __builtin_bit_cast(u2v, __builtin_amdgcn_raw_buffer_load_b64(p.rows, e * 2u, 0, 0));
         h = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(p.hi, e * 4u, 0, 0));
         l = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(p.lo, e * 4u, 0, 0));
@@ -253,226 +418,232 @@ template <>
 struct Quad<GCI_OP_FP64> {
     u2v r;
     d2v a, b;
-    __device__ inline void load(const ChunkRsrc &p, uint32_t e) {
+    // e = step * 256 + lane * 4: the lane's halves sit at step * 256 + lane * 2
+    // and step * 256 + 128 + lane * 2 (val_pos)
+    __device__ inline void load(const StreamRsrc &p, uint32_t e) {
+        const uint32_t h0 = (e & ~(uint32_t)(SE - 1)) + ((e & (SE - 1)) >> 1);
         r = __builtin_bit_cast(u2v, __builtin_amdgcn_raw_buffer_load_b64(p.rows, e * 2u, 0, 0));
-        a = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(p.val, e * 8u, 0, 0));
-        b = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(p.val, e * 8u + 16u, 0, 0));
+        a = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(p.val, h0 * 8u, 0, 0));
+        b = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(p.val, (h0 + SE / 2) * 8u, 0, 0));
     }
     __device__ inline double value(int j) const { return j < 2 ? a[j] : b[j - 2]; }
     __device__ inline void settle() const { asm volatile("" ::"v"(r), "v"(a), "v"(b)); }
 };
 
-template <int CTRL>
-__device__ inline double dpp_double(double x) {
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-
-// Sum over the lanes l, l + S, l + 2S, ... of a 16-lane DPP row (S = M,
-// the row holding E = 16/S such lanes): row rotations by S, 2S, ... (DPP
-// row_ror).  Every lane of the class gets the total; the pairing order is
-// fixed, so the sum is the same on every run.
-template <int S, int E>
-__device__ inline double slot_sum(double x) {
-    if constexpr (E >= 2) x += dpp_double<0x120 + S>(x);
-    if constexpr (E >= 4) x += dpp_double<0x120 + 2 * S>(x);
-    if constexpr (E >= 8) x += dpp_double<0x120 + 4 * S>(x);
-    if constexpr (E >= 16) x += dpp_double<0x120 + 8 * S>(x);
-    return x;
-}
-
-// One lane's share of a quad: the four entries (rows lr, values a) of its
-// slot, each applied to the lane's vector v: y_i[v] += a x_c[v] through an
-// LDS atomic, and the column partial acc += a x_i[v].
-template <int M, int MODE>
-__device__ inline void consume(const Quad<MODE> &q, const typename XT<MODE>::T *xs, double *ys, int v, double xc,
-                               double &acc) {
-    int lr[4];
-    lr[0] = (int)(q.r.x & 0xFFFFu);
-    lr[1] = (int)(q.r.x >> 16);
-    lr[2] = (int)(q.r.y & 0xFFFFu);
-    lr[3] = (int)(q.r.y >> 16);
-    double xv[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) xv[j] = (double)xs[lr[j] * M + v];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const double a = q.value(j);
-        acc = fma(a, xv[j], acc);
-        __hip_atomic_fetch_add(ys + lr[j] * M + v, a * xc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-}
-
-// raw copy of one row of M values of type T, global -> LDS
+// M consecutive values of type T from global memory, unconverted.
 template <int M, class T>
-__device__ inline void copy_row(T *dst, const GAS T *src) {
+__device__ inline void load_raw(const GAS T *p, T (&o)[M]) {
     if constexpr (M * sizeof(T) >= 16) {
 #pragma unroll
-        for (int q = 0; q < (int)(M * sizeof(T) / 16); ++q)
-            reinterpret_cast<f4v *>(dst)[q] = vcast<f4v>(src)[q];
+        for (int q = 0; q < (int)(M * sizeof(T) / 16); ++q) {
+            const f4v w = reinterpret_cast<const GAS f4v *>(p)[q];
+            __builtin_memcpy(reinterpret_cast<char *>(o) + 16 * q, &w, 16);
+        }
     } else if constexpr (M * sizeof(T) == 8) {
-        *reinterpret_cast<f2v *>(dst) = vcast<f2v>(src)[0];
+        const f2v w = reinterpret_cast<const GAS f2v *>(p)[0];
+        __builtin_memcpy(o, &w, 8);
     } else {
-        *dst = *src;
+        o[0] = p[0];
     }
 }
 
-// LDS carve-up of one workgroup (offsets in bytes, 64-byte aligned parts):
-// xs (R x M, T) | ys (R x M, fp64) | xcol (NS x M, fp64: a segment's x_j
-// row while it runs, then its column sums) | sbeg (NS + 1, int) | scol (NS)
-struct LdsMap {
-    size_t ys, xcol, sbeg, scol, total;
+// LDS carve-up of one workgroup (bytes): ys (M x R fp64, vector-major) at 0,
+// then xs (M x R of T), and the tables of the item's (at most NS) slices:
+// cs (column partial sums, NS x 64 rows of M fp64), xj (the x_j rows,
+// NS x 64 rows of M T), cl (columns, NS x 64 ints) and ss (first steps,
+// NS + 1 ints).
+struct SellLds {
+    size_t xs, cs, xj, cl, ss, total;
 };
 template <int M, class T>
-__host__ __device__ constexpr LdsMap lds_map(int R, int NS) {
-    LdsMap m{};
-    m.ys = ((size_t)R * M * sizeof(T) + 63) & ~(size_t)63;
-    m.xcol = (m.ys + (size_t)R * M * sizeof(double) + 63) & ~(size_t)63;
-    m.sbeg = m.xcol + (size_t)NS * M * sizeof(double);
-    m.scol = m.sbeg + (((size_t)(NS + 1) * sizeof(int) + 15) & ~(size_t)15);
-    m.total = m.scol + (size_t)NS * sizeof(int);
+__host__ __device__ constexpr SellLds sell_lds(int R, int NS) {
+    SellLds m{};
+    m.xs = (size_t)R * M * sizeof(double);
+    m.cs = m.xs + (size_t)R * M * sizeof(T);
+    m.xj = m.cs + (size_t)NS * SW * M * sizeof(double);
+    m.cl = m.xj + (size_t)NS * SW * M * sizeof(T);
+    m.ss = m.cl + (size_t)NS * SW * sizeof(int);
+    m.total = m.ss + (((size_t)(NS + 1) * sizeof(int) + 15) & ~(size_t)15);
     return m;
 }
 
-// A group's walk over its segments in batches of E quads: segment i (chunk
-// local), batch start k, segment end e1.
-struct Cursor {
-    int i, k, e1;
-    template <int E, int G>
-    __device__ inline bool advance(const int *sbeg, int nsc) {  // returns true at a segment end
-        if (k + 4 * E < e1) {
-            k += 4 * E;
-            return false;
-        }
-        i += G;
-        if (i < nsc) {
-            k = sbeg[i];
-            e1 = sbeg[i + 1];
-        }
-        return true;
+// One lane's quad: its four entries (row codes off = lr * 8, values a) applied
+// to the M vectors: the column partials acc[v] += a x_i[v] (x_i from LDS) and
+// y_i[v] += a x_j[v] through LDS atomics.  All row reads are issued first.
+template <int M, int MODE, int R>
+__device__ inline void consume(const Quad<MODE> &q, const char *xs, char *ys, const double (&xj)[M],
+                               double (&acc)[M]) {
+    using T = typename XT<MODE>::T;
+    const uint32_t off[4] = {q.r.x & 0xFFFFu, q.r.x >> 16, q.r.y & 0xFFFFu, q.r.y >> 16};
+    T xv[4][M];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const char *xb = xs + (sizeof(T) == 4 ? off[j] >> 1 : off[j]);
+#pragma unroll
+        for (int v = 0; v < M; ++v) xv[j][v] = *reinterpret_cast<const T *>(xb + (size_t)v * R * sizeof(T));
     }
-};
-
-// One turn of the prefetch ring: D + 1 steps, step J consuming register set
-// J while the load D batches ahead goes into set (J + D) mod (D + 1).  The
-// sets are indexed by constants only, so they stay in registers.
-template <int J, int D, class Step, class Q>
-__device__ inline void ring_steps(Step &step, Q (&q)[D + 1], bool (&ok)[D + 1]) {
-    step(q[J], ok[J], q[(J + D) % (D + 1)], ok[(J + D) % (D + 1)]);
-    if constexpr (J < D) ring_steps<J + 1, D>(step, q, ok);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const double a = q.value(j);
+#pragma unroll
+        for (int v = 0; v < M; ++v) acc[v] = fma(a, (double)xv[j][v], acc[v]);
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+            __hip_atomic_fetch_add(reinterpret_cast<double *>(ys + off[j] + (size_t)v * R * 8), a * xj[v],
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
 }
 
-// The SpMM work-item kernel.  A group of L lanes (one DPP row or half of
-// one) holds E = L/M entry slots x M vectors: lane v + M*e loads quad e of
-// the group's batch (the M lanes of a slot load the same address) and
-// applies its four entries to vector v.  So one LDS instruction covers E
-// entries x M vectors, each entry's M lanes on one contiguous y row.
-template <int M, int MODE, int L, int BS, int D>
-__global__ __launch_bounds__(BS) void k_pcsc(const SpmmItem *__restrict__ items, const SpmmJob *__restrict__ jobs,
-                                             int R, int NS) {
+// One turn of the prefetch ring: D + 1 steps, step J consuming register set
+// J while the load D steps ahead goes into set (J + D) mod (D + 1).  The sets
+// are indexed by constants only, so they stay in registers.
+template <int J, int D, class Step, class Q>
+__device__ inline void ring_steps(Step &step, Q (&q)[D + 1]) {
+    step(q[J], q[(J + D) % (D + 1)]);
+    if constexpr (J < D) ring_steps<J + 1, D>(step, q);
+}
+
+// The SpMM work-item kernel (see the file header).
+template <int M, int MODE, int R, int BS, int D>
+__global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items, const SpmmJob *__restrict__ jobs,
+                                             int NS) {
     using T = typename XT<MODE>::T;
-    static_assert(L % M == 0 && L <= 16, "a group is E slots x M vectors inside one DPP row");
-    constexpr int E = L / M;   // entry slots per group
-    constexpr int G = BS / L;  // lane groups per workgroup
+    constexpr int W = BS / 64;  // wavefronts per workgroup
     extern __shared__ __align__(64) char smem[];
     const SpmmItem it = items[blockIdx.x];
     const SpmmJob jb = jobs[it.job];
     if (!jb.active || jb.mode != MODE) return;
     const int rows = it.rows;
-    if (rows > R) return;  // never reached: items are cut for the shape's R
-    const LdsMap lm = lds_map<M, T>(R, NS);
-    T *xs = reinterpret_cast<T *>(smem);
-    double *ys = reinterpret_cast<double *>(smem + lm.ys);
-    double *xcol = reinterpret_cast<double *>(smem + lm.xcol);
-    int *sbeg = reinterpret_cast<int *>(smem + lm.sbeg);
-    int *scl = reinterpret_cast<int *>(smem + lm.scol);
+    if (rows > R) return;  // never reached: items are cut for the shape
+    const SellLds lm = sell_lds<M, T>(R, NS);
+    double *ys = reinterpret_cast<double *>(smem);
+    T *xs = reinterpret_cast<T *>(smem + lm.xs);
+    double *cs = reinterpret_cast<double *>(smem + lm.cs);
+    T *xjt = reinterpret_cast<T *>(smem + lm.xj);
+    int *cl = reinterpret_cast<int *>(smem + lm.cl);
+    int *ss = reinterpret_cast<int *>(smem + lm.ss);
     const GAS T *gx = (const GAS T *)jb.xs;
-    const GAS int32_t *gcol = (const GAS int32_t *)jb.seg_col;
-    const GAS int64_t *gst = (const GAS int64_t *)jb.seg_start;
-    double *gy = jb.ys;
+    GAS double *gy = (GAS double *)jb.ys;
     const int t = threadIdx.x;
-    const int g = t / L, r = t % L, e = r / M, v = r % M;
+    // wavefront-uniform values are read through readfirstlane, so the walk's
+    // control flow is scalar
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
     // the panel's x rows and zeroed y rows stay for the whole item
     for (int rr = t; rr < rows; rr += BS) {
-        copy_row<M, T>(xs + rr * M, gx + (it.row0 + rr) * M);
+        T xr[M];
+        load_raw<M, T>(gx + (it.row0 + rr) * M, xr);
 #pragma unroll
-        for (int u = 0; u < M; ++u) ys[rr * M + u] = 0.0;
+        for (int v = 0; v < M; ++v) {
+            xs[v * R + rr] = xr[v];
+            ys[v * R + rr] = 0.0;
+        }
     }
-    // the item's segments in chunks of at most NS (their tables fill the LDS)
-    for (int64_t cs = it.seg0; cs < it.seg1; cs += NS) {
-        const int nsc = (int)min((int64_t)NS, it.seg1 - cs);
-        const int64_t base = gst[cs];  // first entry of the chunk (a multiple of 4)
-        for (int i = t; i <= nsc; i += BS) {
-            sbeg[i] = (int)(gst[cs + i] - base);
-            if (i < nsc) {
-                const int c = gcol[cs + i];
-                scl[i] = c;
-                double xr[M];
-                load_row<M, T>(gx + (int64_t)c * M, xr);
+    // the item's slices in chunks of at most NS, whose tables fill the LDS
+    for (int64_t c0 = it.slice0; c0 < it.slice1; c0 += NS) {
+        const int nsl = (int)min((int64_t)NS, it.slice1 - c0);
+        const int64_t st0 = uniform64(max(it.step0, jb.sstep[c0]));
+        const int nst = __builtin_amdgcn_readfirstlane((int)(min(it.step1, jb.sstep[c0 + nsl]) - st0));
+        // tables: first steps (relative to st0), columns, x_j rows, zeroed sums
+        for (int i = t; i <= nsl; i += BS) ss[i] = (int)(jb.sstep[c0 + i] - st0);
+        for (int i = t; i < nsl * SW; i += BS) {
+            const int c = jb.scol[c0 * SW + i];
+            cl[i] = c;
+            T xr[M];
+            if (c >= 0) {
+                load_raw<M, T>(gx + (int64_t)c * M, xr);
+            } else {
 #pragma unroll
-                for (int u = 0; u < M; ++u) xcol[i * M + u] = xr[u];
+                for (int v = 0; v < M; ++v) xr[v] = (T)0;
+            }
+#pragma unroll
+            for (int v = 0; v < M; ++v) {
+                xjt[i * M + v] = xr[v];
+                cs[i * M + v] = 0.0;
             }
         }
-        const uint32_t cn = (uint32_t)(gst[cs + nsc] - base);  // entries in the chunk
-        ChunkRsrc P;
-        P.rows = rsrc(jb.lrow + base, cn * 2u);
-        P.hi = rsrc(jb.hi + base, cn * 4u);
-        P.lo = rsrc(jb.lo + base, cn * 4u);
-        P.val = rsrc(jb.val + base, cn * 8u);
         __syncthreads();
 
-        // Main loop.  The load cursor runs D batches ahead of the consume
-        // cursor through a ring of D + 1 register sets, so D batches stay in
-        // flight while one is consumed.  Every step issues its load (out of
-        // range, reading zeros, once the group is done), so all paths carry
-        // the same loads and the same waits; the only global accesses in
-        // the loop are these loads.
-        Cursor cc{g, 0, 0}, lc{g, 0, 0};
-        if (g < nsc) {
-            cc.k = lc.k = sbeg[g];
-            cc.e1 = lc.e1 = sbeg[g + 1];
-        }
-        double acc = 0.0, xc = g < nsc ? xcol[g * M + v] : 0.0;
-        Quad<MODE> q[D + 1];
-        bool ok[D + 1];
-        auto issue = [&](Quad<MODE> &dst, bool &dok) {
-            dok = lc.i < nsc && lc.k + 4 * e < lc.e1;
-            dst.load(P, dok ? (uint32_t)(lc.k + 4 * e) : OFF_NONE);
-            if (lc.i < nsc) lc.advance<E, G>(sbeg, nsc);
-        };
-#pragma unroll
-        for (int d = 0; d < D; ++d) issue(q[d], ok[d]);
-        auto step = [&](Quad<MODE> &cur, bool cur_ok, Quad<MODE> &nxt, bool &nxt_ok) {
-            issue(nxt, nxt_ok);
-            cur.settle();
-            if (cur_ok) consume<M, MODE>(cur, xs, ys, v, xc, acc);
-            if (cc.i < nsc) {
-                const int ci = cc.i;
-                if (cc.advance<E, G>(sbeg, nsc)) {
-                    // segment done: its column sums replace its x_j row
-                    acc = slot_sum<M, E>(acc);
-                    if (e == 0) xcol[ci * M + v] = acc;
-                    acc = 0.0;
-                    if (cc.i < nsc) xc = xcol[cc.i * M + v];
-                }
+        // The chunk's steps split evenly over the wavefronts.  The only
+        // global accesses inside a walk are the stream loads, so the waits
+        // for them are exact.
+        const int a = (int)((int64_t)nst * w / W), b = (int)((int64_t)nst * (w + 1) / W);
+        if (a < b) {
+            // the slice holding step a: the last i with ss[i] <= a (ss[nsl] >= nst)
+            int si = 0;
+            for (int base = 1;; base += 64) {
+                const int i = base + lane;
+                const uint64_t mk = __ballot(i <= nsl && ss[min(i, nsl)] <= a);
+                si += __popcll(mk);
+                if (mk != ~0ull) break;
             }
-        };
-        while (cc.i < nsc) ring_steps<0, D>(step, q, ok);
+            const int64_t e0 = (st0 + a) * (int64_t)SE;
+            const uint32_t ne = (uint32_t)(b - a) * SE;
+            StreamRsrc P;
+            P.rows = rsrc(jb.lrow + e0, ne * 2u);
+            P.hi = rsrc(jb.hi + e0, ne * 4u);
+            P.lo = rsrc(jb.lo + e0, ne * 4u);
+            P.val = rsrc(jb.val + e0, ne * 8u);
+            double xj[M], acc[M];
+            auto enter = [&]() {  // x_j of slice si from its table
 #pragma unroll
-        for (int d = 0; d <= D; ++d) q[d].settle();
+                for (int v = 0; v < M; ++v) {
+                    xj[v] = (double)xjt[(si * SW + lane) * M + v];
+                    acc[v] = 0.0;
+                }
+            };
+            auto leave = [&]() {  // the column partials of slice si into its table
+                double *c = cs + (si * SW + lane) * M;
+#pragma unroll
+                for (int v = 0; v < M; ++v)
+                    __hip_atomic_fetch_add(c + v, acc[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            };
+            enter();
+            int snext = __builtin_amdgcn_readfirstlane(ss[si + 1]);  // first step of the next slice
+            // The loads run D steps ahead of the consumer through a ring of
+            // D + 1 register sets; a load past the wavefront's steps reads
+            // zeros, so every step issues the same loads.
+            Quad<MODE> q[D + 1];
+            int k = a;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                q[d].load(P, (uint32_t)(d * SW + lane) * 4u);
+                __builtin_amdgcn_sched_barrier(0);  // issued in ring order, as the loop issues them
+            }
+            auto step = [&](Quad<MODE> &cur, Quad<MODE> &nxt) {
+                nxt.load(P, (uint32_t)((k - a + D) * SW + lane) * 4u);
+                cur.settle();
+                if (k < b) {
+                    consume<M, MODE, R>(cur, reinterpret_cast<const char *>(xs), reinterpret_cast<char *>(ys), xj,
+                                        acc);
+                    if (k + 1 == snext && k + 1 < b) {
+                        leave();
+                        ++si;
+                        enter();
+                        snext = __builtin_amdgcn_readfirstlane(ss[si + 1]);
+                    }
+                }
+                ++k;
+            };
+            while (k < b) ring_steps<0, D>(step, q);
+#pragma unroll
+            for (int d = 0; d <= D; ++d) q[d].settle();
+            leave();
+        }
         __syncthreads();
-        // the chunk's column sums, as coalesced atomics
-        for (int idx = t; idx < nsc * M; idx += BS) {
-            const double y = xcol[idx];
-            const int u = idx % M;
-            if (y != 0.0 && u < jb.m) unsafeAtomicAdd(&gy[(int64_t)scl[idx / M] * M + u], y);
+        // the chunk's column sums: M consecutive lanes per column row
+        for (int idx = t; idx < nsl * SW * M; idx += BS) {
+            const int c = cl[idx / M];
+            const int v = idx % M;
+            const double y = cs[idx];
+            if (c >= 0 && v < jb.m && y != 0.0) __builtin_amdgcn_global_atomic_fadd_f64(gy + (int64_t)c * M + v, y);
         }
         __syncthreads();  // the next chunk overwrites the tables
     }
+    // the panel's rows, as contiguous runs of the row-major staging buffer
     for (int idx = t; idx < rows * M; idx += BS) {
-        const double y = ys[idx];
-        if (y != 0.0 && idx % M < jb.m) unsafeAtomicAdd(&gy[it.row0 * M + idx], y);
+        const int rr = idx / M, v = idx % M;
+        const double y = ys[v * R + rr];
+        if (v < jb.m && y != 0.0) __builtin_amdgcn_global_atomic_fadd_f64(gy + (it.row0 + rr) * M + v, y);
     }
 }
 
@@ -504,52 +675,59 @@ SpmmShape g_shape;         // shape all launches use
 bool g_shape_set = false;
 uint64_t g_shape_version = 1;
 
-template <int M, int MODE, int L, int BS, int D>
+int chunk_slices(const SpmmShape &sh, int M, int mode);
+
+template <int M, int MODE, int R, int BS, int D>
 void launch_depth(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh,
                   hipStream_t s) {
     using T = typename XT<MODE>::T;
-    const size_t shm = lds_map<M, T>(sh.R, sh.nseg).total;
-    GCI_REQUIRE(shm <= LDS_MAX, "spmm: panel height and segments per chunk exceed the LDS");
+    const int ns = chunk_slices(sh, M, MODE);
+    GCI_REQUIRE(ns >= 1, "spmm: panel height and launch width exceed the LDS");
+    const size_t shm = sell_lds<M, T>(R, ns).total;
     static size_t attr = 0;  // one instantiation per template tuple
     if (shm > attr) {
-        GCI_HIP(hipFuncSetAttribute((const void *)k_pcsc<M, MODE, L, BS, D>,
+        GCI_HIP(hipFuncSetAttribute((const void *)k_sell<M, MODE, R, BS, D>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         attr = shm;
     }
-    hipLaunchKernelGGL((k_pcsc<M, MODE, L, BS, D>), dim3((unsigned)nitems), dim3(BS), shm, s, d_items, d_jobs,
-                       sh.R, sh.nseg);
+    hipLaunchKernelGGL((k_sell<M, MODE, R, BS, D>), dim3((unsigned)nitems), dim3(BS), shm, s, d_items, d_jobs, ns);
 }
 
-template <int M, int MODE, int L, int BS>
+template <int M, int MODE, int R, int BS>
 void launch_shape(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh,
                   hipStream_t s) {
-    switch (sh.depth) {
-        case 3: launch_depth<M, MODE, L, BS, 3>(d_items, nitems, d_jobs, sh, s); break;
-        case 4: launch_depth<M, MODE, L, BS, 4>(d_items, nitems, d_jobs, sh, s); break;
-        default: launch_depth<M, MODE, L, BS, 6>(d_items, nitems, d_jobs, sh, s); break;
+    if (sh.depth == 2) launch_depth<M, MODE, R, BS, 2>(d_items, nitems, d_jobs, sh, s);
+    else launch_depth<M, MODE, R, BS, 4>(d_items, nitems, d_jobs, sh, s);
+}
+
+// 1024-thread workgroups only up to 4 columns (8 would not fit the 128
+// registers per lane that 16 wavefronts leave); wider launches use 512
+template <int M, int MODE, int R>
+void launch_bs(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh, hipStream_t s) {
+    if (sh.block == 256) {
+        launch_shape<M, MODE, R, 256>(d_items, nitems, d_jobs, sh, s);
+    } else if constexpr (M <= 4) {
+        if (sh.block == 1024) launch_shape<M, MODE, R, 1024>(d_items, nitems, d_jobs, sh, s);
+        else launch_shape<M, MODE, R, 512>(d_items, nitems, d_jobs, sh, s);
+    } else {
+        launch_shape<M, MODE, R, 512>(d_items, nitems, d_jobs, sh, s);
     }
 }
 
-template <int M, int MODE, int L>
-void launch_bs(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh, hipStream_t s) {
-    if (sh.block == 256) launch_shape<M, MODE, L, 256>(d_items, nitems, d_jobs, sh, s);
-    else launch_shape<M, MODE, L, 512>(d_items, nitems, d_jobs, sh, s);
-}
-
 template <int M, int MODE>
-void launch_l(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh, hipStream_t s) {
-    if (sh.L == 8) launch_bs<M, MODE, 8>(d_items, nitems, d_jobs, sh, s);
-    else launch_bs<M, MODE, 16>(d_items, nitems, d_jobs, sh, s);
+void launch_r(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh, hipStream_t s) {
+    if (sh.R == 2048) launch_bs<M, MODE, 2048>(d_items, nitems, d_jobs, sh, s);
+    else launch_bs<M, MODE, 1024>(d_items, nitems, d_jobs, sh, s);
 }
 
 template <int MODE>
 void launch_mode(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, int M, const SpmmShape &sh,
                  hipStream_t s) {
     switch (M) {
-        case 1: launch_l<1, MODE>(d_items, nitems, d_jobs, sh, s); break;
-        case 2: launch_l<2, MODE>(d_items, nitems, d_jobs, sh, s); break;
-        case 4: launch_l<4, MODE>(d_items, nitems, d_jobs, sh, s); break;
-        default: launch_l<8, MODE>(d_items, nitems, d_jobs, sh, s); break;
+        case 1: launch_r<1, MODE>(d_items, nitems, d_jobs, sh, s); break;
+        case 2: launch_r<2, MODE>(d_items, nitems, d_jobs, sh, s); break;
+        case 4: launch_r<4, MODE>(d_items, nitems, d_jobs, sh, s); break;
+        default: launch_r<8, MODE>(d_items, nitems, d_jobs, sh, s); break;
     }
 }
 
@@ -558,23 +736,69 @@ int env_int(const char *name, int dflt) {
     return v && *v ? std::atoi(v) : dflt;
 }
 
+size_t lds_bytes(int R, int NS, int M, int mode) {
+    if (mode == GCI_OP_SPLIT_BULK) {
+        switch (M) {
+            case 1: return sell_lds<1, float>(R, NS).total;
+            case 2: return sell_lds<2, float>(R, NS).total;
+            case 4: return sell_lds<4, float>(R, NS).total;
+            default: return sell_lds<8, float>(R, NS).total;
+        }
+    }
+    switch (M) {
+        case 1: return sell_lds<1, double>(R, NS).total;
+        case 2: return sell_lds<2, double>(R, NS).total;
+        case 4: return sell_lds<4, double>(R, NS).total;
+        default: return sell_lds<8, double>(R, NS).total;
+    }
+}
+
+template <class I, class O>
+void scan_exclusive(const I *in, O *out, int64_t n, hipStream_t s) {
+    size_t tb = 0;
+    GCI_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, s));
+    SBuf<char> tmp(tb, s);
+    GCI_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, in, out, n, s));
+}
+
+template <class K, class V>
+void sort_pairs(const K *k0, K *k1, const V *v0, V *v1, int64_t n, hipStream_t s) {
+    size_t tb = 0;
+    GCI_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, n, 0, 8 * (int)sizeof(K), s));
+    SBuf<char> tmp(tb, s);
+    GCI_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0, k1, v0, v1, n, 0, 8 * (int)sizeof(K), s));
+}
+
+unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); }
+
+// Slices per chunk of a launch: the shape's count, or fewer if the LDS of
+// this width and mode holds fewer (0: not even one).
+int chunk_slices(const SpmmShape &sh, int M, int mode) {
+    int ns = sh.nslice;
+    while (ns > 0 && lds_bytes(sh.R, ns, M, mode) > LDS_MAX) ns = ns > 64 ? ns - ns / 8 : ns - 1;
+    return ns;
+}
+
+// A launch width runs only if its chunks hold at least this many slices
+// (narrower column blocks are cheaper than tiny chunks).
+constexpr int NS_MIN = 8;
+
 }  // namespace
 
 bool valid_shape(const SpmmShape &sh) {
-    // the instantiated launch shapes: L in {8, 16}, 256 or 512 threads, depth 3, 4 or 6
-    return sh.R >= 64 && sh.R <= 4096 && (sh.R & (sh.R - 1)) == 0 && (sh.L == 8 || sh.L == 16) &&
-           (sh.block == 256 || sh.block == 512) && sh.target >= 4096 && sh.nseg >= 64 && sh.nseg <= 4096 &&
-           (sh.depth == 3 || sh.depth == 4 || sh.depth == 6) && lds_map<1, double>(sh.R, sh.nseg).total <= LDS_MAX;
+    // the instantiated launch shapes: R 1024 or 2048, 256/512/1024 threads, depth 2 or 4
+    return (sh.R == 1024 || sh.R == 2048) && (sh.block == 256 || sh.block == 512 || sh.block == 1024) &&
+           sh.target >= 4096 && sh.nslice >= 1 && sh.nslice <= 1024 && (sh.depth == 2 || sh.depth == 4) &&
+           chunk_slices(sh, 1, GCI_OP_FP64) >= std::min(NS_MIN, sh.nslice);
 }
 
 SpmmShape spmm_shape() {
     if (!g_shape_set) {
         SpmmShape sh;
         sh.R = env_int("GCI_SPMM_R", sh.R);
-        sh.L = env_int("GCI_SPMM_L", sh.L);
         sh.block = env_int("GCI_SPMM_BLOCK", sh.block);
         sh.target = env_int("GCI_SPMM_TARGET", (int)sh.target);
-        sh.nseg = env_int("GCI_SPMM_NSEG", sh.nseg);
+        sh.nslice = env_int("GCI_SPMM_NSLICE", sh.nslice);
         sh.depth = env_int("GCI_SPMM_DEPTH", sh.depth);
         GCI_REQUIRE(valid_shape(sh), "spmm: invalid launch shape");
         g_shape = sh;
@@ -595,7 +819,7 @@ uint64_t spmm_shape_version() { return g_shape_version; }
 int spmm_max_width(int mode) {
     const SpmmShape sh = spmm_shape();
     for (int w = 8; w > 1; w /= 2)
-        if (spmm_lds_bytes(sh, w, mode) <= LDS_MAX) return w;
+        if (chunk_slices(sh, w, mode) >= std::min(NS_MIN, sh.nslice)) return w;
     return 1;
 }
 
@@ -614,15 +838,25 @@ void spmm_items(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, 
 void build_items(HMat &h, int64_t target) {
     Pcsc &pc = h.pc;
     pc.items.clear();
-    if (pc.nseg == 0) return;
     const int R = pc.R;
-    for (int64_t s0 = 0, s = 1; s <= pc.nseg; ++s) {
-        if (s == pc.nseg || pc.seg_panel_h[s] != pc.seg_panel_h[s0] ||
-            pc.seg_start_h[s] - pc.seg_start_h[s0] >= target) {
-            const int64_t row0 = (int64_t)pc.seg_panel_h[s0] * R;
-            pc.items.push_back(SpmmItem{s0, s, row0, (int32_t)std::min<int64_t>(R, h.n - row0), 0});
-            s0 = s;
+    const int64_t ts = std::max<int64_t>(1, target / SE);  // steps per item
+    const std::vector<int64_t> &T = pc.sstep_h;
+    for (int64_t sa = 0; sa < pc.nslice;) {
+        const int32_t p = pc.spanel_h[sa];
+        int64_t sb = sa + 1;
+        while (sb < pc.nslice && pc.spanel_h[sb] == p) ++sb;
+        const int64_t row0 = (int64_t)p * R;
+        const int32_t rows = (int32_t)std::min<int64_t>(R, h.n - row0);
+        int64_t cur = sa;  // the slice holding step st
+        for (int64_t st = T[sa]; st < T[sb];) {
+            const int64_t en = std::min(st + ts, T[sb]);
+            int64_t s1 = cur;
+            while (s1 < sb && T[s1] < en) ++s1;
+            pc.items.push_back(SpmmItem{st, en, cur, s1, row0, rows, 0});
+            st = en;
+            while (cur + 1 < sb && T[cur + 1] <= st) ++cur;
         }
+        sa = sb;
     }
     pc.target = target;
 }
@@ -646,23 +880,22 @@ void ensure_items(HMat &h) {
 }
 
 size_t spmm_lds_bytes(const SpmmShape &sh, int M, int mode) {
-    if (mode == GCI_OP_SPLIT_BULK) {
-        switch (M) {
-            case 1: return lds_map<1, float>(sh.R, sh.nseg).total;
-            case 2: return lds_map<2, float>(sh.R, sh.nseg).total;
-            case 4: return lds_map<4, float>(sh.R, sh.nseg).total;
-            default: return lds_map<8, float>(sh.R, sh.nseg).total;
-        }
-    }
-    switch (M) {
-        case 1: return lds_map<1, double>(sh.R, sh.nseg).total;
-        case 2: return lds_map<2, double>(sh.R, sh.nseg).total;
-        case 4: return lds_map<4, double>(sh.R, sh.nseg).total;
-        default: return lds_map<8, double>(sh.R, sh.nseg).total;
-    }
+    return lds_bytes(sh.R, std::max(1, chunk_slices(sh, M, mode)), M, mode);
 }
 
 size_t spmm_lds_max() { return LDS_MAX; }
+
+void retain_pool() {
+    static bool done[64] = {};
+    int dev = 0;
+    GCI_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64 || done[dev]) return;
+    hipMemPool_t pool;
+    GCI_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+    uint64_t keep = 4ull << 30;
+    GCI_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+    done[dev] = true;
+}
 
 void ensure_pcsc(HMat &h) {
     if (h.pc.R) return;
@@ -672,32 +905,28 @@ void ensure_pcsc(HMat &h) {
     const int64_t nnz = h.nnz, n = h.n;
     hipStream_t s = h.stream;
     pc.nnz = nnz;
-    pc.nseg = 0;
-    pc.npad = 0;
+    pc.nseg = pc.nslice = pc.nstep = 0;
+    pc.sstep_h.assign(1, 0);
+    pc.spanel_h.clear();
     if (nnz > 0) {
-        DBuf<uint64_t> k0(nnz), k1(nnz);
-        DBuf<int64_t> i0(nnz), i1(nnz);
-        const int64_t blocks = (n * 64 + 255) / 256;
-        hipLaunchKernelGGL(k_keys, dim3((unsigned)blocks), dim3(256), 0, s, n, h.rp.p, h.col.p, R, k0.p, i0.p);
+        const unsigned eb = grid_for(nnz);
+        // entries sorted by (panel, column, row class, row)
+        SBuf<uint64_t> k1(nnz, s);
+        SBuf<int64_t> i1(nnz, s);
+        {
+            SBuf<uint64_t> k0(nnz, s);
+            SBuf<int64_t> i0(nnz, s);
+            const int64_t blocks = (n * 64 + 255) / 256;
+            hipLaunchKernelGGL(k_keys, dim3((unsigned)blocks), dim3(256), 0, s, n, h.rp.p, h.col.p, R, k0.p, i0.p);
+            GCI_HIP(hipGetLastError());
+            sort_pairs(k0.p, k1.p, i0.p, i1.p, nnz, s);
+        }
+        // segments
+        SBuf<int32_t> flag(nnz, s);
+        SBuf<int64_t> segid(nnz, s);
+        hipLaunchKernelGGL(k_seg_flags, dim3(eb), dim3(256), 0, s, nnz, k1.p, flag.p);
         GCI_HIP(hipGetLastError());
-        size_t tb = 0;
-        GCI_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0.p, k1.p, i0.p, i1.p, nnz, 0, 64, s));
-        {
-            DBuf<char> tmp(tb);
-            GCI_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0.p, k1.p, i0.p, i1.p, nnz, 0, 64, s));
-        }
-        k0.release();
-        i0.release();
-        DBuf<int32_t> flag(nnz);
-        DBuf<int64_t> segid(nnz);
-        const int64_t eb = std::min<int64_t>((nnz + 255) / 256, 65536);
-        hipLaunchKernelGGL(k_seg_flags, dim3((unsigned)eb), dim3(256), 0, s, nnz, k1.p, flag.p);
-        tb = 0;
-        GCI_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flag.p, segid.p, nnz, s));
-        {
-            DBuf<char> tmp(tb);
-            GCI_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, flag.p, segid.p, nnz, s));
-        }
+        scan_exclusive(flag.p, segid.p, nnz, s);
         int64_t last = 0;
         int32_t lastf = 0;
         GCI_HIP(hipMemcpyAsync(&last, segid.p + nnz - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -705,46 +934,92 @@ void ensure_pcsc(HMat &h) {
         GCI_HIP(hipStreamSynchronize(s));
         const int64_t nseg = last + lastf;
         pc.nseg = nseg;
-        pc.seg_col.alloc(nseg);
-        DBuf<int64_t> ustart(nseg);
-        DBuf<int32_t> spanel(nseg);
-        hipLaunchKernelGGL(k_seg_meta, dim3((unsigned)eb), dim3(256), 0, s, nnz, k1.p, flag.p, segid.p,
-                           pc.seg_col.p, ustart.p, spanel.p);
+        SBuf<int32_t> seg_col(nseg, s), spanel(nseg, s), ulen(nseg, s);
+        SBuf<int64_t> ustart(nseg, s);
+        hipLaunchKernelGGL(k_seg_meta, dim3(eb), dim3(256), 0, s, nnz, k1.p, flag.p, segid.p, seg_col.p, ustart.p,
+                           spanel.p);
         GCI_HIP(hipGetLastError());
-        DBuf<int64_t> plen(nseg + 1);
-        const int64_t sb = std::min<int64_t>((nseg + 256) / 256, 65536);
-        hipLaunchKernelGGL(k_seg_plen, dim3((unsigned)sb), dim3(256), 0, s, nseg, nnz, ustart.p, plen.p);
-        GCI_HIP(hipGetLastError());
-        pc.seg_start.alloc(nseg + 1);
-        tb = 0;
-        GCI_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, plen.p, pc.seg_start.p, nseg + 1, s));
+        // segments of a panel, longest first
+        const int64_t npanel = (n + R - 1) / R;
+        const unsigned sg = grid_for(nseg);
+        SBuf<int64_t> order(nseg, s);
         {
-            DBuf<char> tmp(tb);
-            GCI_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, plen.p, pc.seg_start.p, nseg + 1, s));
+            SBuf<uint64_t> okey(nseg, s), okey1(nseg, s);
+            SBuf<int64_t> oval(nseg, s);
+            hipLaunchKernelGGL(k_seg_order, dim3(sg), dim3(256), 0, s, nseg, nnz, ustart.p, spanel.p, ulen.p, okey.p,
+                               oval.p);
+            GCI_HIP(hipGetLastError());
+            sort_pairs(okey.p, okey1.p, oval.p, order.p, nseg, s);
         }
-        pc.seg_start_h.resize(nseg + 1);
-        pc.seg_panel_h.resize(nseg);
-        GCI_HIP(hipMemcpyAsync(pc.seg_start_h.data(), pc.seg_start.p, (nseg + 1) * sizeof(int64_t),
-                               hipMemcpyDeviceToHost, s));
-        GCI_HIP(hipMemcpyAsync(pc.seg_panel_h.data(), spanel.p, nseg * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        // slices
+        SBuf<int64_t> pslices(npanel + 1, s), pslice0(npanel + 1, s), pfirst(npanel + 1, s);
+        hipLaunchKernelGGL(k_panel_slices, dim3(grid_for(npanel + 1)), dim3(256), 0, s, npanel, nseg, spanel.p,
+                           pfirst.p, pslices.p);
+        GCI_HIP(hipGetLastError());
+        scan_exclusive(pslices.p, pslice0.p, npanel + 1, s);
+        int64_t nslice = 0;
+        GCI_HIP(hipMemcpyAsync(&nslice, pslice0.p + npanel, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         GCI_HIP(hipStreamSynchronize(s));
-        const int64_t npad = pc.seg_start_h[nseg];
-        pc.npad = npad;
-        pc.lrow.alloc(npad);
-        if (h.val.p) pc.val.alloc(npad);
+        pc.nslice = nslice;
+        pc.scol.alloc((size_t)nslice * SW);
+        SBuf<int32_t> slice_panel(nslice, s), seg_lane(nseg, s);
+        SBuf<int64_t> slane_seg((size_t)nslice * SW, s), slice_steps(nslice + 1, s), seg_slice(nseg, s),
+            pcap(nslice + 1, s), padoff(nslice + 1, s);
+        GCI_HIP(hipMemsetAsync(pc.scol.p, 0xFF, (size_t)nslice * SW * sizeof(int32_t), s));    // -1: empty lane
+        GCI_HIP(hipMemsetAsync(slane_seg.p, 0xFF, (size_t)nslice * SW * sizeof(int64_t), s));  // -1: empty lane
+        GCI_HIP(hipMemsetAsync(pcap.p, 0, (nslice + 1) * sizeof(int64_t), s));
+        hipLaunchKernelGGL(k_seg_place, dim3(sg), dim3(256), 0, s, nseg, order.p, spanel.p, seg_col.p, ulen.p,
+                           pfirst.p, pslice0.p, pc.scol.p, slane_seg.p, slice_panel.p, seg_slice.p, seg_lane.p,
+                           pcap.p);
+        GCI_HIP(hipGetLastError());
+        // the bank schedule: each entry's position in its slice, the pads'
+        // classes and the slices' step counts
+        GCI_REQUIRE(nslice < 0x7FFFFFFF, "spmm: too many slices");
+        scan_exclusive(pcap.p, padoff.p, nslice + 1, s);
+        int64_t npadbuf = 0;
+        GCI_HIP(hipMemcpyAsync(&npadbuf, padoff.p + nslice, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        GCI_HIP(hipStreamSynchronize(s));
+        SBuf<uint8_t> padcls(npadbuf, s);
+        SBuf<int32_t> qpos(nnz, s);
+        SBuf<int> overflow(1, s);
+        GCI_HIP(hipMemsetAsync(padcls.p, 0xFF, npadbuf, s));  // 0xFF: an entry's position
+        GCI_HIP(hipMemsetAsync(overflow.p, 0, sizeof(int), s));
+        hipLaunchKernelGGL(k_sched, dim3((unsigned)nslice), dim3(SW), 0, s, slane_seg.p, ustart.p, ulen.p, k1.p,
+                           slice_panel.p, R, n, padoff.p, slice_steps.p, qpos.p, padcls.p, overflow.p);
+        GCI_HIP(hipGetLastError());
+        pc.sstep.alloc(nslice + 1);
+        scan_exclusive(slice_steps.p, pc.sstep.p, nslice + 1, s);
+        pc.sstep_h.resize(nslice + 1);
+        pc.spanel_h.resize(nslice);
+        GCI_HIP(hipMemcpyAsync(pc.sstep_h.data(), pc.sstep.p, (nslice + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
+                               s));
+        GCI_HIP(hipMemcpyAsync(pc.spanel_h.data(), slice_panel.p, nslice * sizeof(int32_t), hipMemcpyDeviceToHost,
+                               s));
+        int ovf = 0;
+        GCI_HIP(hipMemcpyAsync(&ovf, overflow.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        GCI_HIP(hipStreamSynchronize(s));
+        GCI_REQUIRE(ovf == 0, "spmm: layout schedule overflow");
+        const int64_t nstep = pc.sstep_h[nslice];
+        pc.nstep = nstep;
+        const size_t nst = (size_t)nstep * SE;
+        pc.lrow.alloc(nst);
+        if (h.val.p) pc.val.alloc(nst);
         if (h.precision == GCI_SPLIT32) {
-            pc.hi.alloc(npad);
-            pc.lo.alloc(npad);
+            pc.hi.alloc(nst);
+            pc.lo.alloc(nst);
         }
         // pad entries keep value 0
-        if (pc.val.p) GCI_HIP(hipMemsetAsync(pc.val.p, 0, npad * sizeof(double), s));
+        if (pc.val.p) GCI_HIP(hipMemsetAsync(pc.val.p, 0, nst * sizeof(double), s));
         if (pc.hi.p) {
-            GCI_HIP(hipMemsetAsync(pc.hi.p, 0, npad * sizeof(float), s));
-            GCI_HIP(hipMemsetAsync(pc.lo.p, 0, npad * sizeof(float), s));
+            GCI_HIP(hipMemsetAsync(pc.hi.p, 0, nst * sizeof(float), s));
+            GCI_HIP(hipMemsetAsync(pc.lo.p, 0, nst * sizeof(float), s));
         }
-        hipLaunchKernelGGL(k_seg_fill, dim3((unsigned)eb), dim3(256), 0, s, nnz, nseg, k1.p, flag.p, segid.p,
-                           ustart.p, pc.seg_start.p, i1.p, h.val.p, h.hi.p, h.lo.p, pc.lrow.p, pc.val.p, pc.hi.p,
-                           pc.lo.p);
+        hipLaunchKernelGGL(k_sell_fill, dim3(eb), dim3(256), 0, s, nnz, k1.p, flag.p, segid.p, seg_slice.p,
+                           seg_lane.p, qpos.p, pc.sstep.p, i1.p, h.val.p, h.hi.p, h.lo.p, pc.lrow.p, pc.val.p,
+                           pc.hi.p, pc.lo.p);
+        GCI_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_sell_pads, dim3(grid_for(nslice * SW)), dim3(256), 0, s, nslice, pc.sstep.p, padoff.p,
+                           padcls.p, pc.lrow.p);
         GCI_HIP(hipGetLastError());
         GCI_HIP(hipStreamSynchronize(s));
     }
@@ -776,8 +1051,8 @@ void spmm(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode
         std::vector<SpmmJob> jt(3 * 8);
         for (int md = 0; md < 3; ++md)
             for (int mc = 1; mc <= 8; ++mc)
-                jt[md * 8 + mc - 1] = SpmmJob{pc.seg_col.p, pc.seg_start.p, pc.lrow.p, pc.val.p, pc.hi.p, pc.lo.p,
-                                              pc.stage_x.p, pc.stage_y.p, mc, md, 1, 0};
+                jt[md * 8 + mc - 1] = SpmmJob{pc.scol.p, pc.sstep.p, pc.lrow.p, pc.val.p, pc.hi.p, pc.lo.p,
+                                              pc.stage_x.p, pc.stage_y.p, h.n, mc, md, 1, 0};
         pc.d_jobs.alloc(jt.size());
         GCI_HIP(hipMemcpy(pc.d_jobs.p, jt.data(), jt.size() * sizeof(SpmmJob), hipMemcpyHostToDevice));
     }
@@ -792,20 +1067,20 @@ void spmm(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode
         const int M = spmm_width(mc);
         const double *xc = x + (int64_t)c0 * ldx;
         double *yc = y + (int64_t)c0 * ldy;
-        const int64_t pb = std::min<int64_t>((h.n * M + 255) / 256, 65536);
+        const unsigned pb = grid_for(h.n * M);
         if (mode == GCI_OP_SPLIT_BULK)
-            hipLaunchKernelGGL(k_pack<float>, dim3((unsigned)pb), dim3(256), 0, s, h.n, mc, M, xc, (int64_t)ldx,
+            hipLaunchKernelGGL(k_pack<float>, dim3(pb), dim3(256), 0, s, h.n, mc, M, xc, (int64_t)ldx,
                                (float *)pc.stage_x.p, pc.stage_y.p);
         else
-            hipLaunchKernelGGL(k_pack<double>, dim3((unsigned)pb), dim3(256), 0, s, h.n, mc, M, xc, (int64_t)ldx,
+            hipLaunchKernelGGL(k_pack<double>, dim3(pb), dim3(256), 0, s, h.n, mc, M, xc, (int64_t)ldx,
                                (double *)pc.stage_x.p, pc.stage_y.p);
         GCI_HIP(hipGetLastError());
         // items carry job 0; the table offset selects the descriptor
         spmm_items(pc.d_items.p, (int64_t)pc.items.size(), pc.d_jobs.p + mode * 8 + mc - 1, M, mode, s);
         GCI_HIP(hipGetLastError());
-        const int64_t ub = std::min<int64_t>((h.n * mc + 255) / 256, 65536);
-        hipLaunchKernelGGL(k_unpack, dim3((unsigned)ub), dim3(256), 0, s, h.n, mc, M, (const double *)pc.stage_y.p,
-                           (const double *)h.diag.p, xc, (int64_t)ldx, yc, (int64_t)ldy);
+        hipLaunchKernelGGL(k_unpack, dim3(grid_for(h.n * mc)), dim3(256), 0, s, h.n, mc, M,
+                           (const double *)pc.stage_y.p, (const double *)h.diag.p, xc, (int64_t)ldx, yc,
+                           (int64_t)ldy);
         GCI_HIP(hipGetLastError());
     }
 }

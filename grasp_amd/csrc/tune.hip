@@ -1,12 +1,10 @@
 // tune.hip -- launch shape of the symmetric SpMM and its autotuner.
 //
 // No counterpart in the reference: SPICMV2 (lib9290/spicmv2.f90) is a plain
-// loop.  Here the SpMM's shape -- panel height R, lanes per column segment
-// L, workgroup size, segments per LDS chunk, prefetch depth and entries per
-// work item -- decides LDS
-// occupancy, the number of waves that hide HBM latency and the share of
-// global atomics, and the best choice depends on the matrices' sizes and
-// fill.  The tuner times the operator on the caller's matrices and walks
+// loop.  Here the SpMM's shape -- panel height R, workgroup size, slices
+// and entries per work item, prefetch depth -- decides LDS occupancy, the
+// number of wavefronts that hide HBM latency and the share of global atomics,
+// and the best choice depends on the matrices' sizes and fill.  The tuner times the operator on the caller's matrices and walks
 // each parameter in turn (coordinate search), keeping the fastest shape.
 #include <algorithm>
 #include <vector>
@@ -83,8 +81,8 @@ struct TuneSet {
             HMat &h = *mats[i];
             ensure_items(h);
             const Pcsc &pc = h.pc;
-            jobs.push_back(SpmmJob{pc.seg_col.p, pc.seg_start.p, pc.lrow.p, pc.val.p, pc.hi.p, pc.lo.p, xs[i].p,
-                                   ys[i].p, std::min(W, m), mode, 1, 0});
+            jobs.push_back(SpmmJob{pc.scol.p, pc.sstep.p, pc.lrow.p, pc.val.p, pc.hi.p, pc.lo.p, xs[i].p,
+                                   ys[i].p, h.n, std::min(W, m), mode, 1, 0});
             for (SpmmItem it : pc.items) {
                 it.job = (int32_t)i;
                 items.push_back(it);
@@ -126,15 +124,14 @@ struct TuneSet {
 bool fits(const SpmmShape &sh, int, int) { return valid_shape(sh); }
 
 gci_spmm_shape to_c(const SpmmShape &sh) {
-    return gci_spmm_shape{sh.R, sh.L, sh.block, sh.nseg, sh.depth, sh.target};
+    return gci_spmm_shape{sh.R, sh.block, sh.nslice, sh.depth, sh.target};
 }
 
 SpmmShape from_c(const gci_spmm_shape &c) {
     SpmmShape sh;
     sh.R = c.R;
-    sh.L = c.L;
     sh.block = c.block;
-    sh.nseg = c.nseg;
+    sh.nslice = c.nslice;
     sh.depth = c.depth;
     sh.target = c.target;
     return sh;
@@ -162,6 +159,16 @@ int gci_spmm_set_shape(const gci_spmm_shape *shape) {
     });
 }
 
+int gci_spmm_layout(gci_hmat *h, int64_t *nseg, int64_t *nslice, int64_t *stored) {
+    return guarded([&] {
+        GCI_REQUIRE(h && nseg && nslice && stored, "gci_spmm_layout: null argument");
+        ensure_items(*h);
+        *nseg = h->pc.nseg;
+        *nslice = h->pc.nslice;
+        *stored = h->pc.nstep * 256;
+    });
+}
+
 int gci_spmm_time(gci_hmat *const *mats, int nmats, int m, int mode, int reps, double *ms) {
     return guarded([&] {
         GCI_REQUIRE(ms && reps >= 1, "gci_spmm_time: bad arguments");
@@ -180,9 +187,8 @@ int gci_spmm_autotune(gci_hmat *const *mats, int nmats, int m, int mode, int rep
         GCI_REQUIRE(fits(cur, m, mode), "gci_spmm_autotune: no shape fits this width");
         set_spmm_shape(cur);
         double tbest = ts.time(reps);
-        const std::vector<int> Ls{8, 16}, blocks{256, 512}, nsegs{256, 512, 1024, 2048},
-            Rs{256, 512, 1024, 2048, 4096}, depths{3, 4, 6};
-        const std::vector<int64_t> targets{65536, 131072, 262144, 524288};
+        const std::vector<int> blocks{256, 512, 1024}, nslices{16, 32, 64, 128}, Rs{1024, 2048}, depths{2, 4};
+        const std::vector<int64_t> targets{32768, 65536, 131072, 262144};
         auto try_shape = [&](const SpmmShape &cand) {
             if (!fits(cand, m, mode)) return;
             set_spmm_shape(cand);
@@ -194,9 +200,8 @@ int gci_spmm_autotune(gci_hmat *const *mats, int nmats, int m, int mode, int rep
         };
         for (int pass = 0; pass < 2; ++pass) {
             for (int v : depths) { SpmmShape c = cur; c.depth = v; try_shape(c); }
-            for (int v : Ls) { SpmmShape c = cur; c.L = v; try_shape(c); }
             for (int v : blocks) { SpmmShape c = cur; c.block = v; try_shape(c); }
-            for (int v : nsegs) { SpmmShape c = cur; c.nseg = v; try_shape(c); }
+            for (int v : nslices) { SpmmShape c = cur; c.nslice = v; try_shape(c); }
             for (int64_t v : targets) { SpmmShape c = cur; c.target = v; try_shape(c); }
             for (int v : Rs) { SpmmShape c = cur; c.R = v; try_shape(c); }
         }

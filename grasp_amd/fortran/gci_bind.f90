@@ -48,7 +48,7 @@ module gci_bind
    end type gci_dvd_stats
 
    type, bind(C) :: gci_spmm_shape
-      integer(c_int) :: r, l, block, nseg, depth
+      integer(c_int) :: r, block, nslice, depth
       integer(c_int64_t) :: target
    end type gci_spmm_shape
 
@@ -262,6 +262,13 @@ module gci_bind
          type(gci_spmm_shape), intent(in) :: shape
          integer(c_int) :: st
       end function gci_spmm_set_shape
+
+      function gci_spmm_layout(h, nseg, nslice, stored) bind(C, name="gci_spmm_layout") result(st)
+         import :: c_int, c_int64_t, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int64_t) :: nseg, nslice, stored
+         integer(c_int) :: st
+      end function gci_spmm_layout
 
       function gci_spmm_time(mats, nmats, m, mode, reps, ms) bind(C, name="gci_spmm_time") result(st)
          import :: c_int, c_double, c_ptr

@@ -244,13 +244,13 @@ def batch_solve(blocks, nev, tol, eav=None):
     return evals, evecs, stats
 
 
-_SHAPE_KEYS = ("R", "L", "block", "nseg", "depth", "target")
+_SHAPE_KEYS = ("R", "block", "nslice", "depth", "target")
 
 
 def spmm_shape() -> dict:
-    """The SpMM launch shape in force: panel height R, lanes per column
-    segment L, workgroup size, segments per LDS chunk (nseg), batches in
-    flight per lane (depth) and entries per work item (target)."""
+    """The SpMM launch shape in force: panel height R, workgroup size
+    (block), slices per work item (nslice), steps in flight per wavefront
+    (depth) and entries per work item (target)."""
     s = L.GciSpmmShape()
     check(L.lib().gci_spmm_get_shape(C.byref(s)))
     return {k: int(getattr(s, k)) for k in _SHAPE_KEYS}
@@ -266,6 +266,16 @@ def set_spmm_shape(**kw) -> dict:
     s = L.GciSpmmShape(**cur)
     check(L.lib().gci_spmm_set_shape(C.byref(s)))
     return cur
+
+
+def spmm_layout(block) -> dict:
+    """The block's panel-slice layout for the shape in force: column
+    segments, slices, stored entries (padding included) and their ratio to
+    the off-diagonal entries."""
+    ns, nsl, st = C.c_int64(), C.c_int64(), C.c_int64()
+    check(L.lib().gci_spmm_layout(block.handle, C.byref(ns), C.byref(nsl), C.byref(st)))
+    return {"segments": ns.value, "slices": nsl.value, "stored": st.value,
+            "overhead": st.value / max(1, block.nnz_offdiag)}
 
 
 def spmm_time(blocks, m: int, mode: int = L.GCI_OP_SPLIT_BULK, reps: int = 5) -> float:

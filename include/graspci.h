@@ -198,8 +198,9 @@ int gci_batch_solve(int nblocks, gci_hmat **blocks, const int *nev, const double
 
 int gci_server_create(size_t hbm_budget_bytes, int max_active, gci_server **out);
 int gci_server_destroy(gci_server *s);
-/* Submit one CI solve; the server takes ownership of h.  Returns a request
- * id in *req. */
+/* Submit one CI solve; the server takes ownership of h.  The matrix's SpMM
+ * layout is built in this call (on the caller's thread), so the latency of
+ * a request runs from here.  Returns a request id in *req. */
 int gci_server_submit(gci_server *s, gci_hmat *h, int nev, double tol, double eav, int64_t *req);
 /* Non-blocking poll: GCI_OK with results when done, GCI_ERR_BUSY otherwise. */
 int gci_server_poll(gci_server *s, int64_t req, double *eval, double *evec, double *latency_s);
@@ -219,20 +220,25 @@ int gci_profile_read(double *stats);
 
 /* --------------------------------------------- SpMM launch shape, autotuner
  * No counterpart in the reference (SPICMV2 has no tunables).  The symmetric
- * SpMM's launch shape: panel height R (rows whose x and y stay in LDS; the
- * panel-CSC layout is built for it), lanes per column segment L (8, 16),
- * workgroup size (256, 512), segments per LDS chunk `nseg`, batches each
- * lane keeps in flight `depth` (3, 4, 6), and about `target` entries per
- * work item (a slice of one panel).  One shape is in force per process; a
- * matrix laid out for another R is laid out again at its next use outside a
- * Davidson engine.  Set or tune the shape before solves, not while requests
- * are resident (an engine refuses a change of R under resident problems). */
+ * SpMM's launch shape: panel height R (1024 or 2048: rows whose x and y stay
+ * in LDS; the panel-slice layout is built for it), workgroup size (256, 512,
+ * 1024), at most `nslice` slices (64-column groups, whose column tables are
+ * staged in LDS) and about `target` entries per work item (a run of one
+ * panel's steps), and steps each wavefront keeps in flight `depth` (2, 4).
+ * One shape is in force per process; a matrix laid out for another R is laid
+ * out again at its next use outside a Davidson engine.  Set or tune the shape
+ * before solves, not while requests are resident (an engine refuses a change
+ * of R under resident problems). */
 typedef struct gci_spmm_shape {
-    int R, L, block, nseg, depth;
+    int R, block, nslice, depth;
     int64_t target;
 } gci_spmm_shape;
 int gci_spmm_get_shape(gci_spmm_shape *out);
 int gci_spmm_set_shape(const gci_spmm_shape *shape);
+/* The panel-slice layout of a matrix for the shape in force (laid out now if
+ * it was not): its column segments, slices and stored entries (padding
+ * included: stored / off-diagonal entries is the stream's overhead). */
+int gci_spmm_layout(gci_hmat *h, int64_t *nseg, int64_t *nslice, int64_t *stored);
 /* Mean milliseconds of the SpMM kernel launches of one m-column operator
  * application (mode GCI_OP_*) to every matrix of the set at once -- all
  * matrices' work items in one launch per column block, as the batched

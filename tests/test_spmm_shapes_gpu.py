@@ -1,0 +1,86 @@
+"""SPICMV2 parity across the SpMM launch shapes (panel height, workgroup
+size, prefetch depth, slices and entries per work item), every launch width
+and every operator mode; the autotuner keeps the operator exact."""
+import numpy as np
+import pytest
+
+import oracle as O
+from grasp_amd import _lib as L
+from grasp_amd import rci
+from grasp_amd import workloads as W
+from grasp_amd.rci import HamiltonianBlock
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [
+    dict(R=1024, block=512, depth=4, nslice=32, target=65536),
+    dict(R=2048, block=256, depth=2, nslice=64, target=131072),
+    dict(R=1024, block=1024, depth=2, nslice=1, target=4096),   # one slice and 16 steps per item
+    dict(R=2048, block=512, depth=4, nslice=3, target=8192),
+]
+
+
+@pytest.fixture
+def shape_guard(gpu):
+    saved = rci.spmm_shape()
+    yield
+    rci.set_spmm_shape(**saved)
+
+
+def _check(h, csr, m, mode, seed):
+    rng = np.random.default_rng(seed)
+    B = np.asfortranarray(rng.standard_normal((csr.n, m)))
+    got = h.spicmv2(B, mode)
+    want = O.spicmv2(csr.row_end, csr.col, csr.val, B)
+    scale = O.spicmv2(csr.row_end, csr.col, np.abs(csr.val), np.abs(B))
+    # FP64 and hi+lo: backward-error bound 1e-13 (unordered atomics);
+    # bulk: fp32 values and fp32-rounded B
+    tol = 2.5e-7 if mode == L.GCI_OP_SPLIT_BULK else 1e-13
+    err = np.abs(got - want)
+    assert np.all(err <= tol * scale + 1e-300), (m, mode, float(np.max(err / (scale + 1e-300))))
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"R{s['R']}-b{s['block']}-d{s['depth']}-s{s['nslice']}")
+def test_spmm_shapes_all_widths(shape_guard, shape):
+    rci.set_spmm_shape(**shape)
+    csr = W.clustered_block(5200, 31, fill=0.04)
+    h32 = HamiltonianBlock.from_csr(csr, precision=L.GCI_SPLIT32)
+    h64 = HamiltonianBlock.from_csr(csr)
+    for m in (1, 2, 3, 4, 8):
+        _check(h64, csr, m, L.GCI_OP_FP64, 100 + m)
+        _check(h32, csr, m, L.GCI_OP_SPLIT_FULL, 200 + m)
+        _check(h32, csr, m, L.GCI_OP_SPLIT_BULK, 300 + m)
+
+
+@pytest.mark.parametrize("fill,n", [(0.9, 1500), (0.0004, 9000)], ids=["dense", "single-entry-segments"])
+def test_spmm_segment_extremes(shape_guard, fill, n):
+    # a nearly dense block (segments of whole panel height, one slice per
+    # panel column group) and a very sparse one (segments of one entry,
+    # slices that are one step long)
+    rci.set_spmm_shape(R=1024, block=512, depth=4, nslice=8, target=16384)
+    csr = W.clustered_block(n, 41, fill=fill)
+    h = HamiltonianBlock.from_csr(csr)
+    for m in (1, 4, 8):
+        _check(h, csr, m, L.GCI_OP_FP64, 400 + m)
+
+
+def test_spmm_autotune_keeps_parity(shape_guard):
+    csrs = [W.clustered_block(n, 50 + n, fill=0.05) for n in (1800, 4100)]
+    blocks = [HamiltonianBlock.from_csr(c, precision=L.GCI_SPLIT32) for c in csrs]
+    shape, ms = rci.autotune(blocks, 4, L.GCI_OP_SPLIT_BULK, reps=2)
+    assert ms > 0 and rci.spmm_shape() == shape
+    for c, h in zip(csrs, blocks):
+        _check(h, c, 4, L.GCI_OP_SPLIT_BULK, 7)
+        _check(h, c, 4, L.GCI_OP_SPLIT_FULL, 8)
+
+
+def test_spmm_layout_overhead(shape_guard):
+    # the bank schedule pads a slice by at most a few positions beyond its
+    # longest segment, so the stream stays close to the stored entries
+    rci.set_spmm_shape(R=1024)
+    csr = W.clustered_block(6000, 61, fill=0.05)
+    h = HamiltonianBlock.from_csr(csr, precision=L.GCI_SPLIT32)
+    lay = rci.spmm_layout(h)
+    assert lay["slices"] >= lay["segments"] / 64
+    assert lay["stored"] % 256 == 0
+    assert 1.0 <= lay["overhead"] < 1.5, lay
