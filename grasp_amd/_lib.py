@@ -72,6 +72,7 @@ EXPORTS = [
     "gci_profile_enable", "gci_profile_read", "gci_spmm_get_shape", "gci_spmm_set_shape", "gci_spmm_time",
     "gci_spmm_autotune",
     "gci_spmm_layout",
+    "gci_server_submit_shared",
 ]
 
 

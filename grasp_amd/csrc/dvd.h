@@ -147,6 +147,7 @@ class DavidsonEngine {
     DBuf<int> dphase_;
     DBuf<SpmmJob> djobs_;
     DBuf<SpmmItem> ditems_;
+    DBuf<char> dmeta_;  // per-slot chunk ranges, staged for k_set_chunks
     int64_t nitems_ = 0;
     int modes_used_ = 0;           // bitmask of operator modes over active slots
     int mwidth_iter_ = 1;          // launch width for iterations

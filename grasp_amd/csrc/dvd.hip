@@ -922,6 +922,7 @@ DavidsonEngine::DavidsonEngine(int capacity, hipStream_t s) : cap_(capacity), st
     dphase_.alloc(cap_);
     djobs_.alloc(cap_);
     ditems_.alloc(1);
+    dmeta_.alloc((size_t)cap_ * sizeof(ChunkMeta));
     std::vector<DvdProb> init(cap_);
     for (int i = 0; i < cap_; ++i) {
         std::memset(&init[i], 0, sizeof(DvdProb));
@@ -993,9 +994,10 @@ void DavidsonEngine::rebuild_chunks() {
     update_modes();
     std::vector<ChunkMeta> meta(cap_);
     for (int i = 0; i < cap_; ++i) meta[i] = ChunkMeta{ch[i].first, ch[i].second, 0};
-    DBuf<ChunkMeta> dmeta(cap_);
-    GCI_HIP(hipMemcpyAsync(dmeta.p, meta.data(), cap_ * sizeof(ChunkMeta), hipMemcpyHostToDevice, stream_));
-    launch(k_set_chunks, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, (const ChunkMeta *)dmeta.p);
+    // a persistent staging buffer: a temporary's hipFree would synchronise
+    // the whole device at every admission and release
+    GCI_HIP(hipMemcpyAsync(dmeta_.p, meta.data(), cap_ * sizeof(ChunkMeta), hipMemcpyHostToDevice, stream_));
+    launch(k_set_chunks, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, (const ChunkMeta *)dmeta_.p);
     GCI_HIP(hipStreamSynchronize(stream_));
     graph_valid_ = false;
 }

@@ -53,6 +53,12 @@ struct gci_server {
     bool stop = false;
     std::thread worker;
     std::string fatal;
+    // finished requests' own matrices, freed by the reaper thread: hipFree
+    // synchronises the device, which must not stall the worker's steps
+    std::vector<gci::HMat *> graveyard;
+    std::condition_variable cv_reap;
+    bool stop_reaper = false;
+    std::thread reaper;
 };
 
 namespace gci {
@@ -83,11 +89,28 @@ void finish(gci_server *s, Request *r, int code, const std::string &msg) {
     r->code = code;
     r->err = msg;
     r->state = code == GCI_OK ? 2 : 3;
-    if (r->own && r->h) {
-        HMat *h = r->h;
-        if (h->stream) (void)hipStreamDestroy(h->stream);
-        delete static_cast<gci_hmat *>(h);
+    if (r->own && r->h) {  // called with s->mu held
+        s->graveyard.push_back(r->h);
         r->h = nullptr;
+        s->cv_reap.notify_all();
+    }
+}
+
+void free_matrix(HMat *h) {
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete static_cast<gci_hmat *>(h);
+}
+
+void reaper_loop(gci_server *s) {
+    for (;;) {
+        std::vector<HMat *> dead;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_reap.wait(lk, [&] { return s->stop_reaper || !s->graveyard.empty(); });
+            dead.swap(s->graveyard);
+            if (dead.empty() && s->stop_reaper) break;
+        }
+        for (HMat *h : dead) free_matrix(h);
     }
 }
 
@@ -188,6 +211,7 @@ int gci_server_create(size_t hbm_budget_bytes, int max_active, gci_server **out)
         s->eng = std::make_unique<DavidsonEngine>(max_active, s->stream);
         gci_server *raw = s.release();
         raw->worker = std::thread(worker_loop, raw);
+        raw->reaper = std::thread(reaper_loop, raw);
         *out = raw;
     });
 }
@@ -204,29 +228,39 @@ int gci_server_destroy(gci_server *s) {
         }
         s->cv_work.notify_all();
         if (s->worker.joinable()) s->worker.join();
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop_reaper = true;
+        }
+        s->cv_reap.notify_all();
+        if (s->reaper.joinable()) s->reaper.join();  // frees what is left
         s->eng.reset();
         if (s->stream) (void)hipStreamDestroy(s->stream);
         delete s;
     });
 }
 
-int gci_server_submit(gci_server *s, gci_hmat *h, int nev, double tol, double eav, int64_t *req) {
+namespace {
+
+int submit(gci_server *s, gci_hmat *h, int nev, double tol, double eav, int64_t *req, bool own) {
     return guarded([&] {
         GCI_REQUIRE(s && h && req, "gci_server_submit: null argument");
         GCI_REQUIRE(nev >= 1 && nev <= MAXNUME && h->n >= 2 && 2 * nev + 60 <= MAXLIM + 60,
                     "gci_server_submit: bad nev or block order");
         GCI_REQUIRE(tol > 0.0, "gci_server_submit: tolerance must be positive");
         const Clock::time_point t0 = Clock::now();
-        // the SpMM layout is built here, on the caller's thread and the
-        // matrix's stream, so the worker's engine steps never wait for it
+        // the SpMM layout is built here (once per matrix), on the caller's
+        // thread and the matrix's stream, so the worker's engine steps never
+        // wait for it
         ensure_items(*h);
         auto r = std::make_unique<Request>();
         r->h = h;
-        r->own = true;
+        r->own = own;
         r->nev = nev;
         r->tol = tol;
         r->eav = eav;
-        r->bytes = h->device_bytes() + DavidsonEngine::workspace_bytes(spec_for(*r));
+        // a shared matrix stays the caller's: only the workspace is charged
+        r->bytes = (own ? h->device_bytes() : 0) + DavidsonEngine::workspace_bytes(spec_for(*r));
         r->t_submit = t0;
         std::lock_guard<std::mutex> lk(s->mu);
         GCI_REQUIRE(s->fatal.empty(), "gci_server_submit: server failed: " + s->fatal);
@@ -240,6 +274,16 @@ int gci_server_submit(gci_server *s, gci_hmat *h, int nev, double tol, double ea
         s->reqs[r->id] = std::move(r);
         s->cv_work.notify_all();
     });
+}
+
+}  // namespace
+
+int gci_server_submit(gci_server *s, gci_hmat *h, int nev, double tol, double eav, int64_t *req) {
+    return submit(s, h, nev, tol, eav, req, true);
+}
+
+int gci_server_submit_shared(gci_server *s, gci_hmat *h, int nev, double tol, double eav, int64_t *req) {
+    return submit(s, h, nev, tol, eav, req, false);
 }
 
 int gci_server_poll(gci_server *s, int64_t req, double *eval, double *evec, double *latency_s) {

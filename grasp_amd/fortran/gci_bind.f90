@@ -224,6 +224,15 @@ module gci_bind
          integer(c_int) :: st
       end function gci_server_submit
 
+      function gci_server_submit_shared(s, h, nev, tol, eav, req) bind(C, name="gci_server_submit_shared") result(st)
+         import :: c_int, c_int64_t, c_double, c_ptr
+         type(c_ptr), value :: s, h
+         integer(c_int), value :: nev
+         real(c_double), value :: tol, eav
+         integer(c_int64_t) :: req
+         integer(c_int) :: st
+      end function gci_server_submit_shared
+
       function gci_server_poll(s, req, eval, evec, latency_s) bind(C, name="gci_server_poll") result(st)
          import :: c_int, c_int64_t, c_double, c_ptr
          type(c_ptr), value :: s

@@ -307,12 +307,16 @@ class Server:
                                         C.byref(self._s)))
         self._meta = {}
 
-    def submit(self, block: HamiltonianBlock, nev: int, tol: float, eav: float = 0.0) -> int:
+    def submit(self, block: HamiltonianBlock, nev: int, tol: float, eav: float = 0.0, shared: bool = False) -> int:
+        """Queues one solve.  The server takes the block over, unless `shared`:
+        then the caller keeps it (alive until the request is done) and may
+        submit it again."""
         req = C.c_int64(0)
-        check(L.lib().gci_server_submit(self._s, block.handle, C.c_int(nev), C.c_double(tol), C.c_double(eav),
-                                        C.byref(req)))
+        fn = L.lib().gci_server_submit_shared if shared else L.lib().gci_server_submit
+        check(fn(self._s, block.handle, C.c_int(nev), C.c_double(tol), C.c_double(eav), C.byref(req)))
         self._meta[req.value] = (block.n, nev)
-        block.release()  # the server owns the matrix now
+        if not shared:
+            block.release()  # the server owns the matrix now
         return req.value
 
     def poll(self, req: int):

@@ -202,6 +202,10 @@ int gci_server_destroy(gci_server *s);
  * layout is built in this call (on the caller's thread), so the latency of
  * a request runs from here.  Returns a request id in *req. */
 int gci_server_submit(gci_server *s, gci_hmat *h, int nev, double tol, double eav, int64_t *req);
+/* The same for a matrix the caller keeps: the server does not free h, several
+ * requests may share it, and only their Davidson workspace counts against
+ * the HBM budget.  h must outlive every request submitted with it. */
+int gci_server_submit_shared(gci_server *s, gci_hmat *h, int nev, double tol, double eav, int64_t *req);
 /* Non-blocking poll: GCI_OK with results when done, GCI_ERR_BUSY otherwise. */
 int gci_server_poll(gci_server *s, int64_t req, double *eval, double *evec, double *latency_s);
 /* Block until every submitted request has finished. */

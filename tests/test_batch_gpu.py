@@ -97,3 +97,24 @@ def test_server_budget_rejects_oversize(gpu):
     assert e.value.code == -3
     srv.close()
     h.free()
+
+
+def test_server_shared_matrix(gpu):
+    """Requests that share one matrix (the caller keeps it) give the same
+    eigenvalues as a request that owns its copy, and the matrix stays usable."""
+    tol = 1e-9
+    (csr, eav), = _blocks([1500], 300, [0.05])
+    want = _reference(csr, eav, 3)
+    srv = rci.Server(8 << 30, max_active=8)
+    h = rci.HamiltonianBlock.from_csr(csr, shift=-eav, precision=L.GCI_SPLIT32)
+    rids = [srv.submit(h, k, tol, eav, shared=True) for k in (3, 2, 3)]
+    srv.drain()
+    res = [srv.poll(r)[0] for r in rids]
+    srv.close()
+    for got in res:
+        k = len(got)
+        assert np.all(np.abs(got - want[:k]) / np.abs(want[:k] + eav) < tol)
+    # still the caller's: the operator runs on it after the server is gone
+    B = np.asfortranarray(np.random.default_rng(4).standard_normal((csr.n, 2)))
+    assert np.all(np.isfinite(h.spicmv2(B, L.GCI_OP_SPLIT_FULL)))
+    h.free()
