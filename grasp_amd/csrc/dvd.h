@@ -148,6 +148,18 @@ class DavidsonEngine {
     DBuf<SpmmJob> djobs_;
     DBuf<SpmmItem> ditems_;
     DBuf<char> dmeta_;  // per-slot chunk ranges, staged for k_set_chunks
+    // Persistent staging, so admissions and result reads allocate nothing
+    // (a temporary's hipFree synchronises the device) and copy from pinned
+    // memory (pageable copies block the host one by one).
+    DvdProb *hprob_pin_ = nullptr;  // admitted problems' initial state
+    int *hwant_pin_ = nullptr;      // unit-vector start flags
+    char *hsteps_pin_ = nullptr;    // INITDVD step table
+    double *hres_pin_ = nullptr;    // gathered results
+    void *hrec_pin_ = nullptr;      // admitted problems' slots and workspace sizes
+    DBuf<int> dwant_;
+    DBuf<char> dstage_, drec_;      // admitted problems, staged contiguously
+    DBuf<char> dsteps_;
+    DBuf<double> dres_;
     int64_t nitems_ = 0;
     int modes_used_ = 0;           // bitmask of operator modes over active slots
     int mwidth_iter_ = 1;          // launch width for iterations

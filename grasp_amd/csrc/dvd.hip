@@ -911,6 +911,32 @@ void launch(K kern, int grid, int block, hipStream_t s, Args... args) {
 
 // ----------------------------------------------------------------- host
 
+// Admission of a batch of problems, staged contiguously (q = 0..nq-1):
+// zero each workspace with its diagonal block copied in (one pass), then
+// scatter each initial state into its slot.
+struct AdmitRec {
+    int64_t need;  // workspace doubles
+    int slot;
+    int pad;
+};
+
+__global__ void k_admit_work(const DvdProb *__restrict__ stage, const AdmitRec *__restrict__ rec) {
+    const DvdProb &p = stage[blockIdx.y];
+    const int64_t need = rec[blockIdx.y].need;
+    const int64_t d0 = p.diag - p.basis;  // the diagonal's offset in the workspace
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < need; i += (int64_t)gridDim.x * blockDim.x)
+        p.basis[i] = (i >= d0 && i < d0 + p.n) ? p.diagH[i - d0] : 0.0;
+}
+
+__global__ void k_admit_state(DvdProb *__restrict__ P, const DvdProb *__restrict__ stage,
+                              const AdmitRec *__restrict__ rec) {
+    constexpr int W = (int)(sizeof(DvdProb) / sizeof(uint32_t));
+    static_assert(sizeof(DvdProb) % sizeof(uint32_t) == 0, "DvdProb is copied as 32-bit words");
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(&stage[blockIdx.x]);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&P[rec[blockIdx.x].slot]);
+    for (int i = threadIdx.x; i < W; i += blockDim.x) dst[i] = src[i];
+}
+
 DavidsonEngine::DavidsonEngine(int capacity, hipStream_t s) : cap_(capacity), stream_(s) {
     GCI_REQUIRE(capacity >= 1, "davidson: capacity < 1");
     spec_.resize(cap_);
@@ -933,10 +959,26 @@ DavidsonEngine::DavidsonEngine(int capacity, hipStream_t s) : cap_(capacity), st
     chunk_prob_.alloc(1);
     chunk_row_.alloc(1);
     partial_.alloc(PSTRIDE);
+    dwant_.alloc(cap_);
+    dstage_.alloc((size_t)cap_ * sizeof(DvdProb));
+    drec_.alloc((size_t)cap_ * sizeof(AdmitRec));
+    dsteps_.alloc((size_t)cap_ * sizeof(InitStep));
+    dres_.alloc((size_t)cap_ * RSTRIDE);
+    GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hprob_pin_), (size_t)cap_ * sizeof(DvdProb), hipHostMallocDefault));
+    GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hwant_pin_), (size_t)cap_ * sizeof(int), hipHostMallocDefault));
+    GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hrec_pin_), (size_t)cap_ * sizeof(AdmitRec),
+                          hipHostMallocDefault));
+    GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hsteps_pin_), (size_t)cap_ * sizeof(InitStep),
+                          hipHostMallocDefault));
+    GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hres_pin_), (size_t)cap_ * RSTRIDE * sizeof(double),
+                          hipHostMallocDefault));
 }
 
 DavidsonEngine::~DavidsonEngine() {
     if (gexec_) (void)hipGraphExecDestroy(gexec_);
+    (void)hipStreamSynchronize(stream_);
+    for (void *p : {(void *)hprob_pin_, (void *)hwant_pin_, (void *)hsteps_pin_, (void *)hres_pin_, hrec_pin_})
+        if (p) (void)hipHostFree(p);
 }
 
 size_t DavidsonEngine::workspace_bytes(const DvdSpec &s) {
@@ -1005,6 +1047,8 @@ void DavidsonEngine::rebuild_chunks() {
 std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const std::vector<const double *> &init,
                                        const std::vector<int> &niv) {
     std::vector<int> slots, unit_init;
+    int64_t maxneed = 0;
+    GCI_REQUIRE(specs.size() <= (size_t)cap_, "davidson: more problems than slots");
     for (size_t q = 0; q < specs.size(); ++q) {
         const DvdSpec &sp = specs[q];
         GCI_REQUIRE(sp.h && sp.h->n >= 2, "davidson: matrix order < 2");
@@ -1076,27 +1120,40 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         p.slot = slot;
         p.phase = PH_INIT;
         p.niv = niv[q];
-        GCI_HIP(hipMemsetAsync(w, 0, need * sizeof(double), stream_));
-        GCI_HIP(hipMemcpyAsync(p.diag, sp.h->diag.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-        if (init[q])
-            GCI_HIP(hipMemcpyAsync(p.basis, init[q], (size_t)n * niv[q] * sizeof(double), hipMemcpyHostToDevice,
-                                   stream_));
-        else
-            unit_init.push_back(slot);
-        GCI_HIP(hipMemcpyAsync(&dprob_.p[slot], &p, sizeof(DvdProb), hipMemcpyHostToDevice, stream_));
-        GCI_HIP(hipStreamSynchronize(stream_));  // p lives on this stack frame
+        // staged in pinned memory; uploaded and applied for the whole batch below
+        const size_t nq = slots.size() - 1;
+        hprob_pin_[nq] = p;
+        reinterpret_cast<AdmitRec *>(hrec_pin_)[nq] = AdmitRec{(int64_t)need, slot, 0};
+        maxneed = std::max(maxneed, (int64_t)need);
+        if (!init[q]) unit_init.push_back(slot);
         niv_[slot] = niv[q];
         phase_[slot] = PH_INIT;
         refined_[slot] = 0;
     }
-    if (!unit_init.empty()) {
-        std::vector<int> want(cap_, 0);
-        for (int sl : unit_init) want[sl] = 1;
-        DBuf<int> dw(cap_);
-        GCI_HIP(hipMemcpyAsync(dw.p, want.data(), cap_ * sizeof(int), hipMemcpyHostToDevice, stream_));
-        launch(k_init_unit, cap_, 256, stream_, dprob_.p, (const int *)dw.p);
-        GCI_HIP(hipStreamSynchronize(stream_));
+    if (!slots.empty()) {
+        const int nq = (int)slots.size();
+        GCI_HIP(hipMemcpyAsync(dstage_.p, hprob_pin_, nq * sizeof(DvdProb), hipMemcpyHostToDevice, stream_));
+        GCI_HIP(hipMemcpyAsync(drec_.p, hrec_pin_, nq * sizeof(AdmitRec), hipMemcpyHostToDevice, stream_));
+        const DvdProb *stage = reinterpret_cast<const DvdProb *>(dstage_.p);
+        const AdmitRec *rec = reinterpret_cast<const AdmitRec *>(drec_.p);
+        const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((maxneed + 255) / 256, 1024));
+        hipLaunchKernelGGL(k_admit_work, dim3(gx, (unsigned)nq), dim3(256), 0, stream_, stage, rec);
+        GCI_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_admit_state, dim3((unsigned)nq), dim3(256), 0, stream_, dprob_.p, stage, rec);
+        GCI_HIP(hipGetLastError());
+        // caller-given initial vectors (the single-problem path) after the zeroing
+        for (size_t q = 0; q < specs.size(); ++q)
+            if (init[q])
+                GCI_HIP(hipMemcpyAsync(hprob_pin_[q].basis, init[q], (size_t)specs[q].h->n * niv[q] * sizeof(double),
+                                       hipMemcpyHostToDevice, stream_));
     }
+    if (!unit_init.empty()) {
+        std::fill(hwant_pin_, hwant_pin_ + cap_, 0);
+        for (int sl : unit_init) hwant_pin_[sl] = 1;
+        GCI_HIP(hipMemcpyAsync(dwant_.p, hwant_pin_, cap_ * sizeof(int), hipMemcpyHostToDevice, stream_));
+        launch(k_init_unit, cap_, 256, stream_, dprob_.p, (const int *)dwant_.p);
+    }
+    GCI_HIP(hipStreamSynchronize(stream_));  // the pinned staging is reused by the next admission
     rebuild_chunks();
     run_initdvd(slots);
     return slots;
@@ -1107,7 +1164,7 @@ void DavidsonEngine::run_initdvd(const std::vector<int> &slots) {
     // NIV initial vectors, then orthonormalised AB vectors up to NUME+1,
     // then S = B^T AB.
     const int G = (int)total_chunks_;
-    DBuf<InitStep> dsteps(cap_);
+    InitStep *dsteps = reinterpret_cast<InitStep *>(dsteps_.p);
     std::vector<InitStep> steps(cap_, InitStep{0, 0, 0});
     std::vector<int> ist(cap_, 1), iend(cap_, 0), nume(cap_, 0);
     int init_width = 1;
@@ -1118,10 +1175,12 @@ void DavidsonEngine::run_initdvd(const std::vector<int> &slots) {
         init_width = std::max(init_width, std::max(niv_[sl], spec_[sl].nume));
         steps[sl] = InitStep{1, 0, niv_[sl]};
     }
+    InitStep *hsteps = reinterpret_cast<InitStep *>(hsteps_pin_);
     auto push = [&]() {
-        GCI_HIP(hipMemcpyAsync(dsteps.p, steps.data(), cap_ * sizeof(InitStep), hipMemcpyHostToDevice, stream_));
-        launch(k_init_state, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, (const InitStep *)dsteps.p);
-        GCI_HIP(hipStreamSynchronize(stream_));
+        GCI_HIP(hipStreamSynchronize(stream_));  // the previous table's copy has finished
+        std::copy(steps.begin(), steps.end(), hsteps);
+        GCI_HIP(hipMemcpyAsync(dsteps, hsteps, cap_ * sizeof(InitStep), hipMemcpyHostToDevice, stream_));
+        launch(k_init_state, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, (const InitStep *)dsteps);
     };
     push();
     launch_op(PH_INIT, init_width);
@@ -1143,7 +1202,7 @@ void DavidsonEngine::run_initdvd(const std::vector<int> &slots) {
         if (!any) break;
         push();
         launch(k_init_copy, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p,
-               (const InitStep *)dsteps.p);
+               (const InitStep *)dsteps);
         launch_orth(2, PH_INIT);
         launch_op(PH_INIT, init_width);
     }
@@ -1340,11 +1399,11 @@ void DavidsonEngine::extract(int slot, double *eval, double *evec, double *resid
 }
 
 void DavidsonEngine::extract_all(std::vector<double> &res) {
-    DBuf<double> d((size_t)cap_ * RSTRIDE);
-    launch(k_gather_results, (cap_ + 63) / 64, 64, stream_, (const DvdProb *)dprob_.p, cap_, d.p);
-    res.resize((size_t)cap_ * RSTRIDE);
-    GCI_HIP(hipMemcpyAsync(res.data(), d.p, d.bytes(), hipMemcpyDeviceToHost, stream_));
+    launch(k_gather_results, (cap_ + 63) / 64, 64, stream_, (const DvdProb *)dprob_.p, cap_, dres_.p);
+    GCI_HIP(hipMemcpyAsync(hres_pin_, dres_.p, (size_t)cap_ * RSTRIDE * sizeof(double), hipMemcpyDeviceToHost,
+                           stream_));
     GCI_HIP(hipStreamSynchronize(stream_));
+    res.assign(hres_pin_, hres_pin_ + (size_t)cap_ * RSTRIDE);
 }
 
 void DavidsonEngine::copy_vectors(int slot, double *evec) {
