@@ -48,6 +48,7 @@ struct DvdProb {
     int64_t chunk0;                 // first entry in the flattened chunk list
     int nchunks;
     int slot;
+    double tol_final, critr_final;  // targets of the FP64 refinement pass
     // --- dynamic state
     int kpass, nncv, nnew, rest, nloops, ierr, nmv;
     int phase, ovflow, ncand, conv_all, niv;
@@ -55,6 +56,8 @@ struct DvdProb {
     int refine_loops;
     int bulk_loops, bulk_nmv;       // iterations/products of the bulk phase
     int brk_mask;                   // new columns that lost rank in the first GS pass
+    int init_ist, init_iend;        // INITDVD progress: next AB column to copy, columns built
+    int init_src;                   // AB column copied by the current INITDVD sub-step
     // --- small arrays (1-based reference arrays stored 0-based)
     double eigval[MAXLIM];
     double oldval[MAXLIM + 1];
@@ -100,6 +103,13 @@ class DavidsonEngine {
                            const std::vector<int> &niv);
     void step(int iters);                 // enqueue iterations (no sync)
     std::vector<int> poll();              // sync; slots newly in PH_DONE
+    // Slots newly in PH_DONE as of the iteration before the last step():
+    // waits only for that one, so the device keeps the last step's work
+    // queued while the host decides (every iteration ends by writing the
+    // slots' phases to mapped host memory).
+    std::vector<int> poll_lagged();
+    // every admitted slot is done and needs no refinement pass
+    bool all_done() const;
     // Restart finished bulk-phase slots with the FP64 operator from their
     // Ritz vectors (the refinement pass).
     void refine_restart(const std::vector<int> &slots);
@@ -110,8 +120,13 @@ class DavidsonEngine {
     // compact per-slot results: [nloops, nmv, ierr, refined, bulk_loops,
     // eigval(16), resn(16)] (RESULT_STRIDE doubles per slot)
     void extract_all(std::vector<double> &res);
-    void copy_vectors(int slot, double *evec);  // async on the engine stream
-    void sync() { GCI_HIP(hipStreamSynchronize(stream_)); }
+    // eigenvectors of the given slots, one after another (n x NUME each,
+    // vector-major), into evec; returns when the copy has completed
+    void copy_vectors(const std::vector<int> &slots, double *evec);
+    void sync() {
+        GCI_HIP(hipStreamSynchronize(stream_));
+        waited_seq_ = launched_;
+    }
     int active() const;
     int capacity() const { return cap_; }
     int phase(int slot) const { return phase_[slot]; }
@@ -153,12 +168,33 @@ class DavidsonEngine {
     // memory (pageable copies block the host one by one).
     DvdProb *hprob_pin_ = nullptr;  // admitted problems' initial state
     int *hwant_pin_ = nullptr;      // unit-vector start flags
-    char *hsteps_pin_ = nullptr;    // INITDVD step table
     double *hres_pin_ = nullptr;    // gathered results
     void *hrec_pin_ = nullptr;      // admitted problems' slots and workspace sizes
+    int *hphase_pin_ = nullptr;     // phases read by poll()
+    // iteration snapshot: [sequence number, phase of each slot], written by
+    // the last kernel of every iteration into mapped host memory
+    int *hsnap_pin_ = nullptr;
+    int *dsnap_host_ = nullptr;     // device address of hsnap_pin_
+    DBuf<int> dseq_;                // iterations completed (device counter)
+    int64_t launched_ = 0;          // iterations launched
+    std::vector<int64_t> restart_seq_;  // snapshots up to this one predate a slot's (re)start
+    static constexpr int NEV = 4;
+    hipEvent_t ev_[NEV] = {};
+    int64_t ev_seq_[NEV] = {};      // iterations launched when ev_[i] was recorded
+    int nev_rec_ = 0;               // events recorded
+    int *hmask_pin_ = nullptr;      // refinement restart flags (NEV rows of cap_), mapped
+    int *dmask_host_ = nullptr;
+    int mask_row_ = 0;
+    int64_t mask_seq_[NEV] = {-1, -1, -1, -1};  // launched_ when each flag row was last written
+    int64_t waited_seq_ = 0;        // iterations known complete
+    double *hvec_pin_ = nullptr;    // staging of copy_vectors()
+    size_t hvec_bytes_ = 0;
+    std::vector<hipEvent_t> vec_ev_;
+    int64_t *hvoff_pin_ = nullptr;  // per-slot offsets for copy_vectors()
+    DBuf<int64_t> dvoff_;
+    DBuf<double> dvec_;             // gathered eigenvectors
     DBuf<int> dwant_;
     DBuf<char> dstage_, drec_;      // admitted problems, staged contiguously
-    DBuf<char> dsteps_;
     DBuf<double> dres_;
     int64_t nitems_ = 0;
     int modes_used_ = 0;           // bitmask of operator modes over active slots

@@ -29,11 +29,13 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
 #include <numeric>
+#include <thread>
 
 namespace gci {
 
@@ -757,33 +759,64 @@ __global__ __launch_bounds__(256) void k_unpack_slots(DvdProb *P, const int *chu
 
 // ------------------------------------------------------------ INITDVD
 // INITDVD (dvdson.f90:865-946) sub-steps for problems in PH_INIT/PH_INIT_IDLE.
-struct InitStep {
-    int ist, iend, nnew;
-};
-
-// copy AB columns [ist-1, ist-1+new) to basis[:, iend..]
-__global__ __launch_bounds__(256) void k_init_copy(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
-                                                   const InitStep *steps) {
-    const int pi = chunk_prob[blockIdx.x];
-    DvdProb &p = P[pi];
-    if (p.phase != PH_INIT) return;
-    const InitStep st = steps[pi];
-    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
-    for (int j = 0; j < st.nnew; ++j)
-        for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x)
-            p.basis[r + (int64_t)(st.iend + j) * p.n] = p.ab[r + (int64_t)(st.ist - 1 + j) * p.n];
-}
-
-// KPASS = IEND, NNEW = new for the next orthogonalisation/operator sub-step
-__global__ void k_init_state(DvdProb *P, int nprob, const InitStep *steps) {
+// INITDVD sub-steps (dvdson.f90:865-946).  Stage 0 applies OP to the NIV
+// initial vectors; every later stage copies the newest AB columns into the
+// basis (up to doubling the basis, at most NUME+1 columns in all), and the
+// final stage (-1) forms S over the whole basis.  Each problem advances its
+// own columns, so problems with different NIV and NUME share the launches.
+__global__ void k_init_stage(DvdProb *P, int nprob, int stage) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nprob) return;
     DvdProb &p = P[i];
     if (p.phase != PH_INIT && p.phase != PH_INIT_IDLE) return;
-    const InitStep st = steps[i];
-    p.phase = st.nnew > 0 ? PH_INIT : PH_INIT_IDLE;
-    p.kpass = st.iend;
-    p.nnew = st.nnew;
+    int src = 0, kp, nw;
+    if (stage == 0) {
+        p.init_ist = 1;
+        p.init_iend = p.niv;
+        kp = 0;
+        nw = p.niv;
+    } else if (stage > 0) {
+        const int need = p.nume + 1;
+        kp = p.init_iend;
+        nw = 0;
+        if (p.init_iend < need) {
+            nw = min(need - p.init_iend, p.init_iend - p.init_ist + 1);
+            src = p.init_ist;
+            p.init_ist = p.init_iend + 1;
+            p.init_iend += nw;
+        }
+    } else {
+        kp = 0;
+        nw = p.init_iend;
+    }
+    p.phase = nw > 0 ? PH_INIT : PH_INIT_IDLE;
+    p.kpass = kp;
+    p.nnew = nw;
+    p.init_src = src;
+}
+
+// copy AB columns [src-1, src-1+NNEW) to basis[:, KPASS..]
+__global__ __launch_bounds__(256) void k_init_copy(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row) {
+    const int pi = chunk_prob[blockIdx.x];
+    DvdProb &p = P[pi];
+    if (p.phase != PH_INIT) return;
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    for (int j = 0; j < p.nnew; ++j)
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x)
+            p.basis[r + (int64_t)(p.kpass + j) * p.n] = p.ab[r + (int64_t)(p.init_src - 1 + j) * p.n];
+}
+
+// Number of INITDVD copy stages (after stage 0) for NIV initial vectors
+// and NUME wanted pairs; the host runs the largest over the problems.
+inline int init_stages(int niv, int nume) {
+    int ist = 1, iend = niv, k = 0;
+    while (iend < nume + 1) {
+        const int nw = std::min(nume + 1 - iend, iend - ist + 1);
+        ist = iend + 1;
+        iend += nw;
+        ++k;
+    }
+    return k;
 }
 
 // enter DVDRVR with KPASS = NIV (dvdson.f90:690-705)
@@ -805,18 +838,33 @@ __global__ void k_init_done(DvdProb *P, int nprob) {
 }
 
 // refinement restart: FP64 operator, fresh GDVD state from the Ritz vectors
-// already in basis[:, 0:NUME]; the preconditioner diagonal restarts from H
-__global__ void k_refine_begin(DvdProb *P, int slot, double tol_final, double critr_final) {
-    DvdProb &p = P[slot];
+// already in basis[:, 0:NUME] for the slots flagged in mask (mapped host
+// memory); the preconditioner diagonal restarts from H
+__global__ void k_refine_begin(DvdProb *P, int nprob, const int *mask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprob || !mask[i]) return;
+    DvdProb &p = P[i];
     p.bulk_loops = p.nloops;
     p.bulk_nmv = p.nmv;
     p.refine_phase = 1;
     p.phase = PH_INIT;
     p.niv = p.nume;
-    p.tol_rel = tol_final;
-    p.critr = critr_final;
+    p.tol_rel = p.tol_final;
+    p.critr = p.critr_final;
     p.nloops = 0;
     p.conv_all = 0;
+}
+
+// Last kernel of an iteration: counts it and writes the slots' phases, then
+// the count, to mapped host memory (snap[0] = count, snap[1 + i] = phase).
+__global__ void k_snapshot(const DvdProb *P, int nprob, int *seq, int *snap) {
+    __shared__ int s;
+    if (threadIdx.x == 0) s = ++*seq;
+    __syncthreads();
+    for (int i = threadIdx.x; i < nprob; i += blockDim.x) snap[1 + i] = P[i].phase;
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(snap, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 __global__ __launch_bounds__(256) void k_copy_diag(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row) {
@@ -901,6 +949,28 @@ __global__ void k_gather_results(const DvdProb *P, int nprob, double *out) {
     }
 }
 
+// Copies the NUME eigenvectors of every slot with voff[slot] >= 0 into one
+// contiguous buffer (vector-major per slot, starting at out + voff[slot]).
+__global__ __launch_bounds__(256) void k_gather_vectors(const DvdProb *P, const int *chunk_prob,
+                                                        const int64_t *chunk_row, const int64_t *voff,
+                                                        double *out) {
+    const int pi = chunk_prob[blockIdx.x];
+    const int64_t o = voff[pi];
+    if (o < 0) return;
+    const DvdProb &p = P[pi];
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    const double *b = static_cast<const double *>(p.basis);
+    for (int q = 0; q < p.nume; ++q)
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) out[o + q * p.n + r] = b[q * p.n + r];
+}
+
+// Sets the phase of one slot, or of every slot when slot < 0.
+__global__ void k_set_phase(DvdProb *P, int nprob, int slot, int phase) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprob || (slot >= 0 && i != slot)) return;
+    P[i].phase = phase;
+}
+
 template <class K, class... Args>
 void launch(K kern, int grid, int block, hipStream_t s, Args... args) {
     if (grid <= 0) return;
@@ -962,23 +1032,42 @@ DavidsonEngine::DavidsonEngine(int capacity, hipStream_t s) : cap_(capacity), st
     dwant_.alloc(cap_);
     dstage_.alloc((size_t)cap_ * sizeof(DvdProb));
     drec_.alloc((size_t)cap_ * sizeof(AdmitRec));
-    dsteps_.alloc((size_t)cap_ * sizeof(InitStep));
     dres_.alloc((size_t)cap_ * RSTRIDE);
     GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hprob_pin_), (size_t)cap_ * sizeof(DvdProb), hipHostMallocDefault));
     GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hwant_pin_), (size_t)cap_ * sizeof(int), hipHostMallocDefault));
     GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hrec_pin_), (size_t)cap_ * sizeof(AdmitRec),
                           hipHostMallocDefault));
-    GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hsteps_pin_), (size_t)cap_ * sizeof(InitStep),
-                          hipHostMallocDefault));
     GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hres_pin_), (size_t)cap_ * RSTRIDE * sizeof(double),
                           hipHostMallocDefault));
+    GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hphase_pin_), (size_t)cap_ * sizeof(int), hipHostMallocDefault));
+    GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hvoff_pin_), (size_t)cap_ * sizeof(int64_t),
+                          hipHostMallocDefault));
+    dvoff_.alloc(cap_);
+    // mapped host memory the kernels read (restart flags) and write
+    // (iteration snapshots) directly
+    GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hsnap_pin_), (size_t)(cap_ + 1) * sizeof(int),
+                          hipHostMallocMapped));
+    GCI_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&dsnap_host_), hsnap_pin_, 0));
+    GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hmask_pin_), (size_t)NEV * cap_ * sizeof(int),
+                          hipHostMallocMapped));
+    GCI_HIP(hipHostGetDevicePointer(reinterpret_cast<void **>(&dmask_host_), hmask_pin_, 0));
+    std::fill(hsnap_pin_, hsnap_pin_ + cap_ + 1, 0);
+    dseq_.alloc(1);
+    GCI_HIP(hipMemset(dseq_.p, 0, sizeof(int)));
+    GCI_HIP(hipDeviceSynchronize());
+    restart_seq_.assign(cap_, 0);
+    for (int i = 0; i < NEV; ++i) GCI_HIP(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
 }
 
 DavidsonEngine::~DavidsonEngine() {
     if (gexec_) (void)hipGraphExecDestroy(gexec_);
     (void)hipStreamSynchronize(stream_);
-    for (void *p : {(void *)hprob_pin_, (void *)hwant_pin_, (void *)hsteps_pin_, (void *)hres_pin_, hrec_pin_})
+    for (void *p : {(void *)hprob_pin_, (void *)hwant_pin_, (void *)hres_pin_, hrec_pin_, (void *)hphase_pin_,
+                    (void *)hvoff_pin_, (void *)hsnap_pin_, (void *)hmask_pin_, (void *)hvec_pin_})
         if (p) (void)hipHostFree(p);
+    for (hipEvent_t e : ev_)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : vec_ev_) (void)hipEventDestroy(e);
 }
 
 size_t DavidsonEngine::workspace_bytes(const DvdSpec &s) {
@@ -1115,6 +1204,8 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         p.phi = sp.h->pc.hi.p;
         p.plo = sp.h->pc.lo.p;
         p.tol_rel = sp.tol_rel;
+        p.tol_final = sp.tol_final;
+        p.critr_final = sp.critr_final > 0 ? sp.critr_final : sp.critr;
         p.eshift = sp.eshift;
         p.pnnz = sp.h->nnz;
         p.slot = slot;
@@ -1162,59 +1253,34 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
 void DavidsonEngine::run_initdvd(const std::vector<int> &slots) {
     // INITDVD (dvdson.f90:865-946) for the problems in PH_INIT: OP on the
     // NIV initial vectors, then orthonormalised AB vectors up to NUME+1,
-    // then S = B^T AB.
+    // then S = B^T AB.  Each problem's sub-steps are derived on the device
+    // from its NIV and NUME, so nothing is copied and nothing waits here.
     const int G = (int)total_chunks_;
-    InitStep *dsteps = reinterpret_cast<InitStep *>(dsteps_.p);
-    std::vector<InitStep> steps(cap_, InitStep{0, 0, 0});
-    std::vector<int> ist(cap_, 1), iend(cap_, 0), nume(cap_, 0);
-    int init_width = 1;
+    int init_width = 1, stages = 0;
     update_modes();
     for (int sl : slots) {
-        iend[sl] = niv_[sl];
-        nume[sl] = spec_[sl].nume;
         init_width = std::max(init_width, std::max(niv_[sl], spec_[sl].nume));
-        steps[sl] = InitStep{1, 0, niv_[sl]};
+        stages = std::max(stages, init_stages(niv_[sl], spec_[sl].nume));
     }
-    InitStep *hsteps = reinterpret_cast<InitStep *>(hsteps_pin_);
-    auto push = [&]() {
-        GCI_HIP(hipStreamSynchronize(stream_));  // the previous table's copy has finished
-        std::copy(steps.begin(), steps.end(), hsteps);
-        GCI_HIP(hipMemcpyAsync(dsteps, hsteps, cap_ * sizeof(InitStep), hipMemcpyHostToDevice, stream_));
-        launch(k_init_state, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, (const InitStep *)dsteps);
-    };
-    push();
+    const int pg = (cap_ + 63) / 64;
+    launch(k_init_stage, pg, 64, stream_, dprob_.p, cap_, 0);
     launch_op(PH_INIT, init_width);
-    for (;;) {
-        bool any = false;
-        std::fill(steps.begin(), steps.end(), InitStep{0, 0, 0});
-        for (int sl : slots) {
-            const int need = nume[sl] + 1;
-            if (iend[sl] < need) {
-                const int nw = std::min(need - iend[sl], iend[sl] - ist[sl] + 1);
-                steps[sl] = InitStep{ist[sl], iend[sl], nw};
-                ist[sl] = iend[sl] + 1;
-                iend[sl] += nw;
-                any = true;
-            } else {
-                steps[sl] = InitStep{0, iend[sl], 0};
-            }
-        }
-        if (!any) break;
-        push();
-        launch(k_init_copy, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p,
-               (const InitStep *)dsteps);
+    for (int st = 1; st <= stages; ++st) {
+        launch(k_init_stage, pg, 64, stream_, dprob_.p, cap_, st);
+        launch(k_init_copy, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p);
         launch_orth(2, PH_INIT);
         launch_op(PH_INIT, init_width);
     }
-    for (int sl : slots) steps[sl] = InitStep{0, 0, iend[sl]};
-    push();
+    launch(k_init_stage, pg, 64, stream_, dprob_.p, cap_, -1);
     launch(k_gram, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p,
            partial_.p, 2, (int)PH_INIT);
     launch(k_adds, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, (int)PH_INIT, 0);
-    launch(k_init_done, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_);
+    launch(k_init_done, pg, 64, stream_, dprob_.p, cap_);
     GCI_HIP(hipGetLastError());
-    GCI_HIP(hipStreamSynchronize(stream_));
-    for (int sl : slots) phase_[sl] = PH_ITER;
+    for (int sl : slots) {
+        phase_[sl] = PH_ITER;
+        restart_seq_[sl] = launched_;
+    }
 }
 
 void DavidsonEngine::launch_orth(int passes, int phase) {
@@ -1306,6 +1372,7 @@ void DavidsonEngine::launch_iteration() {
     launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 2, (int)PH_ITER);
     launch(k_adds, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, (int)PH_ITER, 1);
     launch_finalize();
+    launch(k_snapshot, 1, 256, stream_, (const DvdProb *)dprob_.p, cap_, dseq_.p, dsnap_host_);
 }
 
 void DavidsonEngine::drain_profile() {
@@ -1344,15 +1411,48 @@ void DavidsonEngine::step(int iters) {
         for (int it = 0; it < iters; ++it) launch_iteration();
     }
     GCI_HIP(hipGetLastError());
+    if (iters <= 0) return;
+    launched_ += iters;
+    const int e = nev_rec_ % NEV;
+    GCI_HIP(hipEventRecord(ev_[e], stream_));
+    ev_seq_[e] = launched_;
+    ++nev_rec_;
+}
+
+std::vector<int> DavidsonEngine::poll_lagged() {
+    drain_profile();
+    std::vector<int> done;
+    if (nev_rec_ == 0) return done;
+    // the previous step's event when there is one, else the last
+    const int e = (nev_rec_ >= 2 ? nev_rec_ - 2 : nev_rec_ - 1) % NEV;
+    GCI_HIP(hipEventSynchronize(ev_[e]));
+    const int seq = __atomic_load_n(&hsnap_pin_[0], __ATOMIC_ACQUIRE);
+    GCI_REQUIRE(seq >= ev_seq_[e], "davidson: iteration snapshot behind its event");
+    waited_seq_ = std::max<int64_t>(waited_seq_, ev_seq_[e]);
+    for (int i = 0; i < cap_; ++i) {
+        if (phase_[i] == PH_EMPTY || seq <= restart_seq_[i]) continue;
+        const int ph = __atomic_load_n(&hsnap_pin_[1 + i], __ATOMIC_RELAXED);
+        if (ph == PH_DONE && phase_[i] != PH_DONE) done.push_back(i);
+        phase_[i] = ph;
+    }
+    return done;
+}
+
+bool DavidsonEngine::all_done() const {
+    for (int i = 0; i < cap_; ++i) {
+        if (phase_[i] == PH_EMPTY) continue;
+        if (phase_[i] != PH_DONE || needs_refine(i)) return false;
+    }
+    return true;
 }
 
 std::vector<int> DavidsonEngine::poll() {
     drain_profile();
     GCI_HIP(hipMemcpy2DAsync(dphase_.p, sizeof(int), &dprob_.p[0].phase, sizeof(DvdProb), sizeof(int), cap_,
                              hipMemcpyDeviceToDevice, stream_));
-    std::vector<int> ph(cap_);
-    GCI_HIP(hipMemcpyAsync(ph.data(), dphase_.p, cap_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
-    GCI_HIP(hipStreamSynchronize(stream_));
+    int *ph = hphase_pin_;
+    GCI_HIP(hipMemcpyAsync(ph, dphase_.p, cap_ * sizeof(int), hipMemcpyDeviceToHost, stream_));
+    sync();
     std::vector<int> done;
     for (int i = 0; i < cap_; ++i) {
         if (phase_[i] == PH_EMPTY) continue;
@@ -1366,18 +1466,26 @@ bool DavidsonEngine::needs_refine(int slot) const { return spec_[slot].refine &&
 
 void DavidsonEngine::refine_restart(const std::vector<int> &slots) {
     if (slots.empty()) return;
+    // a flag row is read by the device after the iteration queued last, so
+    // it can be rewritten once a later iteration has been waited for
+    if (mask_seq_[mask_row_] >= waited_seq_) sync();  // the row's last reader may not have run yet
+    int *mask = hmask_pin_ + (size_t)mask_row_ * cap_;
+    const int *dmask = dmask_host_ + (size_t)mask_row_ * cap_;
+    mask_seq_[mask_row_] = launched_;
+    mask_row_ = (mask_row_ + 1) % NEV;
+    std::fill(mask, mask + cap_, 0);
     for (int sl : slots) {
         GCI_REQUIRE(phase_[sl] == PH_DONE && needs_refine(sl), "davidson: slot not ready for refinement");
-        const DvdSpec &sp = spec_[sl];
-        launch(k_refine_begin, 1, 1, stream_, dprob_.p, sl, sp.tol_final, sp.critr_final > 0 ? sp.critr_final : sp.critr);
+        mask[sl] = 1;
         refined_[sl] = 1;
         phase_[sl] = PH_INIT;
-        niv_[sl] = sp.nume;
+        niv_[sl] = spec_[sl].nume;
     }
+    std::atomic_thread_fence(std::memory_order_release);
+    launch(k_refine_begin, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, dmask);
     update_modes();
     launch(k_copy_diag, (int)total_chunks_, 256, stream_, dprob_.p, (const int *)chunk_prob_.p,
            (const int64_t *)chunk_row_.p);
-    GCI_HIP(hipStreamSynchronize(stream_));
     run_initdvd(slots);
 }
 
@@ -1406,17 +1514,77 @@ void DavidsonEngine::extract_all(std::vector<double> &res) {
     res.assign(hres_pin_, hres_pin_ + (size_t)cap_ * RSTRIDE);
 }
 
-void DavidsonEngine::copy_vectors(int slot, double *evec) {
-    const int64_t n = spec_[slot].h->n;
-    GCI_HIP(hipMemcpyAsync(evec, work_[slot].p, (size_t)n * spec_[slot].nume * sizeof(double),
-                           hipMemcpyDeviceToHost, stream_));
+void DavidsonEngine::copy_vectors(const std::vector<int> &slots, double *evec) {
+    // one gather kernel into a contiguous device buffer, then one copy to
+    // the host (per-slot copies to pageable memory each stall the host)
+    if (slots.empty()) return;
+    int64_t total = 0;
+    std::fill(hvoff_pin_, hvoff_pin_ + cap_, (int64_t)-1);
+    for (int sl : slots) {
+        GCI_REQUIRE(sl >= 0 && sl < cap_ && phase_[sl] != PH_EMPTY, "davidson: bad slot");
+        GCI_REQUIRE(hvoff_pin_[sl] < 0, "davidson: slot listed twice");
+        hvoff_pin_[sl] = total;
+        total += spec_[sl].h->n * spec_[sl].nume;
+    }
+    if ((int64_t)dvec_.n < total) {
+        GCI_HIP(hipStreamSynchronize(stream_));  // the old buffer may still be read
+        dvec_.alloc((size_t)total);
+    }
+    const size_t bytes = (size_t)total * sizeof(double);
+    if (hvec_bytes_ < bytes) {
+        GCI_HIP(hipStreamSynchronize(stream_));
+        if (hvec_pin_) GCI_HIP(hipHostFree(hvec_pin_));
+        hvec_pin_ = nullptr;
+        hvec_bytes_ = 0;
+        GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hvec_pin_), bytes, hipHostMallocDefault));
+        hvec_bytes_ = bytes;
+    }
+    GCI_HIP(hipMemcpyAsync(dvoff_.p, hvoff_pin_, cap_ * sizeof(int64_t), hipMemcpyHostToDevice, stream_));
+    launch(k_gather_vectors, (int)total_chunks_, 256, stream_, (const DvdProb *)dprob_.p,
+           (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p, (const int64_t *)dvoff_.p, dvec_.p);
+    GCI_HIP(hipGetLastError());
+    // device -> pinned staging in pieces, each followed by an event; host
+    // threads copy a piece on into the caller's (pageable) memory as soon
+    // as it has arrived, overlapping the transfer with the host copies
+    constexpr size_t PIECE = (size_t)2 << 20;
+    const int np = (int)((bytes + PIECE - 1) / PIECE);
+    while ((int)vec_ev_.size() < np) {
+        hipEvent_t e;
+        GCI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        vec_ev_.push_back(e);
+    }
+    char *stage = reinterpret_cast<char *>(hvec_pin_);
+    const char *src = reinterpret_cast<const char *>(dvec_.p);
+    for (int i = 0; i < np; ++i) {
+        const size_t o = (size_t)i * PIECE, len = std::min(PIECE, bytes - o);
+        GCI_HIP(hipMemcpyAsync(stage + o, src + o, len, hipMemcpyDeviceToHost, stream_));
+        GCI_HIP(hipEventRecord(vec_ev_[i], stream_));
+    }
+    char *dst = reinterpret_cast<char *>(evec);
+    const int nth = std::max(1, std::min(np, 8));
+    std::vector<hipError_t> err(nth, hipSuccess);
+    auto worker = [&](int t) {
+        for (int i = t; i < np; i += nth) {
+            const size_t o = (size_t)i * PIECE, len = std::min(PIECE, bytes - o);
+            const hipError_t r = hipEventSynchronize(vec_ev_[i]);
+            if (r != hipSuccess) {
+                err[t] = r;
+                return;
+            }
+            std::memcpy(dst + o, stage + o, len);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; ++t) th.emplace_back(worker, t);
+    worker(0);
+    for (auto &t : th) t.join();
+    for (hipError_t r : err) GCI_HIP(r);
+    GCI_HIP(hipStreamSynchronize(stream_));  // the pinned offsets are reused by the next call
 }
 
 void DavidsonEngine::release_all() {
-    GCI_HIP(hipStreamSynchronize(stream_));
-    std::vector<int> empty_ph(cap_, PH_EMPTY);
-    GCI_HIP(hipMemcpy2D(&dprob_.p[0].phase, sizeof(DvdProb), empty_ph.data(), sizeof(int), sizeof(int), cap_,
-                        hipMemcpyHostToDevice));
+    launch(k_set_phase, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, -1, (int)PH_EMPTY);
+    GCI_HIP(hipGetLastError());
     for (int i = 0; i < cap_; ++i) {
         spec_[i] = DvdSpec{};
         phase_[i] = PH_EMPTY;
@@ -1426,9 +1594,8 @@ void DavidsonEngine::release_all() {
 
 void DavidsonEngine::release(int slot) {
     GCI_REQUIRE(slot >= 0 && slot < cap_ && phase_[slot] != PH_EMPTY, "davidson: bad slot");
-    GCI_HIP(hipStreamSynchronize(stream_));
-    const int empty = PH_EMPTY;
-    GCI_HIP(hipMemcpy(&dprob_.p[slot].phase, &empty, sizeof(int), hipMemcpyHostToDevice));
+    launch(k_set_phase, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, slot, (int)PH_EMPTY);
+    GCI_HIP(hipGetLastError());
     spec_[slot] = DvdSpec{};
     phase_[slot] = PH_EMPTY;
     rebuild_chunks();

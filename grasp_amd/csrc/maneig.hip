@@ -82,25 +82,25 @@ DvdSpec resolve(const HMat &h, const gci_dvd_params *p) {
 }
 
 // Runs an engine until every slot is done, restarting split-precision slots
-// once with the FP64 operator.
+// once with the FP64 operator.  The host stays one iteration ahead: it
+// queues the next iteration before reading the state after the previous
+// one, so the device does not wait for the host's decisions.
 void run_to_completion(DavidsonEngine &eng, int guard_iters) {
     int it = 0;
+    eng.step(1);
+    ++it;
     for (;;) {
-        eng.step(8);
-        it += 8;
-        std::vector<int> done = eng.poll();
+        eng.step(1);
+        ++it;
+        std::vector<int> done = eng.poll_lagged();
         std::vector<int> ref;
         for (int sl : done)
             if (eng.needs_refine(sl)) ref.push_back(sl);
         if (!ref.empty()) eng.refine_restart(ref);
-        bool all = true;
-        for (int i = 0; i < eng.capacity(); ++i) {
-            const int ph = eng.phase(i);
-            if (ph != PH_EMPTY && ph != PH_DONE) all = false;
-        }
-        if (all) break;
+        if (eng.all_done()) break;
         GCI_REQUIRE(it <= guard_iters, "davidson: iteration guard exceeded");
     }
+    eng.sync();
 }
 
 // One GDVD solve on one block; init: n x niv initial vectors (host) or null
@@ -248,7 +248,6 @@ int gci_batch_solve(int nblocks, gci_hmat **blocks, const int *nev, const double
                 eval_off[b] = eo;
                 if (evec_off) evec_off[b] = vo;
                 for (int q = 0; q < nev[b]; ++q) eval[eo + q] = r[5 + q];
-                if (evec) eng.copy_vectors(slots[b], evec + vo);
                 if (stats) {
                     gci_dvd_stats st{};
                     const bool refined = r[3] != 0.0;
@@ -261,6 +260,7 @@ int gci_batch_solve(int nblocks, gci_hmat **blocks, const int *nev, const double
                 eo += nev[b];
                 vo += blocks[b]->n * nev[b];
             }
+            if (evec) eng.copy_vectors(slots, evec);
             eng.sync();
             const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             if (stats)

@@ -224,21 +224,23 @@ def setham(ncf, nw, lists, iab, kstart, keys, vals, lfordr=False, iccut=0, preci
 def batch_solve(blocks, nev, tol, eav=None):
     """Batched form: all blocks' Davidson iterations interleaved on the
     device.  blocks: list of HamiltonianBlock; nev/tol/eav: per block.
-    Returns (list of eigenvalue arrays, list of eigenvector arrays, stats)."""
+    Returns (list of eigenvalue arrays, list of eigenvector arrays, stats);
+    the eigenvector arrays are n x nev column-major views into one buffer."""
     nb = len(blocks)
     nev = np.ascontiguousarray(np.broadcast_to(np.asarray(nev, dtype=np.int32), (nb,)))
     tol = np.ascontiguousarray(np.broadcast_to(np.asarray(tol, dtype=np.float64), (nb,)))
     eav = np.zeros(nb) if eav is None else np.ascontiguousarray(eav, dtype=np.float64)
     handles = (C.c_void_p * nb)(*[b.handle for b in blocks])
     ev = np.zeros(int(nev.sum()))
-    vec = np.zeros(int(sum(b.n * int(k) for b, k in zip(blocks, nev))))
+    vec = np.empty(int(sum(b.n * int(k) for b, k in zip(blocks, nev))))
     eo = np.zeros(nb, dtype=np.int64)
     vo = np.zeros(nb, dtype=np.int64)
     st = (L.GciDvdStats * nb)()
     check(L.lib().gci_batch_solve(C.c_int(nb), handles, i32p(nev), dp(tol), dp(eav), dp(ev), i64p(eo), dp(vec),
                                   i64p(vo), st))
     evals = [ev[eo[b]: eo[b] + nev[b]].copy() for b in range(nb)]
-    evecs = [vec[vo[b]: vo[b] + blocks[b].n * nev[b]].reshape(nev[b], blocks[b].n).T.copy() for b in range(nb)]
+    # n x nev column-major views into the one returned buffer (no copy)
+    evecs = [vec[vo[b]: vo[b] + blocks[b].n * nev[b]].reshape(nev[b], blocks[b].n).T for b in range(nb)]
     stats = [{"nloops": s.nloops, "nmv": s.nmv, "ierr": s.ierr, "refine_loops": s.refine_loops,
               "seconds": s.seconds} for s in st]
     return evals, evecs, stats
