@@ -59,9 +59,9 @@ struct DvdProb {
     int init_ist, init_iend;        // INITDVD progress: next AB column to copy, columns built
     int init_src;                   // AB column copied by the current INITDVD sub-step
     // --- small arrays (1-based reference arrays stored 0-based)
-    double eigval[MAXLIM];
+    double eigval[MAXLIM];          // all KPASS Ritz values, ascending
     double oldval[MAXLIM + 1];
-    double svec[MAXLIM * MAXLIM];   // kpass x ifind, ld = kpass
+    double svec[MAXLIM * MAXLIM];   // kpass x kpass Ritz coefficients, ld = kpass
     double s[MAXLIM * (MAXLIM + 1) / 2];
     double resn[MAXLIM];            // SCRA1(1:NUME) residual norms
     double epsil[MAXLIM];           // SCRA1(LIM+1:) eigenvalue shifts (zero)

@@ -57,47 +57,87 @@ __device__ inline double block_sum(double v, double *sh) {
     return r;  // valid on thread 0
 }
 
+// out[o] = sum over the problem's chunks of partial[chunk][o], o < nout.
+// The block is split into G groups of nout threads (G = blockDim / nout,
+// at most the chunk count); group g sums chunks g, g + G, ... and the group
+// sums are added in group order, so every output is summed in one fixed
+// order.  Called by every thread of the block; ends with a barrier.
+__device__ void reduce_partials(const DvdProb &p, const double *partial, int nout, double *out) {
+    __shared__ double red[256];
+    const int nt = blockDim.x, t = threadIdx.x;
+    const int G = max(1, min(min(nt, 256) / max(nout, 1), p.nchunks));
+    if (G == 1) {
+        for (int o = t; o < nout; o += nt) {
+            double s = 0.0;
+            for (int c = 0; c < p.nchunks; ++c) s += partial[(p.chunk0 + c) * PSTRIDE + o];
+            out[o] = s;
+        }
+        __syncthreads();
+        return;
+    }
+    const int o = t % nout, g = t / nout;
+    if (g < G) {
+        double s = 0.0;
+        for (int c = g; c < p.nchunks; c += G) s += partial[(p.chunk0 + c) * PSTRIDE + o];
+        red[g * nout + o] = s;
+    }
+    __syncthreads();
+    if (t < nout) {
+        double s = 0.0;
+        for (int q = 0; q < G; ++q) s += red[q * nout + t];
+        out[t] = s;
+    }
+    __syncthreads();
+}
+
 // ------------------------------------------------------------ Jacobi RR
 
 __device__ inline int rr_pos(int j, int r, int np) { return j == 0 ? 0 : ((j - 1 + r) % (np - 1)) + 1; }
 
-__global__ __launch_bounds__(256) void k_rr(DvdProb *P) {
+// Rayleigh-Ritz on the packed S (k = KPASS): all k eigenpairs by cyclic
+// parallel Jacobi, eigenvalues ascending into EIGVAL and eigenvectors into
+// SVEC (k x k, ld k).  A round applies one rotation per index pair; a
+// thread owns a 2 x 2 block of A (and its mirror) or one row of a V column
+// pair, so a round is one pass over LDS with two barriers.
+__global__ __launch_bounds__(512) void k_rr(DvdProb *P) {
     DvdProb &p = P[blockIdx.x];
     if (p.phase != PH_ITER) return;
     __shared__ double A[MAXLIM][MAXLIM + 1];
     __shared__ double V[MAXLIM][MAXLIM + 1];
-    __shared__ double cs[MAXLIM / 2], sn[MAXLIM / 2];
+    __shared__ double cs[MAXLIM / 2], sn[MAXLIM / 2], tn[MAXLIM / 2];
     __shared__ int pa[MAXLIM / 2], pb[MAXLIM / 2];
     __shared__ int any_rot;
     __shared__ int rank_of[MAXLIM];
     const int t = threadIdx.x, nt = blockDim.x;
     const int k = p.kpass;
     const int ifind = min(k, p.rest);
+    const int np = k + (k & 1);  // an odd order gets a zero row and column that never rotates
+    const int npairs = np / 2;
     // DCOPY(IFIND, EIGVAL, OLDVAL) (dvdson.f90:722)
     for (int i = t; i < ifind; i += nt) p.oldval[i] = p.eigval[i];
-    for (int idx = t; idx < k * k; idx += nt) {
-        const int i = idx % k, j = idx / k;
-        const int lo = min(i, j), hi = max(i, j);
-        A[i][j] = p.s[lo + hi * (hi + 1) / 2];
-        V[i][j] = (i == j) ? 1.0 : 0.0;
+    for (int idx = t; idx < np * np; idx += nt) {
+        const int i = idx % np, j = idx / np;
+        double a = 0.0;
+        if (i < k && j < k) {
+            const int lo = min(i, j), hi = max(i, j);
+            a = p.s[lo + hi * (hi + 1) / 2];
+        }
+        A[i][j] = a;
+        V[i][j] = (i == j && i < k) ? 1.0 : 0.0;
     }
-    __syncthreads();
-    const int np = k + (k & 1);
-    const int npairs = np / 2;
     for (int sweep = 0; sweep < 40; ++sweep) {
-        __syncthreads();
         if (t == 0) any_rot = 0;
         __syncthreads();
         for (int r = 0; r < np - 1; ++r) {
             if (t < npairs) {
                 int a = rr_pos(t, r, np), b = rr_pos(np - 1 - t, r, np);
                 if (a > b) { int tmp = a; a = b; b = tmp; }
-                double c = 1.0, s = 0.0;
+                double c = 1.0, s = 0.0, tt = 0.0;
                 if (b < k) {
                     const double app = A[a][a], aqq = A[b][b], apq = A[a][b];
                     if (apq != 0.0 && fabs(apq) > 2.2e-17 * sqrt(fabs(app * aqq))) {
                         const double tau = (aqq - app) / (2.0 * apq);
-                        const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                         c = 1.0 / sqrt(1.0 + tt * tt);
                         s = tt * c;
                         any_rot = 1;
@@ -107,39 +147,50 @@ __global__ __launch_bounds__(256) void k_rr(DvdProb *P) {
                 pb[t] = b;
                 cs[t] = c;
                 sn[t] = s;
+                tn[t] = tt;
             }
             __syncthreads();
-            // columns of A and V
+            // A <- J^T A J by 2 x 2 blocks of index pairs (q <= u)
+            for (int idx = t; idx < npairs * npairs; idx += nt) {
+                const int q = idx / npairs, u = idx % npairs;
+                if (q > u) continue;
+                const double sq = sn[q], su = sn[u];
+                if (sq == 0.0 && su == 0.0) continue;
+                const int aq = pa[q], bq = pb[q];
+                if (q == u) {
+                    const double apq = A[aq][bq], tt = tn[q];
+                    A[aq][aq] -= tt * apq;
+                    A[bq][bq] += tt * apq;
+                    A[aq][bq] = 0.0;
+                    A[bq][aq] = 0.0;
+                    continue;
+                }
+                const int au = pa[u], bu = pb[u];
+                const double cq = cs[q], cu = cs[u];
+                const double x00 = A[aq][au], x01 = A[aq][bu], x10 = A[bq][au], x11 = A[bq][bu];
+                const double u00 = cq * x00 - sq * x10, u01 = cq * x01 - sq * x11;
+                const double u10 = sq * x00 + cq * x10, u11 = sq * x01 + cq * x11;
+                const double y00 = cu * u00 - su * u01, y01 = su * u00 + cu * u01;
+                const double y10 = cu * u10 - su * u11, y11 = su * u10 + cu * u11;
+                A[aq][au] = y00;
+                A[aq][bu] = y01;
+                A[bq][au] = y10;
+                A[bq][bu] = y11;
+                A[au][aq] = y00;
+                A[bu][aq] = y01;
+                A[au][bq] = y10;
+                A[bu][bq] = y11;
+            }
+            // V <- V J
             for (int idx = t; idx < npairs * k; idx += nt) {
                 const int q = idx / k, i = idx % k;
+                const double s = sn[q];
+                if (s == 0.0) continue;
                 const int a = pa[q], b = pb[q];
-                if (b >= k || sn[q] == 0.0) continue;
-                const double c = cs[q], s = sn[q];
-                const double x = A[i][a], y = A[i][b];
-                A[i][a] = c * x - s * y;
-                A[i][b] = s * x + c * y;
+                const double c = cs[q];
                 const double vx = V[i][a], vy = V[i][b];
                 V[i][a] = c * vx - s * vy;
                 V[i][b] = s * vx + c * vy;
-            }
-            __syncthreads();
-            // rows of A
-            for (int idx = t; idx < npairs * k; idx += nt) {
-                const int q = idx / k, i = idx % k;
-                const int a = pa[q], b = pb[q];
-                if (b >= k || sn[q] == 0.0) continue;
-                const double c = cs[q], s = sn[q];
-                const double x = A[a][i], y = A[b][i];
-                A[a][i] = c * x - s * y;
-                A[b][i] = s * x + c * y;
-            }
-            __syncthreads();
-            if (t < npairs) {
-                const int a = pa[t], b = pb[t];
-                if (b < k && sn[t] != 0.0) {
-                    A[a][b] = 0.0;
-                    A[b][a] = 0.0;
-                }
             }
             __syncthreads();
         }
@@ -161,10 +212,8 @@ __global__ __launch_bounds__(256) void k_rr(DvdProb *P) {
     for (int idx = t; idx < k * k; idx += nt) {
         const int i = idx / k, r = idx % k;  // eigen index i, component r
         const int rk = rank_of[i];
-        if (rk < ifind) {
-            p.svec[rk * k + r] = V[r][i];
-            if (r == 0) p.eigval[rk] = A[i][i];
-        }
+        p.svec[rk * k + r] = V[r][i];
+        if (r == 0) p.eigval[rk] = A[i][i];
     }
     __syncthreads();
     if (t != 0) return;
@@ -315,17 +364,17 @@ __global__ __launch_bounds__(256) void k_newvec(DvdProb *P, const int *chunk_pro
     }
 }
 
-__global__ void k_newvec_decide(DvdProb *P, int nprob, const double *partial) {
-    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pi >= nprob) return;
-    DvdProb &p = P[pi];
+__global__ void k_newvec_decide(DvdProb *P, const double *partial) {
+    DvdProb &p = P[blockIdx.x];
     if (p.phase != PH_ITER) return;
+    __shared__ double nrm2[MAXNUME];
     const int nc = p.ncand, k = p.kpass;
+    reduce_partials(p, partial, nc, nrm2);
+    if (threadIdx.x != 0) return;
     const int limadd = min(p.lim, p.mblock + k);
     int nadded = 0, icvc = 0;
     for (int t = 0; t < nc; ++t) {
-        double s = 0.0;
-        for (int c = 0; c < p.nchunks; ++c) s += partial[(p.chunk0 + c) * PSTRIDE + t];
+        const double s = nrm2[t];
         const int indx = p.incv[t];
         const double res = sqrt(s);
         p.resn[indx - 1] = res;
@@ -492,13 +541,6 @@ __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob,
 }
 
 // Sum chunk partials of problem blockIdx.x into out (fixed chunk order).
-__device__ void reduce_partials(const DvdProb &p, const double *partial, int nout, double *out) {
-    for (int o = threadIdx.x; o < nout; o += blockDim.x) {
-        double s = 0.0;
-        for (int c = 0; c < p.nchunks; ++c) s += partial[(p.chunk0 + c) * PSTRIDE + o];
-        out[o] = s;
-    }
-}
 
 // projection coefficients for the classical Gram-Schmidt pass
 __global__ void k_coef(DvdProb *P, const double *partial, int want_phase) {
@@ -544,11 +586,10 @@ __global__ void k_chol(DvdProb *P, const double *partial, int want_phase, int pa
     DvdProb &p = P[blockIdx.x];
     if (p.phase != want_phase) return;
     __shared__ double G[MAXNUME * MAXNUME];
+    __shared__ double R[MAXNUME][MAXNUME + 1];
     const int nn = p.nnew;
     reduce_partials(p, partial, nn * nn, G);
-    __syncthreads();
     if (threadIdx.x != 0) return;
-    double R[MAXNUME][MAXNUME];
     int brk = 0;
     for (int i = 0; i < nn; ++i)
         for (int j = 0; j < nn; ++j) R[i][j] = 0.0;
@@ -567,23 +608,25 @@ __global__ void k_chol(DvdProb *P, const double *partial, int want_phase, int pa
             R[j][i] = v / R[j][j];
         }
     }
-    // invert upper triangular R
+    // invert upper triangular R (into G, no longer needed)
+    double *Ri = G;
     for (int j = 0; j < nn; ++j) {
-        for (int i = 0; i < nn; ++i) p.rinv[i * nn + j] = 0.0;
-        p.rinv[j * nn + j] = 1.0 / R[j][j];
+        for (int i = 0; i < nn; ++i) Ri[i * nn + j] = 0.0;
+        Ri[j * nn + j] = 1.0 / R[j][j];
         for (int i = j - 1; i >= 0; --i) {
             double s = 0.0;
-            for (int q = i + 1; q <= j; ++q) s += R[i][q] * p.rinv[q * nn + j];
-            p.rinv[i * nn + j] = -s / R[i][i];
+            for (int q = i + 1; q <= j; ++q) s += R[i][q] * Ri[q * nn + j];
+            Ri[i * nn + j] = -s / R[i][i];
         }
     }
     for (int j = 0; j < nn; ++j) {
         if (!(brk & (1 << j))) continue;
         for (int i = 0; i < nn; ++i) {
-            p.rinv[i * nn + j] = 0.0;
-            p.rinv[j * nn + i] = 0.0;
+            Ri[i * nn + j] = 0.0;
+            Ri[j * nn + i] = 0.0;
         }
     }
+    for (int i = 0; i < nn * nn; ++i) p.rinv[i] = Ri[i];
     if (pass == 0) p.brk_mask = brk;
     else if (brk) p.ierr |= 1 << 20;  // orthogonalisation failed twice
 }
@@ -1361,11 +1404,11 @@ void DavidsonEngine::launch_iteration() {
     const int G = (int)total_chunks_;
     const int *cp = chunk_prob_.p;
     const int64_t *cr = chunk_row_.p;
-    launch(k_rr, cap_, 256, stream_, dprob_.p);
+    launch(k_rr, cap_, 512, stream_, dprob_.p);
     launch(k_rotate, G, 256, stream_, dprob_.p, cp, cr, 0);
     launch(k_ovf_fix, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_);
     launch(k_newvec, G, 256, stream_, dprob_.p, cp, cr, partial_.p);
-    launch(k_newvec_decide, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, (const double *)partial_.p);
+    launch(k_newvec_decide, cap_, 256, stream_, dprob_.p, (const double *)partial_.p);
     launch(k_precond, G, 256, stream_, dprob_.p, cp, cr);
     launch_orth(2, PH_ITER);
     launch_op(PH_ITER, mwidth_iter_);
