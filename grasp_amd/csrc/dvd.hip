@@ -217,11 +217,18 @@ __global__ __launch_bounds__(512) void k_rr(DvdProb *P) {
     }
     __syncthreads();
     if (t != 0) return;
-    // TSTSEL (dvdson.f90:1328-1411); ISELEC = 1..NUME
+    // TSTSEL (dvdson.f90:1328-1411); ISELEC = 1..NUME.  A CRITE below the
+    // rounding unit of the eigenvalue (MANEIG's 1e-17) can only be met by a
+    // bitwise-unchanged Ritz value; the Jacobi sweeps reproduce the previous
+    // values exactly when the new columns couple below that unit, where
+    // DSPEVX's tridiagonal rounding moves them, so such a CRITE is not
+    // taken as met.
     const int nume = p.nume;
     int nnce = 0;
-    for (int i = 0; i < nume; ++i)
-        if (!(fabs(p.oldval[i] - p.eigval[i]) < p.crite)) ++nnce;
+    for (int i = 0; i < nume; ++i) {
+        const bool resolvable = p.crite >= 2.220446049250313e-16 * fabs(p.eigval[i]);
+        if (!(resolvable && fabs(p.oldval[i] - p.eigval[i]) < p.crite)) ++nnce;
+    }
     if (nnce == 0) {
         p.phase = PH_FINAL;
         return;

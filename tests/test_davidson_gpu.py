@@ -72,3 +72,22 @@ def test_gdvd_split_with_refinement(gpu):
     want, *_ = O.maneig(csr.row_end, csr.col, csr.val, eav, nvec, nthreads=8)
     assert np.all(_rel(ev, want, eav) < 1e-10)
     assert st["refine_loops"] >= 1
+
+
+def test_maneig_new_column_below_rounding_unit(gpu):
+    # A block whose INIEST2 start (eigenvectors of the leading 2000 rows)
+    # gets, at the second iteration, a new basis column that couples to the
+    # wanted Ritz vectors only at 1e-10: the lowest Ritz values do not move
+    # by one rounding unit, which must not count as CRITE = 1e-17 being met
+    # (the residuals are still 1e-4 there).
+    from grasp_amd import synth
+    n, fill, seed = synth.batched_workload(64, 2024)[55]
+    h, eav = synth.synth_block(n, seed, fill=fill, precision=L.GCI_FP64)
+    try:
+        ev, evec, st = maneig(h, 4)
+        r = h.spicmv2(np.asfortranarray(evec)) - evec * ev
+        rel = np.linalg.norm(r, axis=0) / np.abs(ev + eav)
+        assert st["nloops"] > 2 and st["ierr"] == 0
+        assert np.all(rel < 1e-8), rel
+    finally:
+        h.free()
