@@ -62,6 +62,9 @@ def precision_form(args, L, rci, synth, sync):
     checked against the plain FP64 form (MANEIG: INIEST2 + GDVD with the
     reference's criteria) on the same device-resident block."""
     nev = 8
+    # 8-vector launches: 2048-row panels keep the 8 x-vectors of a panel and
+    # 32 slices' tables in LDS (4096-row panels would split them in two)
+    rci.set_spmm_shape(R=2048, nslice=32, depth=8)
     t0 = time.time()
     b, eav = synth.synth_block(args.prec_n, args.seed, fill=args.prec_fill)
     sync()
@@ -82,10 +85,18 @@ def precision_form(args, L, rci, synth, sync):
     L.check(L.lib().gci_profile_enable(0))
     launches, ms, byts = stats.reshape(3, 3).T
     log(f"mixed form: {t_mixed:.3f}s, loops {st[0]['nloops']}, refinement loops {st[0]['refine_loops']}")
+    # the plain form on the same block held in FP64 (its own lower-triangle
+    # layout, built before timing), after the split copy is freed
+    b.free()
+    pb, eav64 = synth.synth_block(args.prec_n, args.seed, fill=args.prec_fill, precision=L.GCI_FP64)
+    assert abs(eav64 - eav) <= 1e-12 * abs(eav)
+    rci.spmm_layout(pb)  # builds the layout
+    sync()
     t0 = time.perf_counter()
-    ev64, _, st64 = rci.maneig(b, nev)
+    ev64, _, st64 = rci.maneig(pb, nev)
     sync()
     t_plain = time.perf_counter() - t0
+    pb.free()
     log(f"plain FP64: {t_plain:.3f}s, loops {st64['nloops']}, products {st64['nmv']}")
     tot = ev64 + eav
     rel = np.abs(evals[0] - ev64) / np.abs(tot)
@@ -316,6 +327,8 @@ def main():
         for (n, f, s), e in zip(spec, eavs):
             b, e2 = synth.synth_block(n, s, fill=f, precision=L.GCI_FP64)
             assert abs(e2 - e) <= 1e-12 * abs(e)
+            if b.n > 2000:
+                rci.spmm_layout(b)  # the Davidson branch's layout, built before timing
             plain_blocks.append(b)
         sync(0)
         t_plain0 = time.perf_counter()

@@ -2,6 +2,9 @@
 #pragma once
 #include "gci_internal.h"
 
+#include <algorithm>
+#include <cstdlib>
+
 namespace gci {
 
 constexpr int MAXLIM = 80;    // largest LIM (2*nvec+60 for nvec <= 10)
@@ -37,7 +40,8 @@ struct DvdProb {
     const uint16_t *plrow;
     const double *pval;
     const float *phi, *plo;
-    int64_t pnnz;                   // stored off-diagonal entries
+    int64_t pnnz;                   // stored off-diagonal entries (lower triangle)
+    int pfull;                      // the layout holds both triangles
     void *xs;                       // operator staging: n x 8 (fp32 or fp64)
     double *ys;                     // n x 8 fp64
     double *basis, *ab;             // n x lim each (column-major, ld = n)
@@ -73,6 +77,15 @@ struct DvdProb {
     double coef[MAXLIM * MAXNUME];
     double rinv[MAXNUME * MAXNUME];
 };
+
+// Residual target of the split-precision bulk phase for a final target tol.
+inline double bulk_tolerance(double tol) {
+    static const double floor_ = [] {
+        const char *e = std::getenv("GCI_BULK_FLOOR");
+        return e ? std::atof(e) : 2.0e-10;
+    }();
+    return std::max(0.5 * tol, floor_);
+}
 
 struct DvdSpec {
     const HMat *h = nullptr;

@@ -771,6 +771,7 @@ __global__ void k_make_jobs(DvdProb *P, int nprob, SpmmJob *jobs, int want_phase
         j.ys = p.ys;
         j.m = op_cols(p, c0, cb);
         j.mode = op_mode(p);
+        j.full = p.pfull;
         if (alg_bytes) atomicAdd(&alg_bytes[j.mode], op_alg_bytes(p, j.m, j.mode));
     }
     jobs[i] = j;
@@ -1188,6 +1189,14 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
     std::vector<int> slots, unit_init;
     int64_t maxneed = 0;
     GCI_REQUIRE(specs.size() <= (size_t)cap_, "davidson: more problems than slots");
+    // the FP64 operator on a split matrix needs its layout rebuilt with the
+    // FP64 values, which only an idle engine may do (running problems hold
+    // the old layout's addresses)
+    for (const DvdSpec &sp : specs) {
+        if (!sp.h || sp.mode != GCI_OP_FP64 || sp.h->precision != GCI_SPLIT32 || sp.h->layout_fp64) continue;
+        GCI_REQUIRE(active() == 0, "davidson: FP64 operator on a split matrix while other problems run");
+        layout_fp64_values(*const_cast<HMat *>(sp.h));
+    }
     for (size_t q = 0; q < specs.size(); ++q) {
         const DvdSpec &sp = specs[q];
         GCI_REQUIRE(sp.h && sp.h->n >= 2, "davidson: matrix order < 2");
@@ -1258,6 +1267,7 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         p.critr_final = sp.critr_final > 0 ? sp.critr_final : sp.critr;
         p.eshift = sp.eshift;
         p.pnnz = sp.h->nnz;
+        p.pfull = sp.h->pc.full ? 1 : 0;
         p.slot = slot;
         p.phase = PH_INIT;
         p.niv = niv[q];

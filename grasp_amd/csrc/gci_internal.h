@@ -126,13 +126,13 @@ struct SpmmItem {
 
 struct SpmmJob;
 
-// Launch shape of the SpMM: panel height R (1024 or 2048; the layout is
+// Launch shape of the SpMM: panel height R (1024, 2048 or 4096; the layout is
 // built for it), workgroup size, entries per work item, slices per chunk
 // (the most whose tables a launch stages in LDS at once; fewer when the
 // launch width leaves less LDS), and prefetch depth (steps each wavefront
 // keeps in flight).
 struct SpmmShape {
-    int R = 1024;
+    int R = 4096;
     int block = 512;
     int64_t target = 131072;
     int nslice = 32;
@@ -141,7 +141,8 @@ struct SpmmShape {
 
 struct Pcsc {
     int R = 0;                  // rows per panel (0 = not built)
-    int64_t nseg = 0, nnz = 0;  // segments, stored (unpadded) entries
+    bool full = false;          // both triangles laid out (split-precision matrices)
+    int64_t nseg = 0, nnz = 0;  // segments, laid-out (unpadded) entries
     int64_t nslice = 0, nstep = 0;
     int64_t target = 0;         // entries per work item the items were cut for
     DBuf<int32_t> scol;         // nslice x 64 columns (-1: empty lane)
@@ -177,6 +178,7 @@ struct HMat {
     DBuf<float> hi, lo;       // nnz (split form)
     DBuf<double> diag;        // n (shifted)
     Pcsc pc;
+    bool layout_fp64 = false; // a split matrix's layout also carries the FP64 values
     hipStream_t stream = nullptr;
     size_t device_bytes() const {
         return rp.bytes() + col.bytes() + val.bytes() + hi.bytes() + lo.bytes() + diag.bytes() + pc.bytes();
@@ -195,7 +197,8 @@ struct SpmmJob {
     const void *xs;
     double *ys;
     int64_t n;
-    int m, mode, active, pad;
+    int m, mode, active;
+    int full;  // 1: both triangles laid out (column sums only), 0: the lower one
 };
 
 // The launch shape in force (defaults, overridable once through the
@@ -209,6 +212,9 @@ void ensure_pcsc(HMat &h);
 // Lays the matrix out again if it was built for another panel height,
 // ensure_pcsc, then recut_items.
 void ensure_items(HMat &h);
+// A split matrix's layout carries hi/lo only; before an FP64-operator use it
+// is rebuilt once with the FP64 values too.
+void layout_fp64_values(HMat &h);
 // Re-cuts the work items if the shape's entries per item changed since they
 // were cut (drops the device copy of the items); the panel height must match.
 void recut_items(HMat &h);

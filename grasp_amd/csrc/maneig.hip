@@ -229,7 +229,7 @@ int gci_batch_solve(int nblocks, gci_hmat **blocks, const int *nev, const double
             const bool split = blocks[b]->precision == GCI_SPLIT32;
             sp.mode = split ? GCI_OP_SPLIT_BULK : GCI_OP_FP64;
             sp.refine = split;
-            sp.tol_rel = split ? std::max(0.5 * tol[b], 2.0e-10) : tol[b];
+            sp.tol_rel = split ? bulk_tolerance(tol[b]) : tol[b];
             sp.tol_final = tol[b];
             sp.critr = 0.0;
             sp.eshift = eav ? eav[b] : 0.0;

@@ -78,7 +78,7 @@ DvdSpec spec_for(const Request &r) {
     const bool split = r.h->precision == GCI_SPLIT32;
     sp.mode = split ? GCI_OP_SPLIT_BULK : GCI_OP_FP64;
     sp.refine = split;
-    sp.tol_rel = split ? std::max(0.5 * r.tol, 2.0e-10) : r.tol;
+    sp.tol_rel = split ? bulk_tolerance(r.tol) : r.tol;
     sp.tol_final = r.tol;
     sp.eshift = r.eav;
     return sp;

@@ -31,6 +31,9 @@
 // column partials sum_i H_ij x_i in registers; per entry it reads x_i from
 // LDS and adds H_ij x_j into y_i with an LDS atomic.  At a slice end the
 // column partials leave as global atomics, at the item end the panel's rows.
+// A full layout (split-precision matrices) holds every entry also as row j
+// of column i, so the column partials alone are y: no LDS atomics, which
+// bound the lower-triangle walk, at twice the stream.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -60,16 +63,23 @@ typedef unsigned u2v __attribute__((ext_vector_type(2)));
 __host__ __device__ inline uint32_t row_code(uint32_t lr) { return ((lr & 31u) << 11) | (lr >> 5); }
 __host__ __device__ inline uint32_t code_row(uint32_t c) { return ((c & 0x7FFu) << 5) | (c >> 11); }
 
-__global__ void k_keys(int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int R,
-                       uint64_t *__restrict__ keys, int64_t *__restrict__ idx) {
+// Sort keys of the laid-out entries: entry e = (i, j) of the lower triangle
+// as row i of column j; with full, also as row j of column i (key nnz + e).
+__global__ void k_keys(int64_t n, int64_t nnz, const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                       int R, int full, uint64_t *__restrict__ keys, uint32_t *__restrict__ idx) {
     // one wave per row
     const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
     const uint64_t panel = (uint64_t)(row / R), lr = (uint64_t)(row % R);
     for (int64_t e = rp[row] + lane; e < rp[row + 1]; e += 64) {
-        keys[e] = (panel << 40) | ((uint64_t)col[e] << 16) | row_code((uint32_t)lr);
-        idx[e] = e;
+        const uint64_t c = (uint64_t)col[e];
+        keys[e] = (panel << 40) | (c << 16) | row_code((uint32_t)lr);
+        idx[e] = (uint32_t)e;
+        if (full) {
+            keys[nnz + e] = ((c / R) << 40) | ((uint64_t)row << 16) | row_code((uint32_t)(c % R));
+            idx[nnz + e] = (uint32_t)e;
+        }
     }
 }
 
@@ -307,7 +317,7 @@ __global__ __launch_bounds__(64) void k_sched(const int64_t *__restrict__ slane_
 __global__ void k_sell_fill(int64_t nnz, const uint64_t *__restrict__ keys, const int32_t *__restrict__ flag,
                             const int64_t *__restrict__ segid, const int64_t *__restrict__ seg_slice,
                             const int32_t *__restrict__ seg_lane, const int32_t *__restrict__ qpos,
-                            const int64_t *__restrict__ sstep, const int64_t *__restrict__ perm,
+                            const int64_t *__restrict__ sstep, const uint32_t *__restrict__ perm,
                             const double *__restrict__ val, const float *__restrict__ hi,
                             const float *__restrict__ lo, uint16_t *__restrict__ lrow, double *__restrict__ pval,
                             float *__restrict__ phi, float *__restrict__ plo) {
@@ -316,7 +326,7 @@ __global__ void k_sell_fill(int64_t nnz, const uint64_t *__restrict__ keys, cons
         const int q = qpos[e], lane = seg_lane[sg], t = q & 3;
         const int64_t step = sstep[seg_slice[sg]] + q / 4;
         lrow[slot_pos(step, lane, t)] = (uint16_t)(code_row((uint32_t)(keys[e] & 0xFFFFull)) * 8u);
-        const int64_t src = perm[e];
+        const int64_t src = (int64_t)perm[e];
         if (pval) pval[val_pos(step, lane, t)] = val[src];
         if (phi) {
             phi[slot_pos(step, lane, t)] = hi[src];
@@ -447,7 +457,8 @@ __device__ inline void load_raw(const GAS T *p, T (&o)[M]) {
     }
 }
 
-// LDS carve-up of one workgroup (bytes): ys (M x R fp64, vector-major) at 0,
+// LDS carve-up of one workgroup (bytes): ys (M x R fp64, vector-major) at 0
+// when the launch may meet lower-triangle layouts (FP64 operator),
 // then xs (M x R of T), and the tables of the item's (at most NS) slices:
 // cs (column partial sums, NS x 64 rows of M fp64), xj (the x_j rows,
 // NS x 64 rows of M T), cl (columns, NS x 64 ints) and ss (first steps,
@@ -456,9 +467,9 @@ struct SellLds {
     size_t xs, cs, xj, cl, ss, total;
 };
 template <int M, class T>
-__host__ __device__ constexpr SellLds sell_lds(int R, int NS) {
+__host__ __device__ constexpr SellLds sell_lds(int R, int NS, bool rows) {
     SellLds m{};
-    m.xs = (size_t)R * M * sizeof(double);
+    m.xs = rows ? (size_t)R * M * sizeof(double) : 0;  // ys only for lower-triangle layouts
     m.cs = m.xs + (size_t)R * M * sizeof(T);
     m.xj = m.cs + (size_t)NS * SW * M * sizeof(double);
     m.cl = m.xj + (size_t)NS * SW * M * sizeof(T);
@@ -470,7 +481,7 @@ __host__ __device__ constexpr SellLds sell_lds(int R, int NS) {
 // One lane's quad: its four entries (row codes off = lr * 8, values a) applied
 // to the M vectors: the column partials acc[v] += a x_i[v] (x_i from LDS) and
 // y_i[v] += a x_j[v] through LDS atomics.  All row reads are issued first.
-template <int M, int MODE, int R>
+template <int M, int MODE, int R, bool SCATTER>
 __device__ inline void consume(const Quad<MODE> &q, const char *xs, char *ys, const double (&xj)[M],
                                double (&acc)[M]) {
     using T = typename XT<MODE>::T;
@@ -487,10 +498,12 @@ __device__ inline void consume(const Quad<MODE> &q, const char *xs, char *ys, co
         const double a = q.value(j);
 #pragma unroll
         for (int v = 0; v < M; ++v) acc[v] = fma(a, (double)xv[j][v], acc[v]);
+        if constexpr (SCATTER) {
 #pragma unroll
-        for (int v = 0; v < M; ++v)
-            __hip_atomic_fetch_add(reinterpret_cast<double *>(ys + off[j] + (size_t)v * R * 8), a * xj[v],
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            for (int v = 0; v < M; ++v)
+                __hip_atomic_fetch_add(reinterpret_cast<double *>(ys + off[j] + (size_t)v * R * 8), a * xj[v],
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
     }
 }
 
@@ -515,7 +528,10 @@ __global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items,
     if (!jb.active || jb.mode != MODE) return;
     const int rows = it.rows;
     if (rows > R) return;  // never reached: items are cut for the shape
-    const SellLds lm = sell_lds<M, T>(R, NS);
+    // split-precision matrices are always laid out full; FP64 ones may be
+    // either (the launch's LDS then holds the row accumulators)
+    constexpr bool ROWS = MODE == GCI_OP_FP64;
+    const SellLds lm = sell_lds<M, T>(R, NS, ROWS);
     double *ys = reinterpret_cast<double *>(smem);
     T *xs = reinterpret_cast<T *>(smem + lm.xs);
     double *cs = reinterpret_cast<double *>(smem + lm.cs);
@@ -525,6 +541,7 @@ __global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items,
     const GAS T *gx = (const GAS T *)jb.xs;
     GAS double *gy = (GAS double *)jb.ys;
     const int t = threadIdx.x;
+    const bool full = !ROWS || jb.full != 0;
     // wavefront-uniform values are read through readfirstlane, so the walk's
     // control flow is scalar
     const int w = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
@@ -535,7 +552,7 @@ __global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items,
 #pragma unroll
         for (int v = 0; v < M; ++v) {
             xs[v * R + rr] = xr[v];
-            ys[v * R + rr] = 0.0;
+            if (!full) ys[v * R + rr] = 0.0;
         }
     }
     // the item's slices in chunks of at most NS, whose tables fill the LDS
@@ -613,8 +630,14 @@ __global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items,
                 nxt.load(P, (uint32_t)((k - a + D) * SW + lane) * 4u);
                 cur.settle();
                 if (k < b) {
-                    consume<M, MODE, R>(cur, reinterpret_cast<const char *>(xs), reinterpret_cast<char *>(ys), xj,
-                                        acc);
+                    // a full layout's entries appear in both orientations:
+                    // column sums alone give y
+                    if (full)
+                        consume<M, MODE, R, false>(cur, reinterpret_cast<const char *>(xs),
+                                                   reinterpret_cast<char *>(ys), xj, acc);
+                    else
+                        consume<M, MODE, R, true>(cur, reinterpret_cast<const char *>(xs),
+                                                  reinterpret_cast<char *>(ys), xj, acc);
                     if (k + 1 == snext && k + 1 < b) {
                         leave();
                         ++si;
@@ -640,6 +663,7 @@ __global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items,
         __syncthreads();  // the next chunk overwrites the tables
     }
     // the panel's rows, as contiguous runs of the row-major staging buffer
+    if (full) return;
     for (int idx = t; idx < rows * M; idx += BS) {
         const int rr = idx / M, v = idx % M;
         const double y = ys[v * R + rr];
@@ -683,7 +707,7 @@ void launch_depth(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs
     using T = typename XT<MODE>::T;
     const int ns = chunk_slices(sh, M, MODE);
     GCI_REQUIRE(ns >= 1, "spmm: panel height and launch width exceed the LDS");
-    const size_t shm = sell_lds<M, T>(R, ns).total;
+    const size_t shm = sell_lds<M, T>(R, ns, MODE == GCI_OP_FP64).total;
     static size_t attr = 0;  // one instantiation per template tuple
     if (shm > attr) {
         GCI_HIP(hipFuncSetAttribute((const void *)k_sell<M, MODE, R, BS, D>,
@@ -697,6 +721,7 @@ template <int M, int MODE, int R, int BS>
 void launch_shape(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh,
                   hipStream_t s) {
     if (sh.depth == 2) launch_depth<M, MODE, R, BS, 2>(d_items, nitems, d_jobs, sh, s);
+    else if (sh.depth == 8) launch_depth<M, MODE, R, BS, 8>(d_items, nitems, d_jobs, sh, s);
     else launch_depth<M, MODE, R, BS, 4>(d_items, nitems, d_jobs, sh, s);
 }
 
@@ -717,6 +742,7 @@ void launch_bs(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, c
 template <int M, int MODE>
 void launch_r(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh, hipStream_t s) {
     if (sh.R == 2048) launch_bs<M, MODE, 2048>(d_items, nitems, d_jobs, sh, s);
+    else if (sh.R == 4096) launch_bs<M, MODE, 4096>(d_items, nitems, d_jobs, sh, s);
     else launch_bs<M, MODE, 1024>(d_items, nitems, d_jobs, sh, s);
 }
 
@@ -739,17 +765,17 @@ int env_int(const char *name, int dflt) {
 size_t lds_bytes(int R, int NS, int M, int mode) {
     if (mode == GCI_OP_SPLIT_BULK) {
         switch (M) {
-            case 1: return sell_lds<1, float>(R, NS).total;
-            case 2: return sell_lds<2, float>(R, NS).total;
-            case 4: return sell_lds<4, float>(R, NS).total;
-            default: return sell_lds<8, float>(R, NS).total;
+            case 1: return sell_lds<1, float>(R, NS, false).total;
+            case 2: return sell_lds<2, float>(R, NS, false).total;
+            case 4: return sell_lds<4, float>(R, NS, false).total;
+            default: return sell_lds<8, float>(R, NS, false).total;
         }
     }
     switch (M) {
-        case 1: return sell_lds<1, double>(R, NS).total;
-        case 2: return sell_lds<2, double>(R, NS).total;
-        case 4: return sell_lds<4, double>(R, NS).total;
-        default: return sell_lds<8, double>(R, NS).total;
+        case 1: return sell_lds<1, double>(R, NS, mode == GCI_OP_FP64).total;
+        case 2: return sell_lds<2, double>(R, NS, mode == GCI_OP_FP64).total;
+        case 4: return sell_lds<4, double>(R, NS, mode == GCI_OP_FP64).total;
+        default: return sell_lds<8, double>(R, NS, mode == GCI_OP_FP64).total;
     }
 }
 
@@ -759,6 +785,19 @@ void scan_exclusive(const I *in, O *out, int64_t n, hipStream_t s) {
     GCI_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, s));
     SBuf<char> tmp(tb, s);
     GCI_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, in, out, n, s));
+}
+
+// Sorts (k, v) in place between two buffer pairs (no third copy); returns
+// true when the result is in the second pair.
+template <class K, class V>
+bool sort_pairs_db(K *k0, K *k1, V *v0, V *v1, int64_t n, hipStream_t s) {
+    hipcub::DoubleBuffer<K> kb(k0, k1);
+    hipcub::DoubleBuffer<V> vb(v0, v1);
+    size_t tb = 0;
+    GCI_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kb, vb, n, 0, 8 * (int)sizeof(K), s));
+    SBuf<char> tmp(tb, s);
+    GCI_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, kb, vb, n, 0, 8 * (int)sizeof(K), s));
+    return kb.selector == 1;
 }
 
 template <class K, class V>
@@ -786,9 +825,10 @@ constexpr int NS_MIN = 8;
 }  // namespace
 
 bool valid_shape(const SpmmShape &sh) {
-    // the instantiated launch shapes: R 1024 or 2048, 256/512/1024 threads, depth 2 or 4
-    return (sh.R == 1024 || sh.R == 2048) && (sh.block == 256 || sh.block == 512 || sh.block == 1024) &&
-           sh.target >= 4096 && sh.nslice >= 1 && sh.nslice <= 1024 && (sh.depth == 2 || sh.depth == 4) &&
+    // the instantiated launch shapes: R 1024, 2048 or 4096, 256/512/1024 threads, depth 2, 4 or 8
+    return (sh.R == 1024 || sh.R == 2048 || sh.R == 4096) && (sh.block == 256 || sh.block == 512 || sh.block == 1024) &&
+           sh.target >= 4096 && sh.nslice >= 1 && sh.nslice <= 1024 &&
+           (sh.depth == 2 || sh.depth == 4 || sh.depth == 8) &&
            chunk_slices(sh, 1, GCI_OP_FP64) >= std::min(NS_MIN, sh.nslice);
 }
 
@@ -870,6 +910,16 @@ void recut_items(HMat &h) {
     }
 }
 
+void layout_fp64_values(HMat &h) {
+    if (h.precision != GCI_SPLIT32 || h.layout_fp64) return;
+    GCI_REQUIRE(h.val.p != nullptr, "spmm: FP64 operator on a matrix without FP64 values");
+    h.layout_fp64 = true;
+    if (h.pc.R) {
+        GCI_HIP(hipDeviceSynchronize());  // queued launches may still read the old layout
+        h.pc = Pcsc();
+    }
+}
+
 void ensure_items(HMat &h) {
     if (h.pc.R != 0 && h.pc.R != spmm_shape().R) {
         GCI_HIP(hipDeviceSynchronize());  // queued launches may still read the old layout
@@ -902,7 +952,12 @@ void ensure_pcsc(HMat &h) {
     const SpmmShape sh = spmm_shape();
     const int R = sh.R;
     Pcsc &pc = h.pc;
-    const int64_t nnz = h.nnz, n = h.n;
+    // Split-precision matrices (the batched operators, several vectors per
+    // product) lay out both triangles: every output is then a column sum
+    // of gathered x, with no LDS atomics, at twice the stream bytes.  FP64
+    // matrices (the plain single-vector form) keep the lower triangle.
+    pc.full = h.precision == GCI_SPLIT32;
+    const int64_t nnz = pc.full ? 2 * h.nnz : h.nnz, n = h.n;
     hipStream_t s = h.stream;
     pc.nnz = nnz;
     pc.nseg = pc.nslice = pc.nstep = 0;
@@ -910,16 +965,22 @@ void ensure_pcsc(HMat &h) {
     pc.spanel_h.clear();
     if (nnz > 0) {
         const unsigned eb = grid_for(nnz);
-        // entries sorted by (panel, column, row class, row)
+        // entries sorted by (panel, column, row class, row); 12 bytes per
+        // entry and its double buffer (the largest temporaries of the build)
+        GCI_REQUIRE(h.nnz < ((int64_t)1 << 32), "spmm: more than 2^32 - 1 off-diagonal entries");
         SBuf<uint64_t> k1(nnz, s);
-        SBuf<int64_t> i1(nnz, s);
+        SBuf<uint32_t> i1(nnz, s);
         {
             SBuf<uint64_t> k0(nnz, s);
-            SBuf<int64_t> i0(nnz, s);
+            SBuf<uint32_t> i0(nnz, s);
             const int64_t blocks = (n * 64 + 255) / 256;
-            hipLaunchKernelGGL(k_keys, dim3((unsigned)blocks), dim3(256), 0, s, n, h.rp.p, h.col.p, R, k0.p, i0.p);
+            hipLaunchKernelGGL(k_keys, dim3((unsigned)blocks), dim3(256), 0, s, n, h.nnz, h.rp.p, h.col.p, R,
+                               pc.full ? 1 : 0, k0.p, i0.p);
             GCI_HIP(hipGetLastError());
-            sort_pairs(k0.p, k1.p, i0.p, i1.p, nnz, s);
+            if (!sort_pairs_db(k0.p, k1.p, i0.p, i1.p, nnz, s)) {
+                std::swap(k0.p, k1.p);  // keep the sorted pair, free the other
+                std::swap(i0.p, i1.p);
+            }
         }
         // segments
         SBuf<int32_t> flag(nnz, s);
@@ -1003,7 +1064,7 @@ void ensure_pcsc(HMat &h) {
         pc.nstep = nstep;
         const size_t nst = (size_t)nstep * SE;
         pc.lrow.alloc(nst);
-        if (h.val.p) pc.val.alloc(nst);
+        if (h.val.p && (h.precision != GCI_SPLIT32 || h.layout_fp64)) pc.val.alloc(nst);
         if (h.precision == GCI_SPLIT32) {
             pc.hi.alloc(nst);
             pc.lo.alloc(nst);
@@ -1033,7 +1094,8 @@ void spmm(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode
     if (mode != GCI_OP_FP64)
         GCI_REQUIRE(h.precision == GCI_SPLIT32, "spmm: split operator on an FP64-only matrix");
     else
-        GCI_REQUIRE(h.val.p != nullptr, "spmm: FP64 operator on a matrix without FP64 values");
+        layout_fp64_values(h);
+    GCI_REQUIRE(mode != GCI_OP_FP64 || h.val.p != nullptr, "spmm: FP64 operator on a matrix without FP64 values");
     if (h.n == 0) return;
     ensure_items(h);
     Pcsc &pc = h.pc;
@@ -1052,7 +1114,7 @@ void spmm(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode
         for (int md = 0; md < 3; ++md)
             for (int mc = 1; mc <= 8; ++mc)
                 jt[md * 8 + mc - 1] = SpmmJob{pc.scol.p, pc.sstep.p, pc.lrow.p, pc.val.p, pc.hi.p, pc.lo.p,
-                                              pc.stage_x.p, pc.stage_y.p, h.n, mc, md, 1, 0};
+                                              pc.stage_x.p, pc.stage_y.p, h.n, mc, md, 1, pc.full ? 1 : 0};
         pc.d_jobs.alloc(jt.size());
         GCI_HIP(hipMemcpy(pc.d_jobs.p, jt.data(), jt.size() * sizeof(SpmmJob), hipMemcpyHostToDevice));
     }

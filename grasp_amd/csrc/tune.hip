@@ -82,7 +82,7 @@ struct TuneSet {
             ensure_items(h);
             const Pcsc &pc = h.pc;
             jobs.push_back(SpmmJob{pc.scol.p, pc.sstep.p, pc.lrow.p, pc.val.p, pc.hi.p, pc.lo.p, xs[i].p,
-                                   ys[i].p, h.n, std::min(W, m), mode, 1, 0});
+                                   ys[i].p, h.n, std::min(W, m), mode, 1, pc.full ? 1 : 0});
             for (SpmmItem it : pc.items) {
                 it.job = (int32_t)i;
                 items.push_back(it);
@@ -187,7 +187,7 @@ int gci_spmm_autotune(gci_hmat *const *mats, int nmats, int m, int mode, int rep
         GCI_REQUIRE(fits(cur, m, mode), "gci_spmm_autotune: no shape fits this width");
         set_spmm_shape(cur);
         double tbest = ts.time(reps);
-        const std::vector<int> blocks{256, 512, 1024}, nslices{16, 32, 64, 128}, Rs{1024, 2048}, depths{2, 4};
+        const std::vector<int> blocks{256, 512, 1024}, nslices{8, 16, 32, 64}, Rs{1024, 2048, 4096}, depths{2, 4, 8};
         const std::vector<int64_t> targets{32768, 65536, 131072, 262144};
         auto try_shape = [&](const SpmmShape &cand) {
             if (!fits(cand, m, mode)) return;

@@ -272,12 +272,14 @@ def set_spmm_shape(**kw) -> dict:
 
 def spmm_layout(block) -> dict:
     """The block's panel-slice layout for the shape in force: column
-    segments, slices, stored entries (padding included) and their ratio to
-    the off-diagonal entries."""
+    segments, slices, stored entries (padding included), the laid-out
+    entries (both triangles for a split-precision block, the lower one
+    otherwise) and the padding ratio stored / laid out."""
     ns, nsl, st = C.c_int64(), C.c_int64(), C.c_int64()
     check(L.lib().gci_spmm_layout(block.handle, C.byref(ns), C.byref(nsl), C.byref(st)))
-    return {"segments": ns.value, "slices": nsl.value, "stored": st.value,
-            "overhead": st.value / max(1, block.nnz_offdiag)}
+    laid = block.nnz_offdiag * (2 if block.precision == L.GCI_SPLIT32 else 1)
+    return {"segments": ns.value, "slices": nsl.value, "stored": st.value, "laid_out": laid,
+            "overhead": st.value / max(1, laid)}
 
 
 def spmm_time(blocks, m: int, mode: int = L.GCI_OP_SPLIT_BULK, reps: int = 5) -> float:

@@ -224,11 +224,12 @@ int gci_profile_read(double *stats);
 
 /* --------------------------------------------- SpMM launch shape, autotuner
  * No counterpart in the reference (SPICMV2 has no tunables).  The symmetric
- * SpMM's launch shape: panel height R (1024 or 2048: rows whose x and y stay
- * in LDS; the panel-slice layout is built for it), workgroup size (256, 512,
- * 1024), at most `nslice` slices (64-column groups, whose column tables are
- * staged in LDS) and about `target` entries per work item (a run of one
- * panel's steps), and steps each wavefront keeps in flight `depth` (2, 4).
+ * SpMM's launch shape: panel height R (1024, 2048 or 4096: rows whose x
+ * (and, for lower-triangle layouts, y) stay in LDS; the panel-slice layout
+ * is built for it), workgroup size (256, 512, 1024), at most `nslice` slices
+ * (64-column groups, whose column tables are staged in LDS) and about
+ * `target` entries per work item (a run of one panel's steps), and steps
+ * each wavefront keeps in flight `depth` (2, 4, 8).
  * One shape is in force per process; a matrix laid out for another R is laid
  * out again at its next use outside a Davidson engine.  Set or tune the shape
  * before solves, not while requests are resident (an engine refuses a change
