@@ -62,9 +62,6 @@ def precision_form(args, L, rci, synth, sync):
     checked against the plain FP64 form (MANEIG: INIEST2 + GDVD with the
     reference's criteria) on the same device-resident block."""
     nev = 8
-    # 8-vector launches: 2048-row panels keep the 8 x-vectors of a panel and
-    # 32 slices' tables in LDS (4096-row panels would split them in two)
-    rci.set_spmm_shape(R=2048, nslice=32, depth=8)
     t0 = time.time()
     b, eav = synth.synth_block(args.prec_n, args.seed, fill=args.prec_fill)
     sync()

@@ -472,17 +472,37 @@ __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob,
         d4v acc[2];
         acc[0] = d4v{0.0, 0.0, 0.0, 0.0};
         acc[1] = acc[0];
-        for (int64_t rt = r0; rt < r1; rt += 64) {
+        // the next tile's elements are loaded into registers while the
+        // matrix cores work on the current one (256 threads, 64 rows)
+        constexpr int QA = 64 * MAXLIM / 256, QB = 64 * MAXNUME / 256;
+        double ra[QA], rb[QB];
+        auto fetch = [&](int64_t rt) {
             const int rows = (int)min<int64_t>(64, r1 - rt);
-            for (int idx = threadIdx.x; idx < 64 * ka; idx += blockDim.x) {
-                const int rr = idx & 63, i = idx >> 6;
-                ta[rr][i] = rr < rows ? A[rt + rr + (int64_t)i * p.n] : 0.0;
+#pragma unroll
+            for (int q = 0; q < QA; ++q) {
+                const int idx = threadIdx.x + q * 256, rr = idx & 63, i = idx >> 6;
+                ra[q] = (i < ka && rr < rows) ? A[rt + rr + (int64_t)i * p.n] : 0.0;
             }
-            for (int idx = threadIdx.x; idx < 64 * kb; idx += blockDim.x) {
-                const int rr = idx & 63, j = idx >> 6;
-                tb[rr][j] = rr < rows ? B[rt + rr + (int64_t)j * p.n] : 0.0;
+#pragma unroll
+            for (int q = 0; q < QB; ++q) {
+                const int idx = threadIdx.x + q * 256, rr = idx & 63, j = idx >> 6;
+                rb[q] = (j < kb && rr < rows) ? B[rt + rr + (int64_t)j * p.n] : 0.0;
+            }
+        };
+        fetch(r0);
+        for (int64_t rt = r0; rt < r1; rt += 64) {
+#pragma unroll
+            for (int q = 0; q < QA; ++q) {
+                const int idx = threadIdx.x + q * 256;
+                if ((idx >> 6) < ka) ta[idx & 63][idx >> 6] = ra[q];
+            }
+#pragma unroll
+            for (int q = 0; q < QB; ++q) {
+                const int idx = threadIdx.x + q * 256;
+                if ((idx >> 6) < kb) tb[idx & 63][idx >> 6] = rb[q];
             }
             __syncthreads();
+            if (rt + 64 < r1) fetch(rt + 64);
             // operands: lane l holds A^T[i = l%16][r = l/16] and B[r = l/16][j = l%16]
             const int j = lane & 15;
 #pragma unroll
@@ -752,13 +772,13 @@ __device__ inline double op_alg_bytes(const DvdProb &p, int m, int mode) {
     return nnz * 12.0 + (n + 1) * 8.0 + 2.0 * n * m * 8.0;
 }
 
-__global__ void k_make_jobs(DvdProb *P, int nprob, SpmmJob *jobs, int want_phase, int c0, int cb,
+__global__ void k_make_jobs(DvdProb *P, int nprob, SpmmJob *jobs, int want_phase, int c0, int cb, int mode,
                             double *alg_bytes) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nprob) return;
     const DvdProb &p = P[i];
     SpmmJob j{};
-    j.active = (p.phase == want_phase && op_cols(p, c0, cb) > 0) ? 1 : 0;
+    j.active = (p.phase == want_phase && op_mode(p) == mode && op_cols(p, c0, cb) > 0) ? 1 : 0;
     if (j.active) {
         j.scol = p.pscol;
         j.sstep = p.psstep;
@@ -778,10 +798,10 @@ __global__ void k_make_jobs(DvdProb *P, int nprob, SpmmJob *jobs, int want_phase
 }
 
 __global__ __launch_bounds__(256) void k_pack_slots(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
-                                                    int want_phase, int M, int c0, int cb) {
+                                                    int want_phase, int M, int c0, int cb, int mode) {
     DvdProb &p = P[chunk_prob[blockIdx.x]];
     const int m = op_cols(p, c0, cb);
-    if (p.phase != want_phase || m <= 0) return;
+    if (p.phase != want_phase || op_mode(p) != mode || m <= 0) return;
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
     const double *X = p.basis + (int64_t)(p.kpass + c0) * p.n;
     const bool f32 = op_mode(p) == GCI_OP_SPLIT_BULK;
@@ -796,10 +816,10 @@ __global__ __launch_bounds__(256) void k_pack_slots(DvdProb *P, const int *chunk
 }
 
 __global__ __launch_bounds__(256) void k_unpack_slots(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
-                                                      int want_phase, int M, int c0, int cb) {
+                                                      int want_phase, int M, int c0, int cb, int mode) {
     DvdProb &p = P[chunk_prob[blockIdx.x]];
     const int m = min(op_cols(p, c0, cb), M);
-    if (p.phase != want_phase || m <= 0) return;
+    if (p.phase != want_phase || op_mode(p) != mode || m <= 0) return;
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
     const double *X = p.basis + (int64_t)(p.kpass + c0) * p.n;
     double *Y = p.ab + (int64_t)(p.kpass + c0) * p.n;
@@ -1374,23 +1394,22 @@ void DavidsonEngine::update_modes() {
     shape_version_ = spmm_shape_version();
 }
 
-// maxcols: upper bound of NNEW over the problems in the stage
+// maxcols: upper bound of NNEW over the problems in the stage.  Each
+// operator mode in use gets its own launches, in column blocks as wide as
+// that mode's LDS allows.
 void DavidsonEngine::launch_op(int phase, int maxcols) {
     const int G = (int)total_chunks_;
     const int *cp = chunk_prob_.p;
     const int64_t *cr = chunk_row_.p;
-    // column blocks as wide as every operator mode in use fits in LDS
-    int cb = 8;
-    for (int mode = 0; mode < 3; ++mode)
-        if (modes_used_ & (1 << mode)) cb = std::min(cb, spmm_max_width(mode));
-    for (int c0 = 0; c0 < maxcols; c0 += cb) {
-        const int M = spmm_width(std::min(cb, maxcols - c0));
-        OpProfile &pf = op_profile();
-        launch(k_make_jobs, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, djobs_.p, phase, c0, cb,
-               pf.on ? op_bytes_device() : (double *)nullptr);
-        launch(k_pack_slots, G, 256, stream_, dprob_.p, cp, cr, phase, M, c0, cb);
-        for (int mode = 0; mode < 3; ++mode) {
-            if (!(modes_used_ & (1 << mode))) continue;
+    for (int mode = 0; mode < 3; ++mode) {
+        if (!(modes_used_ & (1 << mode))) continue;
+        const int cb = std::min(8, spmm_max_width(mode));
+        for (int c0 = 0; c0 < maxcols; c0 += cb) {
+            const int M = spmm_width(std::min(cb, maxcols - c0));
+            OpProfile &pf = op_profile();
+            launch(k_make_jobs, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, djobs_.p, phase, c0, cb, mode,
+                   pf.on ? op_bytes_device() : (double *)nullptr);
+            launch(k_pack_slots, G, 256, stream_, dprob_.p, cp, cr, phase, M, c0, cb, mode);
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (pf.on) {
                 GCI_HIP(hipEventCreate(&e0));
@@ -1402,8 +1421,8 @@ void DavidsonEngine::launch_op(int phase, int maxcols) {
                 GCI_HIP(hipEventRecord(e1, stream_));
                 prof_events_.push_back({mode, e0, e1});
             }
+            launch(k_unpack_slots, G, 256, stream_, dprob_.p, cp, cr, phase, M, c0, cb, mode);
         }
-        launch(k_unpack_slots, G, 256, stream_, dprob_.p, cp, cr, phase, M, c0, cb);
     }
 }
 

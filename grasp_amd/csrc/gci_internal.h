@@ -132,11 +132,11 @@ struct SpmmJob;
 // launch width leaves less LDS), and prefetch depth (steps each wavefront
 // keeps in flight).
 struct SpmmShape {
-    int R = 4096;
+    int R = 2048;
     int block = 512;
     int64_t target = 131072;
     int nslice = 32;
-    int depth = 4;
+    int depth = 8;
 };
 
 struct Pcsc {
