@@ -73,6 +73,7 @@ EXPORTS = [
     "gci_spmm_autotune",
     "gci_spmm_layout",
     "gci_server_submit_shared",
+    "gci_host_alloc", "gci_host_free",
 ]
 
 

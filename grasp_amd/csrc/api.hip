@@ -115,6 +115,20 @@ int gci_init(int device) {
     });
 }
 
+int gci_host_alloc(size_t bytes, void **ptr) {
+    return guarded([&] {
+        GCI_REQUIRE(ptr && bytes > 0, "gci_host_alloc: bad arguments");
+        *ptr = nullptr;
+        GCI_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    });
+}
+
+int gci_host_free(void *ptr) {
+    return guarded([&] {
+        if (ptr) GCI_HIP(hipHostFree(ptr));
+    });
+}
+
 int gci_radgrd(int n, double rnt, double h, double *r, double *rp, double *rpor) {
     return guarded([&] {
         GCI_REQUIRE(n > 0 && r && rp && rpor, "gci_radgrd: bad arguments");

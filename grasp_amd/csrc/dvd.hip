@@ -1610,6 +1610,19 @@ void DavidsonEngine::copy_vectors(const std::vector<int> &slots, double *evec) {
         dvec_.alloc((size_t)total);
     }
     const size_t bytes = (size_t)total * sizeof(double);
+    GCI_HIP(hipMemcpyAsync(dvoff_.p, hvoff_pin_, cap_ * sizeof(int64_t), hipMemcpyHostToDevice, stream_));
+    launch(k_gather_vectors, (int)total_chunks_, 256, stream_, (const DvdProb *)dprob_.p,
+           (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p, (const int64_t *)dvoff_.p, dvec_.p);
+    GCI_HIP(hipGetLastError());
+    // a page-locked destination takes the copy directly
+    hipPointerAttribute_t attr{};
+    const hipError_t qa = hipPointerGetAttributes(&attr, evec);
+    (void)hipGetLastError();  // pageable memory reports an error here; it is not one
+    if (qa == hipSuccess && attr.type == hipMemoryTypeHost) {
+        GCI_HIP(hipMemcpyAsync(evec, dvec_.p, bytes, hipMemcpyDeviceToHost, stream_));
+        GCI_HIP(hipStreamSynchronize(stream_));
+        return;
+    }
     if (hvec_bytes_ < bytes) {
         GCI_HIP(hipStreamSynchronize(stream_));
         if (hvec_pin_) GCI_HIP(hipHostFree(hvec_pin_));
@@ -1618,10 +1631,6 @@ void DavidsonEngine::copy_vectors(const std::vector<int> &slots, double *evec) {
         GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hvec_pin_), bytes, hipHostMallocDefault));
         hvec_bytes_ = bytes;
     }
-    GCI_HIP(hipMemcpyAsync(dvoff_.p, hvoff_pin_, cap_ * sizeof(int64_t), hipMemcpyHostToDevice, stream_));
-    launch(k_gather_vectors, (int)total_chunks_, 256, stream_, (const DvdProb *)dprob_.p,
-           (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p, (const int64_t *)dvoff_.p, dvec_.p);
-    GCI_HIP(hipGetLastError());
     // device -> pinned staging in pieces, each followed by an event; host
     // threads copy a piece on into the caller's (pageable) memory as soon
     // as it has arrived, overlapping the transfer with the host copies

@@ -221,18 +221,48 @@ def setham(ncf, nw, lists, iab, kstart, keys, vals, lfordr=False, iccut=0, preci
     return HamiltonianBlock(h, nn.value, nz.value, pr.value), eav.value, nel.value
 
 
-def batch_solve(blocks, nev, tol, eav=None):
+class _PinnedBlock:
+    """Owner of a page-locked host allocation (freed with the last view)."""
+
+    def __init__(self, nbytes: int):
+        self.ptr = C.c_void_p()
+        check(L.lib().gci_host_alloc(C.c_size_t(nbytes), C.byref(self.ptr)))
+
+    def __del__(self):
+        if self.ptr:
+            L.lib().gci_host_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+def pinned_empty(count: int) -> np.ndarray:
+    """An uninitialised float64 array of `count` elements in page-locked host
+    memory: the device writes results into it by DMA, with no staging copy."""
+    owner = _PinnedBlock(max(1, int(count)) * 8)
+    buf = (C.c_double * max(1, int(count))).from_address(owner.ptr.value)
+    buf._owner = owner  # the memory lives while any view of it does
+    return np.frombuffer(buf, dtype=np.float64, count=int(count))
+
+
+def batch_solve(blocks, nev, tol, eav=None, out=None):
     """Batched form: all blocks' Davidson iterations interleaved on the
     device.  blocks: list of HamiltonianBlock; nev/tol/eav: per block.
     Returns (list of eigenvalue arrays, list of eigenvector arrays, stats);
-    the eigenvector arrays are n x nev column-major views into one buffer."""
+    the eigenvector arrays are n x nev column-major views into one buffer,
+    `out` when given (float64, at least sum(n * nev) elements; a
+    `pinned_empty` buffer takes the copy directly)."""
     nb = len(blocks)
     nev = np.ascontiguousarray(np.broadcast_to(np.asarray(nev, dtype=np.int32), (nb,)))
     tol = np.ascontiguousarray(np.broadcast_to(np.asarray(tol, dtype=np.float64), (nb,)))
     eav = np.zeros(nb) if eav is None else np.ascontiguousarray(eav, dtype=np.float64)
     handles = (C.c_void_p * nb)(*[b.handle for b in blocks])
     ev = np.zeros(int(nev.sum()))
-    vec = np.empty(int(sum(b.n * int(k) for b, k in zip(blocks, nev))))
+    total = int(sum(b.n * int(k) for b, k in zip(blocks, nev)))
+    if out is None:
+        vec = np.empty(total)
+    else:
+        if out.dtype != np.float64 or out.ndim != 1 or not out.flags.c_contiguous or out.size < total:
+            raise ValueError(f"batch_solve: out must be a contiguous float64 vector of at least {total} elements")
+        vec = out[:total]
     eo = np.zeros(nb, dtype=np.int64)
     vo = np.zeros(nb, dtype=np.int64)
     st = (L.GciDvdStats * nb)()

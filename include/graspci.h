@@ -37,6 +37,12 @@ const char *gci_last_error(void);
 /* Select the device (hipSetDevice) and warm up the runtime. */
 int gci_init(int device);
 
+/* Page-locked host memory (hipHostMalloc / hipHostFree).  No counterpart in
+ * the reference: result buffers for callers that keep eigenvectors in
+ * pinned memory, which the device then writes by DMA with no staging copy. */
+int gci_host_alloc(size_t bytes, void **ptr);
+int gci_host_free(void *ptr);
+
 /* ------------------------------------------------------------------ grid
  * Radial grid r_j = RNT (exp((j-1) H) - 1), RP = dr/dt, RPOR = RP / r,
  * n + 10 points (lib9290/radgrd.f90:41-60; rci90/getcid.f90:233-235). */

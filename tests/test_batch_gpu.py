@@ -118,3 +118,26 @@ def test_server_shared_matrix(gpu):
     B = np.asfortranarray(np.random.default_rng(4).standard_normal((csr.n, 2)))
     assert np.all(np.isfinite(h.spicmv2(B, L.GCI_OP_SPLIT_FULL)))
     h.free()
+
+
+def test_batch_solve_result_buffers(gpu):
+    """Eigenvectors copied into a caller's pageable buffer, a page-locked
+    one (direct DMA) and a fresh array agree (runs differ only by the
+    order of the SpMM's atomic sums)."""
+    sizes, fills, nev = [700, 2600, 1500], [0.06, 0.03, 0.05], [3, 4, 2]
+    data = _blocks(sizes, 700, fills)
+    blocks = [rci.HamiltonianBlock.from_csr(c, shift=-e, precision=L.GCI_SPLIT32) for c, e in data]
+    eav = [e for _, e in data]
+    total = sum(n * k for n, k in zip(sizes, nev))
+    ev0, V0, _ = rci.batch_solve(blocks, nev, 1e-9, eav)
+    pinned = rci.pinned_empty(total + 5)
+    ev1, V1, _ = rci.batch_solve(blocks, nev, 1e-9, eav, out=pinned)
+    plain = np.empty(total)
+    ev2, V2, _ = rci.batch_solve(blocks, nev, 1e-9, eav, out=plain)
+    for b in range(len(blocks)):
+        for ev, V in ((ev1, V1), (ev2, V2)):
+            assert np.all(np.abs(ev[b] - ev0[b]) <= 1e-10 * np.abs(ev0[b] + eav[b]))
+            assert np.all(np.abs(np.abs(np.sum(V[b] * V0[b], axis=0)) - 1.0) < 1e-8)
+    assert np.shares_memory(V1[0], pinned) and np.shares_memory(V2[0], plain)
+    with pytest.raises(ValueError):
+        rci.batch_solve(blocks, nev, 1e-9, eav, out=np.empty(total - 1))
