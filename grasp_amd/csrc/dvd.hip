@@ -94,9 +94,209 @@ __device__ void reduce_partials(const DvdProb &p, const double *partial, int nou
 
 __device__ inline int rr_pos(int j, int r, int np) { return j == 0 ? 0 : ((j - 1 + r) % (np - 1)) + 1; }
 
-// Rayleigh-Ritz on the packed S (k = KPASS): all k eigenpairs by cyclic
-// parallel Jacobi, eigenvalues ascending into EIGVAL and eigenvectors into
-// SVEC (k x k, ld k).  A round applies one rotation per index pair; a
+// The lowest eigenpairs of the symmetric k x k matrix in A (LDS, full
+// storage), as DSPEVX finds them: Householder reduction to a tridiagonal T
+// (DSPTRD), bisection for the eigenvalues (DSTEBZ; 32 Sturm counts per round
+// per eigenvalue), inverse iteration for the eigenvectors with
+// reorthogonalisation inside clusters (DSTEIN), and back-transformation by
+// the reflectors (DOPMTR).  nval eigenvalues go to p.eigval, the first nvec
+// eigenvectors to p.svec (k x nvec, ld k).  Called by all 512 threads;
+// W (at least 6400 doubles of LDS) is scratch.
+__device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k, int nval, int nvec) {
+    constexpr double EPS = 2.220446049250313e-16, SAFMIN = 2.2250738585072014e-308;
+    __shared__ double d[MAXLIM], e[MAXLIM], bet[MAXLIM], pv[MAXLIM], lam[MAXNUME + 1];
+    __shared__ double red[8];
+    __shared__ double tnorm_s, pivmin_s;
+    const int t = threadIdx.x, nt = blockDim.x, lane = t & 63, wave = t >> 6;
+    // --- Householder reduction: column j's reflector v (v_0 = x_0 - alpha,
+    // v_i = x_i) replaces A[j+1.., j]; T's subdiagonal goes to e[j].
+    for (int j = 0; j < k - 2; ++j) {
+        const int L = k - j - 1;
+        if (wave == 0) {
+            double s = 0.0;
+            for (int i = 1 + lane; i < L; i += 64) s = fma(A[j + 1 + i][j], A[j + 1 + i][j], s);
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            if (lane == 0) {
+                const double x0 = A[j + 1][j];
+                if (s == 0.0) {
+                    bet[j] = 0.0;
+                    e[j] = x0;
+                } else {
+                    const double nrm = sqrt(fma(x0, x0, s));
+                    const double alpha = x0 >= 0.0 ? -nrm : nrm;
+                    const double v0 = x0 - alpha;
+                    A[j + 1][j] = v0;
+                    bet[j] = 2.0 / fma(v0, v0, s);
+                    e[j] = alpha;
+                }
+            }
+        }
+        __syncthreads();
+        const double b = bet[j];
+        if (b != 0.0) {
+            // pv = b A22 v: G threads per row, G a power of two with G L <= 512
+            int G = 64;
+            while (G > 1 && G * L > nt) G >>= 1;
+            const int r = t / G, g = t % G;
+            double acc = 0.0;
+            if (r < L)
+                for (int c = g; c < L; c += G) acc = fma(A[j + 1 + r][j + 1 + c], A[j + 1 + c][j], acc);
+            for (int off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+            if (r < L && g == 0) pv[r] = b * acc;
+            __syncthreads();
+            if (wave == 0) {
+                double s = 0.0;
+                for (int i = lane; i < L; i += 64) s = fma(pv[i], A[j + 1 + i][j], s);
+                for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+                if (lane == 0) red[0] = 0.5 * b * s;
+            }
+            __syncthreads();
+            const double K = red[0];
+            // A22 -= v w^T + w v^T, w = pv - K v
+            for (int idx = t; idx < L * L; idx += nt) {
+                const int r2 = idx / L, c2 = idx % L;
+                const double vr = A[j + 1 + r2][j], vc = A[j + 1 + c2][j];
+                const double wr = fma(-K, vr, pv[r2]), wc = fma(-K, vc, pv[c2]);
+                A[j + 1 + r2][j + 1 + c2] -= fma(vr, wc, wr * vc);
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = t; i < k; i += nt) d[i] = A[i][i];
+    if (t == 0 && k >= 2) e[k - 2] = A[k - 1][k - 2];
+    __syncthreads();
+    if (t == 0) {
+        double tn = 0.0, emax = 0.0;
+        for (int i = 0; i < k; ++i) {
+            const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < k - 1 ? fabs(e[i]) : 0.0);
+            tn = fmax(tn, fabs(d[i]) + r);
+            if (i < k - 1) emax = fmax(emax, e[i] * e[i]);
+        }
+        tnorm_s = tn;
+        pivmin_s = SAFMIN * fmax(1.0, emax);
+    }
+    __syncthreads();
+    const double tnorm = tnorm_s, pivmin = pivmin_s;
+    // --- multisection: 32 lanes per eigenvalue, 33-fold shrink per round
+    for (int m0 = 0; m0 < nval; m0 += nt / 32) {
+        const int m = m0 + t / 32, l = t & 31;
+        double lo = -tnorm - 2.0 * EPS * tnorm - pivmin, hi = tnorm + 2.0 * EPS * tnorm + pivmin;
+        for (int it = 0; it < 16; ++it) {
+            const bool done = hi - lo <= 2.0 * EPS * fmax(fabs(lo), fabs(hi)) + 2.0 * pivmin;
+            const double x = lo + (hi - lo) * (double)(l + 1) / 33.0;
+            int c = 0;
+            double q = d[0] - x;
+            if (fabs(q) < pivmin) q = -pivmin;
+            c += q < 0.0;
+            for (int i = 1; i < k; ++i) {
+                q = (d[i] - x) - e[i - 1] * e[i - 1] / q;
+                if (fabs(q) < pivmin) q = -pivmin;
+                c += q < 0.0;
+            }
+            // points with fewer than m + 1 eigenvalues below them bound m from below
+            const uint64_t below = __ballot(c <= m) >> (t & 32);
+            const int nb = __popcll(below & 0xFFFFFFFFull);
+            const double nlo = nb > 0 ? __shfl(x, (t & 32) + nb - 1, 64) : lo;
+            const double nhi = nb < 32 ? __shfl(x, (t & 32) + nb, 64) : hi;
+            if (!done) {
+                lo = nlo;
+                hi = nhi;
+            }
+        }
+        if (m < nval && l == 0) lam[m] = 0.5 * (lo + hi);
+    }
+    __syncthreads();
+    // --- inverse iteration: one thread per cluster (eigenvalues closer than
+    // 1e-3 |T| in a row), members in order, each orthogonalised against the
+    // cluster's earlier members at every step; scratch W: 4 x MAXLIM per
+    // thread after the nvec vectors
+    double *Z = W;  // nvec x MAXLIM
+    if (t < nvec && (t == 0 || lam[t] - lam[t - 1] >= 1.0e-3 * tnorm)) {
+        double *dd = W + MAXNUME * MAXLIM + t * 4 * MAXLIM, *dl = dd + MAXLIM, *du = dl + MAXLIM, *d2 = du + MAXLIM;
+        const double tiny = EPS * tnorm;
+        for (int m = t; m < nvec && (m == t || lam[m] - lam[m - 1] < 1.0e-3 * tnorm); ++m) {
+            double *z = Z + m * MAXLIM;
+            for (int i = 0; i < k; ++i) {
+                const double f = (double)(i + 1) * 0.6180339887498949 * (double)(m + 1);
+                z[i] = (f - floor(f)) - 0.5;
+            }
+            for (int it = 0; it < 3; ++it) {
+                // (T - lam) y = z by LU with partial pivoting (DGTTRF/DGTTS2)
+                for (int i = 0; i < k; ++i) {
+                    dd[i] = d[i] - lam[m];
+                    if (i < k - 1) { dl[i] = e[i]; du[i] = e[i]; }
+                    d2[i] = 0.0;
+                }
+                uint64_t swp[2] = {0, 0};
+                for (int i = 0; i < k - 1; ++i) {
+                    if (fabs(dd[i]) >= fabs(dl[i])) {
+                        if (dd[i] == 0.0) dd[i] = tiny;
+                        const double f = dl[i] / dd[i];
+                        dl[i] = f;
+                        dd[i + 1] -= f * du[i];
+                    } else {
+                        const double f = dd[i] / dl[i];
+                        dd[i] = dl[i];
+                        dl[i] = f;
+                        const double tmp = du[i];
+                        du[i] = dd[i + 1];
+                        dd[i + 1] = tmp - f * dd[i + 1];
+                        if (i < k - 2) {
+                            d2[i] = du[i + 1];
+                            du[i + 1] = -f * du[i + 1];
+                        }
+                        swp[i >> 6] |= 1ull << (i & 63);
+                    }
+                }
+                for (int i = 0; i < k - 1; ++i) {
+                    if (!((swp[i >> 6] >> (i & 63)) & 1)) {
+                        z[i + 1] -= dl[i] * z[i];
+                    } else {
+                        const double tmp = z[i];
+                        z[i] = z[i + 1];
+                        z[i + 1] = tmp - dl[i] * z[i];
+                    }
+                }
+                auto piv = [&](double x) { return fabs(x) >= tiny ? x : (x >= 0.0 ? tiny : -tiny); };
+                z[k - 1] /= piv(dd[k - 1]);
+                if (k > 1) z[k - 2] = (z[k - 2] - du[k - 2] * z[k - 1]) / piv(dd[k - 2]);
+                for (int i = k - 3; i >= 0; --i) z[i] = (z[i] - du[i] * z[i + 1] - d2[i] * z[i + 2]) / piv(dd[i]);
+                for (int q = t; q < m; ++q) {  // earlier members of the cluster
+                    const double *zq = Z + q * MAXLIM;
+                    double s = 0.0;
+                    for (int i = 0; i < k; ++i) s = fma(zq[i], z[i], s);
+                    for (int i = 0; i < k; ++i) z[i] = fma(-s, zq[i], z[i]);
+                }
+                double nrm = 0.0;
+                for (int i = 0; i < k; ++i) nrm = fma(z[i], z[i], nrm);
+                nrm = 1.0 / sqrt(nrm);
+                for (int i = 0; i < k; ++i) z[i] *= nrm;
+            }
+        }
+    }
+    __syncthreads();
+    // --- back-transformation x = H_0 ... H_{k-3} z: a 64-lane wave per vector
+    for (int m = wave; m < nvec; m += nt / 64) {
+        double *z = Z + m * MAXLIM;
+        for (int j = k - 3; j >= 0; --j) {
+            const double b = bet[j];
+            if (b == 0.0) continue;
+            const int L = k - j - 1;
+            double s = 0.0;
+            for (int i = lane; i < L; i += 64) s = fma(A[j + 1 + i][j], z[j + 1 + i], s);
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            for (int i = lane; i < L; i += 64) z[j + 1 + i] = fma(-b * s, A[j + 1 + i][j], z[j + 1 + i]);
+        }
+    }
+    __syncthreads();
+    for (int idx = t; idx < k * nvec; idx += nt) p.svec[idx] = Z[(idx / k) * MAXLIM + idx % k];
+    for (int m = t; m < nval; m += nt) p.eigval[m] = lam[m];
+}
+
+// Rayleigh-Ritz on the packed S (k = KPASS), then TSTSEL.  Below LIM the
+// NUME lowest eigenpairs come from rr_tridiag (DSPEVX's method); at LIM all
+// k eigenpairs come from cyclic parallel Jacobi, eigenvalues ascending into
+// EIGVAL and eigenvectors into SVEC (k x k, ld k).  A round applies one rotation per index pair; a
 // thread owns a 2 x 2 block of A (and its mirror) or one row of a V column
 // pair, so a round is one pass over LDS with two barriers.
 __global__ __launch_bounds__(512) void k_rr(DvdProb *P) {
@@ -125,97 +325,106 @@ __global__ __launch_bounds__(512) void k_rr(DvdProb *P) {
         A[i][j] = a;
         V[i][j] = (i == j && i < k) ? 1.0 : 0.0;
     }
-    for (int sweep = 0; sweep < 40; ++sweep) {
-        if (t == 0) any_rot = 0;
+    if (k >= 3 && k < p.lim) {
+        // the NUME lowest eigenpairs (and eigenvalue NUME + 1, NEWVEC's gap)
+        // as DSPEVX computes them; a full basis (the OVFLOW step, which keeps
+        // REST Ritz vectors) takes the Jacobi sweeps below
         __syncthreads();
-        for (int r = 0; r < np - 1; ++r) {
-            if (t < npairs) {
-                int a = rr_pos(t, r, np), b = rr_pos(np - 1 - t, r, np);
-                if (a > b) { int tmp = a; a = b; b = tmp; }
-                double c = 1.0, s = 0.0, tt = 0.0;
-                if (b < k) {
-                    const double app = A[a][a], aqq = A[b][b], apq = A[a][b];
-                    if (apq != 0.0 && fabs(apq) > 2.2e-17 * sqrt(fabs(app * aqq))) {
-                        const double tau = (aqq - app) / (2.0 * apq);
-                        tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                        c = 1.0 / sqrt(1.0 + tt * tt);
-                        s = tt * c;
-                        any_rot = 1;
+        rr_tridiag(p, A, &V[0][0], k, min(k, p.nume + 1), min(k, p.nume));
+        __syncthreads();
+    } else {
+        for (int sweep = 0; sweep < 40; ++sweep) {
+            if (t == 0) any_rot = 0;
+            __syncthreads();
+            for (int r = 0; r < np - 1; ++r) {
+                if (t < npairs) {
+                    int a = rr_pos(t, r, np), b = rr_pos(np - 1 - t, r, np);
+                    if (a > b) { int tmp = a; a = b; b = tmp; }
+                    double c = 1.0, s = 0.0, tt = 0.0;
+                    if (b < k) {
+                        const double app = A[a][a], aqq = A[b][b], apq = A[a][b];
+                        if (apq != 0.0 && fabs(apq) > 2.2e-17 * sqrt(fabs(app * aqq))) {
+                            const double tau = (aqq - app) / (2.0 * apq);
+                            tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                            c = 1.0 / sqrt(1.0 + tt * tt);
+                            s = tt * c;
+                            any_rot = 1;
+                        }
                     }
+                    pa[t] = a;
+                    pb[t] = b;
+                    cs[t] = c;
+                    sn[t] = s;
+                    tn[t] = tt;
                 }
-                pa[t] = a;
-                pb[t] = b;
-                cs[t] = c;
-                sn[t] = s;
-                tn[t] = tt;
-            }
-            __syncthreads();
-            // A <- J^T A J by 2 x 2 blocks of index pairs (q <= u)
-            for (int idx = t; idx < npairs * npairs; idx += nt) {
-                const int q = idx / npairs, u = idx % npairs;
-                if (q > u) continue;
-                const double sq = sn[q], su = sn[u];
-                if (sq == 0.0 && su == 0.0) continue;
-                const int aq = pa[q], bq = pb[q];
-                if (q == u) {
-                    const double apq = A[aq][bq], tt = tn[q];
-                    A[aq][aq] -= tt * apq;
-                    A[bq][bq] += tt * apq;
-                    A[aq][bq] = 0.0;
-                    A[bq][aq] = 0.0;
-                    continue;
+                __syncthreads();
+                // A <- J^T A J by 2 x 2 blocks of index pairs (q <= u)
+                for (int idx = t; idx < npairs * npairs; idx += nt) {
+                    const int q = idx / npairs, u = idx % npairs;
+                    if (q > u) continue;
+                    const double sq = sn[q], su = sn[u];
+                    if (sq == 0.0 && su == 0.0) continue;
+                    const int aq = pa[q], bq = pb[q];
+                    if (q == u) {
+                        const double apq = A[aq][bq], tt = tn[q];
+                        A[aq][aq] -= tt * apq;
+                        A[bq][bq] += tt * apq;
+                        A[aq][bq] = 0.0;
+                        A[bq][aq] = 0.0;
+                        continue;
+                    }
+                    const int au = pa[u], bu = pb[u];
+                    const double cq = cs[q], cu = cs[u];
+                    const double x00 = A[aq][au], x01 = A[aq][bu], x10 = A[bq][au], x11 = A[bq][bu];
+                    const double u00 = cq * x00 - sq * x10, u01 = cq * x01 - sq * x11;
+                    const double u10 = sq * x00 + cq * x10, u11 = sq * x01 + cq * x11;
+                    const double y00 = cu * u00 - su * u01, y01 = su * u00 + cu * u01;
+                    const double y10 = cu * u10 - su * u11, y11 = su * u10 + cu * u11;
+                    A[aq][au] = y00;
+                    A[aq][bu] = y01;
+                    A[bq][au] = y10;
+                    A[bq][bu] = y11;
+                    A[au][aq] = y00;
+                    A[bu][aq] = y01;
+                    A[au][bq] = y10;
+                    A[bu][bq] = y11;
                 }
-                const int au = pa[u], bu = pb[u];
-                const double cq = cs[q], cu = cs[u];
-                const double x00 = A[aq][au], x01 = A[aq][bu], x10 = A[bq][au], x11 = A[bq][bu];
-                const double u00 = cq * x00 - sq * x10, u01 = cq * x01 - sq * x11;
-                const double u10 = sq * x00 + cq * x10, u11 = sq * x01 + cq * x11;
-                const double y00 = cu * u00 - su * u01, y01 = su * u00 + cu * u01;
-                const double y10 = cu * u10 - su * u11, y11 = su * u10 + cu * u11;
-                A[aq][au] = y00;
-                A[aq][bu] = y01;
-                A[bq][au] = y10;
-                A[bq][bu] = y11;
-                A[au][aq] = y00;
-                A[bu][aq] = y01;
-                A[au][bq] = y10;
-                A[bu][bq] = y11;
+                // V <- V J
+                for (int idx = t; idx < npairs * k; idx += nt) {
+                    const int q = idx / k, i = idx % k;
+                    const double s = sn[q];
+                    if (s == 0.0) continue;
+                    const int a = pa[q], b = pb[q];
+                    const double c = cs[q];
+                    const double vx = V[i][a], vy = V[i][b];
+                    V[i][a] = c * vx - s * vy;
+                    V[i][b] = s * vx + c * vy;
+                }
+                __syncthreads();
             }
-            // V <- V J
-            for (int idx = t; idx < npairs * k; idx += nt) {
-                const int q = idx / k, i = idx % k;
-                const double s = sn[q];
-                if (s == 0.0) continue;
-                const int a = pa[q], b = pb[q];
-                const double c = cs[q];
-                const double vx = V[i][a], vy = V[i][b];
-                V[i][a] = c * vx - s * vy;
-                V[i][b] = s * vx + c * vy;
+            const int rotated = any_rot;
+            __syncthreads();  // every thread has read the flag before it is reset
+            if (!rotated) break;
+        }
+        // ascending order (ties by index)
+        for (int i = t; i < k; i += nt) {
+            const double di = A[i][i];
+            int rk = 0;
+            for (int j = 0; j < k; ++j) {
+                const double dj = A[j][j];
+                rk += (dj < di) || (dj == di && j < i);
             }
-            __syncthreads();
+            rank_of[i] = rk;
         }
-        const int rotated = any_rot;
-        __syncthreads();  // every thread has read the flag before it is reset
-        if (!rotated) break;
-    }
-    // ascending order (ties by index)
-    for (int i = t; i < k; i += nt) {
-        const double di = A[i][i];
-        int rk = 0;
-        for (int j = 0; j < k; ++j) {
-            const double dj = A[j][j];
-            rk += (dj < di) || (dj == di && j < i);
+        __syncthreads();
+        for (int idx = t; idx < k * k; idx += nt) {
+            const int i = idx / k, r = idx % k;  // eigen index i, component r
+            const int rk = rank_of[i];
+            p.svec[rk * k + r] = V[r][i];
+            if (r == 0) p.eigval[rk] = A[i][i];
         }
-        rank_of[i] = rk;
+        __syncthreads();
     }
-    __syncthreads();
-    for (int idx = t; idx < k * k; idx += nt) {
-        const int i = idx / k, r = idx % k;  // eigen index i, component r
-        const int rk = rank_of[i];
-        p.svec[rk * k + r] = V[r][i];
-        if (r == 0) p.eigval[rk] = A[i][i];
-    }
-    __syncthreads();
     if (t != 0) return;
     // TSTSEL (dvdson.f90:1328-1411); ISELEC = 1..NUME.  A CRITE below the
     // rounding unit of the eigenvalue (MANEIG's 1e-17) can only be met by a
