@@ -66,15 +66,19 @@ HMat *upload(int64_t n, const int64_t *iendc, const int32_t *irow, const double 
         h->n = n;
         h->nnz = nnz;
         h->precision = precision;
-        GCI_HIP(hipStreamCreate(&h->stream));
+        // a non-blocking stream and stream-ordered copies: nothing here
+        // touches the legacy stream, which a server's worker thread may be
+        // capturing an iteration graph beside
+        GCI_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        hipStream_t s = h->stream;
         h->rp.alloc(n + 1);
         h->col.alloc(nnz);
         h->diag.alloc(n);
-        GCI_HIP(hipMemcpy(h->rp.p, rp.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
-        if (nnz) GCI_HIP(hipMemcpy(h->col.p, col.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice));
-        if (n) GCI_HIP(hipMemcpy(h->diag.p, diag.data(), n * sizeof(double), hipMemcpyHostToDevice));
         h->val.alloc(nnz);
-        if (nnz) GCI_HIP(hipMemcpy(h->val.p, val.data(), nnz * sizeof(double), hipMemcpyHostToDevice));
+        GCI_HIP(hipMemcpyAsync(h->rp.p, rp.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        if (nnz) GCI_HIP(hipMemcpyAsync(h->col.p, col.data(), nnz * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        if (n) GCI_HIP(hipMemcpyAsync(h->diag.p, diag.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+        if (nnz) GCI_HIP(hipMemcpyAsync(h->val.p, val.data(), nnz * sizeof(double), hipMemcpyHostToDevice, s));
         if (precision == GCI_SPLIT32) {
             h->hi.alloc(nnz);
             h->lo.alloc(nnz);

@@ -183,6 +183,7 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
         double lo = -tnorm - 2.0 * EPS * tnorm - pivmin, hi = tnorm + 2.0 * EPS * tnorm + pivmin;
         for (int it = 0; it < 16; ++it) {
             const bool done = hi - lo <= 2.0 * EPS * fmax(fabs(lo), fabs(hi)) + 2.0 * pivmin;
+            if (__ballot(!done) == 0) break;  // both eigenvalues of this wavefront found
             const double x = lo + (hi - lo) * (double)(l + 1) / 33.0;
             int c = 0;
             double q = d[0] - x;
@@ -206,75 +207,97 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
         if (m < nval && l == 0) lam[m] = 0.5 * (lo + hi);
     }
     __syncthreads();
-    // --- inverse iteration: one thread per cluster (eigenvalues closer than
-    // 1e-3 |T| in a row), members in order, each orthogonalised against the
-    // cluster's earlier members at every step; scratch W: 4 x MAXLIM per
-    // thread after the nvec vectors
+    // --- inverse iteration, one thread per eigenvector: T - lam is
+    // factorised once (DGTTRF, LU with partial pivoting, recurrences carried
+    // in registers), then three solves (DGTTS2); after every solve wavefront 0
+    // reorthogonalises each vector against the earlier members of its
+    // cluster (eigenvalues closer than 1e-3 |T| in a row, DSTEIN's rule) and
+    // normalises it.  Scratch W: nvec vectors, then 4 x MAXLIM per thread.
     double *Z = W;  // nvec x MAXLIM
-    if (t < nvec && (t == 0 || lam[t] - lam[t - 1] >= 1.0e-3 * tnorm)) {
-        double *dd = W + MAXNUME * MAXLIM + t * 4 * MAXLIM, *dl = dd + MAXLIM, *du = dl + MAXLIM, *d2 = du + MAXLIM;
-        const double tiny = EPS * tnorm;
-        for (int m = t; m < nvec && (m == t || lam[m] - lam[m - 1] < 1.0e-3 * tnorm); ++m) {
-            double *z = Z + m * MAXLIM;
-            for (int i = 0; i < k; ++i) {
-                const double f = (double)(i + 1) * 0.6180339887498949 * (double)(m + 1);
-                z[i] = (f - floor(f)) - 0.5;
-            }
-            for (int it = 0; it < 3; ++it) {
-                // (T - lam) y = z by LU with partial pivoting (DGTTRF/DGTTS2)
-                for (int i = 0; i < k; ++i) {
-                    dd[i] = d[i] - lam[m];
-                    if (i < k - 1) { dl[i] = e[i]; du[i] = e[i]; }
-                    d2[i] = 0.0;
-                }
-                uint64_t swp[2] = {0, 0};
-                for (int i = 0; i < k - 1; ++i) {
-                    if (fabs(dd[i]) >= fabs(dl[i])) {
-                        if (dd[i] == 0.0) dd[i] = tiny;
-                        const double f = dl[i] / dd[i];
-                        dl[i] = f;
-                        dd[i + 1] -= f * du[i];
-                    } else {
-                        const double f = dd[i] / dl[i];
-                        dd[i] = dl[i];
-                        dl[i] = f;
-                        const double tmp = du[i];
-                        du[i] = dd[i + 1];
-                        dd[i + 1] = tmp - f * dd[i + 1];
-                        if (i < k - 2) {
-                            d2[i] = du[i + 1];
-                            du[i + 1] = -f * du[i + 1];
-                        }
-                        swp[i >> 6] |= 1ull << (i & 63);
-                    }
-                }
-                for (int i = 0; i < k - 1; ++i) {
-                    if (!((swp[i >> 6] >> (i & 63)) & 1)) {
-                        z[i + 1] -= dl[i] * z[i];
-                    } else {
-                        const double tmp = z[i];
-                        z[i] = z[i + 1];
-                        z[i + 1] = tmp - dl[i] * z[i];
-                    }
-                }
-                auto piv = [&](double x) { return fabs(x) >= tiny ? x : (x >= 0.0 ? tiny : -tiny); };
-                z[k - 1] /= piv(dd[k - 1]);
-                if (k > 1) z[k - 2] = (z[k - 2] - du[k - 2] * z[k - 1]) / piv(dd[k - 2]);
-                for (int i = k - 3; i >= 0; --i) z[i] = (z[i] - du[i] * z[i + 1] - d2[i] * z[i + 2]) / piv(dd[i]);
-                for (int q = t; q < m; ++q) {  // earlier members of the cluster
-                    const double *zq = Z + q * MAXLIM;
-                    double s = 0.0;
-                    for (int i = 0; i < k; ++i) s = fma(zq[i], z[i], s);
-                    for (int i = 0; i < k; ++i) z[i] = fma(-s, zq[i], z[i]);
-                }
-                double nrm = 0.0;
-                for (int i = 0; i < k; ++i) nrm = fma(z[i], z[i], nrm);
-                nrm = 1.0 / sqrt(nrm);
-                for (int i = 0; i < k; ++i) z[i] *= nrm;
+    const double tiny = EPS * tnorm;
+    auto piv = [&](double x) { return fabs(x) >= tiny ? x : (x >= 0.0 ? tiny : -tiny); };
+    uint64_t swp0 = 0, swp1 = 0;
+    double *dd = W + MAXNUME * MAXLIM + t * 4 * MAXLIM, *dl = dd + MAXLIM, *du = dl + MAXLIM, *d2 = du + MAXLIM;
+    if (t < nvec) {
+        const double lm = lam[t];
+        double *z = Z + t * MAXLIM;
+        double dcur = d[0] - lm, ucur = k > 1 ? e[0] : 0.0;  // row i's diagonal and superdiagonal
+        for (int i = 0; i < k - 1; ++i) {
+            const double sub = e[i], dn = d[i + 1] - lm, un = i + 1 < k - 1 ? e[i + 1] : 0.0;
+            if (fabs(dcur) >= fabs(sub)) {
+                if (dcur == 0.0) dcur = tiny;
+                const double f = sub / dcur;
+                dd[i] = dcur;
+                dl[i] = f;
+                du[i] = ucur;
+                d2[i] = 0.0;
+                dcur = dn - f * ucur;
+                ucur = un;
+            } else {
+                const double f = dcur / sub;
+                dd[i] = sub;
+                dl[i] = f;
+                du[i] = dn;
+                d2[i] = un;
+                dcur = ucur - f * dn;
+                ucur = -f * un;
+                if (i < 64) swp0 |= 1ull << i;
+                else swp1 |= 1ull << (i - 64);
             }
         }
+        dd[k - 1] = dcur;
+        for (int i = 0; i < k; ++i) {
+            const double f = (double)(i + 1) * 0.6180339887498949 * (double)(t + 1);
+            z[i] = (f - floor(f)) - 0.5;
+        }
     }
-    __syncthreads();
+    for (int it = 0; it < 3; ++it) {
+        if (t < nvec) {
+            double *z = Z + t * MAXLIM;
+            double zc = z[0];
+            for (int i = 0; i < k - 1; ++i) {  // forward, with the row swaps
+                const bool sw = ((i < 64 ? swp0 >> i : swp1 >> (i - 64)) & 1) != 0;
+                const double zn = z[i + 1];
+                if (!sw) {
+                    z[i] = zc;
+                    zc = zn - dl[i] * zc;
+                } else {
+                    z[i] = zn;
+                    zc = zc - dl[i] * zn;
+                }
+            }
+            z[k - 1] = zc;
+            double z1 = zc / piv(dd[k - 1]), z2 = 0.0;  // x[i+1], x[i+2]
+            z[k - 1] = z1;
+            for (int i = k - 2; i >= 0; --i) {
+                const double x = (z[i] - du[i] * z1 - d2[i] * z2) / piv(dd[i]);
+                z[i] = x;
+                z2 = z1;
+                z1 = x;
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {
+            for (int m = 0; m < nvec; ++m) {
+                double *z = Z + m * MAXLIM;
+                int q0 = m;
+                while (q0 > 0 && lam[q0] - lam[q0 - 1] < 1.0e-3 * tnorm) --q0;
+                for (int q = q0; q < m; ++q) {
+                    const double *zq = Z + q * MAXLIM;
+                    double sd = 0.0;
+                    for (int i = lane; i < k; i += 64) sd = fma(zq[i], z[i], sd);
+                    for (int off = 32; off > 0; off >>= 1) sd += __shfl_xor(sd, off, 64);
+                    for (int i = lane; i < k; i += 64) z[i] = fma(-sd, zq[i], z[i]);
+                }
+                double nr = 0.0;
+                for (int i = lane; i < k; i += 64) nr = fma(z[i], z[i], nr);
+                for (int off = 32; off > 0; off >>= 1) nr += __shfl_xor(nr, off, 64);
+                const double sc = 1.0 / sqrt(nr);
+                for (int i = lane; i < k; i += 64) z[i] *= sc;
+            }
+        }
+        __syncthreads();
+    }
     // --- back-transformation x = H_0 ... H_{k-3} z: a 64-lane wave per vector
     for (int m = wave; m < nvec; m += nt / 64) {
         double *z = Z + m * MAXLIM;

@@ -256,7 +256,7 @@ extern "C" int gci_h_assemble(int ncf, int nw, const gci_pair_lists *L, const do
             h->n = ncf;
             h->nnz = total;
             h->precision = precision;
-            GCI_HIP(hipStreamCreate(&h->stream));
+            GCI_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
             h->rp.alloc(ncf + 1);
             h->col.alloc(total);
             h->val.alloc(total);

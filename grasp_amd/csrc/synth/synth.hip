@@ -150,7 +150,7 @@ int gcs_synth_block(int64_t n, uint64_t seed, double fill, double diag_mean, dou
         try {
             h->n = n;
             h->precision = precision;
-            GCI_HIP(hipStreamCreate(&h->stream));
+            GCI_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
             hipStream_t s = h->stream;
             GCI_HIP(hipMemsetAsync(cnt.p, 0, (n + 1) * sizeof(int64_t), s));
             const int64_t blocks = (n * 64 + 255) / 256;

@@ -153,7 +153,9 @@ int gci_hmat_download(const gci_hmat *h, int64_t *iendc, int32_t *irow, double *
 #define GCI_OP_SPLIT_FULL 2
 int gci_spmm(gci_hmat *h, int m, const double *B, double *C, int mode);
 /* Same on device pointers (leading dimension ld >= n), enqueued on the
- * handle's stream; no synchronisation. */
+ * handle's stream; no synchronisation.  The handle's stream is
+ * non-blocking: work that produced dB on another stream must have completed
+ * (or be waited for) before the call. */
 int gci_spmm_device(gci_hmat *h, int m, const double *dB, int ldb, double *dC, int ldc, int mode);
 
 /* --------------------------------------------- piece 4: Davidson
