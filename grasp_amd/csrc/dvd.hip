@@ -837,17 +837,26 @@ __global__ __launch_bounds__(256) void k_update(DvdProb *P, const int *chunk_pro
 
 // Cholesky of the nnew x nnew Gram matrix; rinv = R^{-1} (upper).  A new
 // column that is numerically in the span of the basis and of the earlier
-// new columns (|w_j|^2 after projection below 1e-24 of its Gram entry, e.g.
-// A e_i for a CSF row with no couplings) is dropped from the product; in
-// the first pass it is then replaced by a pseudo-random direction that the
-// second pass orthonormalises.
+// new columns (|w_j|^2 after projection below 1e-24 of |w_j|^2 before it,
+// e.g. A x for an exact eigenvector x, or A e_i for a CSF row with no
+// couplings) is dropped from the product; in the first pass it is then
+// replaced by a pseudo-random direction that the second pass
+// orthonormalises.
 __global__ void k_chol(DvdProb *P, const double *partial, int want_phase, int pass) {
     DvdProb &p = P[blockIdx.x];
     if (p.phase != want_phase) return;
     __shared__ double G[MAXNUME * MAXNUME];
     __shared__ double R[MAXNUME][MAXNUME + 1];
-    const int nn = p.nnew;
+    __shared__ double w0[MAXNUME];  // squared norms of the new columns before this pass's projection
+    const int nn = p.nnew, k = p.kpass;
     reduce_partials(p, partial, nn * nn, G);
+    // |w|^2 before projection = |w_projected|^2 + sum_i coef_ij^2 (orthonormal basis)
+    for (int j = threadIdx.x; j < nn; j += blockDim.x) {
+        double c2 = 0.0;
+        for (int i = 0; i < k; ++i) c2 = fma(p.coef[i * nn + j], p.coef[i * nn + j], c2);
+        w0[j] = G[j * nn + j] + c2;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
     int brk = 0;
     for (int i = 0; i < nn; ++i)
@@ -855,7 +864,7 @@ __global__ void k_chol(DvdProb *P, const double *partial, int want_phase, int pa
     for (int j = 0; j < nn; ++j) {
         double s = G[j * nn + j];
         for (int q = 0; q < j; ++q) s -= R[q][j] * R[q][j];
-        if (!(s > 1.0e-24 * fabs(G[j * nn + j])) || !(s > 1.0e-300)) {
+        if (!(s > 1.0e-24 * w0[j]) || !(s > 1.0e-300)) {
             brk |= 1 << j;
             R[j][j] = 1.0;
             continue;  // R[j][i > j] stay zero

@@ -141,3 +141,21 @@ def test_batch_solve_result_buffers(gpu):
     assert np.shares_memory(V1[0], pinned) and np.shares_memory(V2[0], plain)
     with pytest.raises(ValueError):
         rci.batch_solve(blocks, nev, 1e-9, eav, out=np.empty(total - 1))
+
+
+def test_batch_solve_decoupled_block(gpu):
+    """A very sparse block whose Ritz vectors are exact eigenvectors: at the
+    refinement restart A x lies in the basis span, which the Gram-Schmidt
+    breakdown test must catch (a normalised remainder of rounding noise
+    duplicated a basis vector and the solve never converged)."""
+    from grasp_amd import synth
+    h, eav = synth.synth_block(308, 1706206421809167660, fill=0.0107)
+    iendc, irow, emt = h.download()
+    # the held matrix (diagonal shifted by -EAV), densified by the oracle operator
+    want = np.linalg.eigvalsh(O.spicmv2(iendc, irow, emt, np.eye(h.n)))
+    for nev in (3, 5):
+        ev, V, st = rci.batch_solve([h], [nev], 1e-8, [eav])
+        assert st[0]["ierr"] == 0 and st[0]["nloops"] < 50, st[0]
+        rel = np.abs(ev[0] - want[:nev]) / np.abs(want[:nev] + eav)
+        assert np.all(rel < 1e-8), rel
+    h.free()
