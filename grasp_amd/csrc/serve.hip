@@ -154,8 +154,10 @@ void worker_loop(gci_server *s) {
                 any_running = !s->running.empty();
             }
             if (!any_running) continue;
-            s->eng->step(4);
-            std::vector<int> done = s->eng->poll();
+            // one iteration queued ahead of the state the host reads, so the
+            // device does not idle while finished requests are handled
+            s->eng->step(1);
+            std::vector<int> done = s->eng->poll_lagged();
             std::vector<int> ref;
             for (int sl : done)
                 if (s->eng->needs_refine(sl)) ref.push_back(sl);
