@@ -57,6 +57,7 @@ struct DvdProb {
     int kpass, nncv, nnew, rest, nloops, ierr, nmv;
     int phase, ovflow, ncand, conv_all, niv;
     int refine_phase;               // 0 bulk operator, 1 FP64 (hi+lo) operator
+    int will_refine;                // an FP64 refinement follows the bulk phase
     int refine_loops;
     int bulk_loops, bulk_nmv;       // iterations/products of the bulk phase
     int brk_mask;                   // new columns that lost rank in the first GS pass

@@ -501,6 +501,9 @@ __global__ __launch_bounds__(512) void k_rr(DvdProb *P) {
 // --------------------------------------------------- basis rotation
 // mode 0: OVFLOW (phase ITER, ovflow set): keep REST columns.
 // mode 1: exit (phase FINAL): keep NUME columns.
+// The exit of a split-precision bulk phase that the FP64 refinement follows.
+__device__ inline bool bulk_exit(const DvdProb &p) { return p.will_refine && !p.refine_phase; }
+
 __global__ __launch_bounds__(256) void k_rotate(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
                                                 int mode) {
     DvdProb &p = P[chunk_prob[blockIdx.x]];
@@ -513,7 +516,10 @@ __global__ __launch_bounds__(256) void k_rotate(DvdProb *P, const int *chunk_pro
     for (int i = threadIdx.x; i < kin * kout; i += blockDim.x) sv[i] = p.svec[i];
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
     __syncthreads();
-    for (int which = 0; which < 2; ++which) {
+    // a bulk exit followed by the FP64 refinement keeps only the Ritz
+    // vectors: the restart recomputes their products with the FP64 operator
+    const int nrot = (mode == 1 && bulk_exit(p)) ? 1 : 2;
+    for (int which = 0; which < nrot; ++which) {
         double *M = which == 0 ? p.basis : p.ab;
         for (int64_t rt = r0; rt < r1; rt += 64) {
             const int rows = (int)min<int64_t>(64, r1 - rt);
@@ -687,7 +693,7 @@ typedef double d4v __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
                                               double *partial, int sel, int want_phase) {
     DvdProb &p = P[chunk_prob[blockIdx.x]];
-    if (p.phase != want_phase) return;
+    if (p.phase != want_phase || (sel == 3 && bulk_exit(p))) return;
     __shared__ double ta[64][MAXLIM + 1];
     __shared__ double tb[64][MAXNUME + 1];
     const int k = p.kpass, nn = p.nnew;
@@ -971,22 +977,19 @@ __global__ void k_adds(DvdProb *P, const double *partial, int want_phase, int co
     }
 }
 
-// exit: OLDVAL = |OLDVAL - EIGVAL| (dvdson.f90:819-821)
-__global__ void k_final_pre(DvdProb *P, int nprob) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nprob) return;
-    DvdProb &p = P[i];
-    if (p.phase != PH_FINAL) return;
-    for (int q = 0; q < p.nume; ++q) p.oldval[q] = fabs(p.oldval[q] - p.eigval[q]);
-}
-
+// exit: OLDVAL = |OLDVAL - EIGVAL| (dvdson.f90:819-821), residual norms,
+// then PH_DONE
 __global__ void k_final_post(DvdProb *P, const double *partial) {
     DvdProb &p = P[blockIdx.x];
     if (p.phase != PH_FINAL) return;
     __shared__ double G[MAXNUME * MAXNUME];
     const int nu = p.nume;
+    if (threadIdx.x < nu) p.oldval[threadIdx.x] = fabs(p.oldval[threadIdx.x] - p.eigval[threadIdx.x]);
+    if (bulk_exit(p)) {  // no residuals for a phase the refinement restarts
+        if (threadIdx.x == 0) p.phase = PH_DONE;
+        return;
+    }
     reduce_partials(p, partial, nu * nu, G);
-    __syncthreads();
     if (threadIdx.x == 0) {
         for (int i = 0; i < nu; ++i) p.resn[i] = sqrt(G[i * nu + i]);
         p.phase = PH_DONE;
@@ -1195,6 +1198,70 @@ __global__ __launch_bounds__(256) void k_init_unit(DvdProb *P, const int *want) 
     __shared__ int64_t bi[256];
     __shared__ int64_t picked[MAXLIM];
     const int t = threadIdx.x;
+    if (p.niv <= MAXNUME) {
+        // one pass: each thread keeps its NIV smallest (value, index) in
+        // ascending order; then NIV rounds of a block minimum over the
+        // threads' heads, the winner advancing its list
+        const int niv = p.niv;
+        double tv[MAXNUME];
+        int64_t ti[MAXNUME];
+#pragma unroll
+        for (int q = 0; q < MAXNUME; ++q) {
+            tv[q] = 1.0e300;
+            ti[q] = -1;
+        }
+        for (int64_t i = t; i < p.n; i += blockDim.x) {
+            double d = p.diagH[i];
+            int64_t di = i;
+            // insertion: the entry takes the first place it beats, and the
+            // rest shift down (indices ascend along a thread's scan, so ties
+            // keep index order)
+            bool ins = false;
+#pragma unroll
+            for (int q = 0; q < MAXNUME; ++q) {
+                if (q < niv && (ins || ti[q] < 0 || d < tv[q])) {
+                    ins = true;
+                    const double sv = tv[q];
+                    const int64_t si = ti[q];
+                    tv[q] = d;
+                    ti[q] = di;
+                    d = sv;
+                    di = si;
+                    if (di < 0) break;
+                }
+            }
+        }
+        int head = 0;
+        for (int j = 0; j < niv; ++j) {
+            double hv = 1.0e300;
+            int64_t hi = -1;
+#pragma unroll
+            for (int q = 0; q < MAXNUME; ++q)
+                if (q == head) {
+                    hv = tv[q];
+                    hi = ti[q];
+                }
+            bv[t] = hv;
+            bi[t] = hi;
+            __syncthreads();
+            for (int off = 128; off > 0; off >>= 1) {
+                if (t < off) {
+                    const double o = bv[t + off];
+                    const int64_t oi = bi[t + off];
+                    if (oi >= 0 && (bi[t] < 0 || o < bv[t] || (o == bv[t] && oi < bi[t]))) {
+                        bv[t] = o;
+                        bi[t] = oi;
+                    }
+                }
+                __syncthreads();
+            }
+            const int64_t win = bi[0];
+            if (win >= 0 && win == hi) ++head;
+            if (t == 0 && win >= 0) p.basis[win + (int64_t)j * p.n] = 1.0;
+            __syncthreads();
+        }
+        return;
+    }
     for (int j = 0; j < p.niv; ++j) {
         double best = 1.0e300;
         int64_t bidx = -1;
@@ -1528,6 +1595,7 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         p.critr_final = sp.critr_final > 0 ? sp.critr_final : sp.critr;
         p.eshift = sp.eshift;
         p.pnnz = sp.h->nnz;
+        p.will_refine = sp.refine ? 1 : 0;
         p.pfull = sp.h->pc.full ? 1 : 0;
         p.slot = slot;
         p.phase = PH_INIT;
@@ -1671,7 +1739,6 @@ void DavidsonEngine::launch_finalize() {
     const int G = (int)total_chunks_;
     const int *cp = chunk_prob_.p;
     const int64_t *cr = chunk_row_.p;
-    launch(k_final_pre, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_);
     launch(k_rotate, G, 256, stream_, dprob_.p, cp, cr, 1);
     launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 3, (int)PH_FINAL);
     launch(k_final_post, cap_, 256, stream_, dprob_.p, (const double *)partial_.p);
