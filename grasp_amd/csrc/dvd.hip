@@ -353,7 +353,7 @@ __global__ __launch_bounds__(512) void k_rr(DvdProb *P) {
         // as DSPEVX computes them; a full basis (the OVFLOW step, which keeps
         // REST Ritz vectors) takes the Jacobi sweeps below
         __syncthreads();
-        rr_tridiag(p, A, &V[0][0], k, min(k, p.nume + 1), min(k, p.nume));
+        rr_tridiag(p, A, &V[0][0], k, min(k, p.nume + 1), min(k, p.nume + 1));
         __syncthreads();
     } else {
         for (int sweep = 0; sweep < 40; ++sweep) {
@@ -512,7 +512,9 @@ __global__ __launch_bounds__(256) void k_rotate(DvdProb *P, const int *chunk_pro
     __shared__ double sv[MAXLIM * MAXLIM];
     __shared__ double tile[64][MAXLIM + 1];
     const int kin = p.kpass;
-    const int kout = (mode == 0) ? p.rest : p.nume;
+    // a bulk exit keeps NUME + 1 Ritz vectors: the refinement restarts from
+    // all of them, with no INITDVD expansion step
+    const int kout = (mode == 0) ? p.rest : (bulk_exit(p) ? min(p.kpass, p.nume + 1) : p.nume);
     for (int i = threadIdx.x; i < kin * kout; i += blockDim.x) sv[i] = p.svec[i];
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
     __syncthreads();
@@ -1163,7 +1165,7 @@ __global__ void k_refine_begin(DvdProb *P, int nprob, const int *mask) {
     p.bulk_nmv = p.nmv;
     p.refine_phase = 1;
     p.phase = PH_INIT;
-    p.niv = p.nume;
+    p.niv = min(p.kpass, p.nume + 1);
     p.tol_rel = p.tol_final;
     p.critr = p.critr_final;
     p.nloops = 0;
@@ -1866,7 +1868,7 @@ void DavidsonEngine::refine_restart(const std::vector<int> &slots) {
         mask[sl] = 1;
         refined_[sl] = 1;
         phase_[sl] = PH_INIT;
-        niv_[sl] = spec_[sl].nume;
+        niv_[sl] = spec_[sl].nume + 1;  // the bulk exit kept NUME + 1 Ritz vectors
     }
     std::atomic_thread_fence(std::memory_order_release);
     launch(k_refine_begin, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, dmask);
