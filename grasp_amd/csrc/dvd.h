@@ -3,7 +3,6 @@
 #include "gci_internal.h"
 
 #include <algorithm>
-#include <cstdlib>
 
 namespace gci {
 
@@ -79,14 +78,18 @@ struct DvdProb {
     double rinv[MAXNUME * MAXNUME];
 };
 
-// Residual target of the split-precision bulk phase for a final target tol.
-inline double bulk_tolerance(double tol) {
-    static const double floor_ = [] {
-        const char *e = std::getenv("GCI_BULK_FLOOR");
-        return e ? std::atof(e) : 2.0e-10;
-    }();
-    return std::max(0.5 * tol, floor_);
-}
+// Unit start vectors of a batched or served solve.  NUME + 1 of them replace
+// INITDVD's expansion step (orthogonalising A b_1 and one more operator
+// product) when they still fit one 8-wide operator launch; past that the
+// Krylov direction A b_1 is the better extra vector (an 8-pair block took
+// 13 bulk iterations from 9 unit vectors against 11 with the expansion).
+inline int start_vectors(int nume, int lim) { return nume + 1 <= 8 ? std::min(nume + 1, lim) : nume; }
+
+// Residual target of the split-precision bulk phase for a final target
+// tol.  The 2e-10 floor is below the fp32 rounding of H on purpose: a bulk
+// phase stopped at 1e-8 or 3e-7 left the FP64 refinement more iterations and
+// the batched solve slower (22.8 and 23.7 ms against 21.1 ms).
+inline double bulk_tolerance(double tol) { return std::max(0.5 * tol, 2.0e-10); }
 
 struct DvdSpec {
     const HMat *h = nullptr;

@@ -233,8 +233,7 @@ int gci_batch_solve(int nblocks, gci_hmat **blocks, const int *nev, const double
             sp.tol_final = tol[b];
             sp.critr = 0.0;
             sp.eshift = eav ? eav[b] : 0.0;
-            // NUME + 1 unit start vectors: INITDVD then needs no expansion step
-            niv[b] = std::min(nev[b] + 1, sp.lim);
+            niv[b] = start_vectors(nev[b], sp.lim);
             guard = std::max(guard, 2 * sp.maxiter + 128);
         }
         std::vector<int> slots;

@@ -139,7 +139,7 @@ void worker_loop(gci_server *s) {
                 for (Request *r : admit) {
                     specs.push_back(spec_for(*r));
                     init.push_back(nullptr);
-                    niv.push_back(std::min(r->nev + 1, specs.back().lim));  // NUME + 1 unit start vectors
+                    niv.push_back(start_vectors(r->nev, specs.back().lim));
                 }
                 std::vector<int> slots = s->eng->admit(specs, init, niv);
                 std::lock_guard<std::mutex> lk(s->mu);
