@@ -204,6 +204,10 @@ class DavidsonEngine {
     int mask_row_ = 0;
     int64_t mask_seq_[NEV] = {-1, -1, -1, -1};  // launched_ when each flag row was last written
     int64_t waited_seq_ = 0;        // iterations known complete
+    char *hchunk_pin_ = nullptr;    // chunk and item tables staged for upload
+    size_t hchunk_cap_ = 0;
+    hipEvent_t chunk_ev_ = nullptr; // their last upload
+    bool chunk_ev_rec_ = false;
     double *hvec_pin_ = nullptr;    // staging of copy_vectors()
     size_t hvec_bytes_ = 0;
     std::vector<hipEvent_t> vec_ev_;

@@ -1438,6 +1438,7 @@ DavidsonEngine::DavidsonEngine(int capacity, hipStream_t s) : cap_(capacity), st
     GCI_HIP(hipDeviceSynchronize());
     restart_seq_.assign(cap_, 0);
     for (int i = 0; i < NEV; ++i) GCI_HIP(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
+    GCI_HIP(hipEventCreateWithFlags(&chunk_ev_, hipEventDisableTiming));
 }
 
 DavidsonEngine::~DavidsonEngine() {
@@ -1449,6 +1450,8 @@ DavidsonEngine::~DavidsonEngine() {
     for (hipEvent_t e : ev_)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : vec_ev_) (void)hipEventDestroy(e);
+    if (chunk_ev_) (void)hipEventDestroy(chunk_ev_);
+    if (hchunk_pin_) (void)hipHostFree(hchunk_pin_);
 }
 
 size_t DavidsonEngine::workspace_bytes(const DvdSpec &s) {
@@ -1477,19 +1480,6 @@ void DavidsonEngine::rebuild_chunks() {
             hrow_.push_back((int64_t)c * CHUNK);
         }
     }
-    total_chunks_ = (int64_t)hprob_.size();
-    if ((size_t)std::max<int64_t>(total_chunks_, 1) > chunk_prob_.n) {
-        GCI_HIP(hipStreamSynchronize(stream_));
-        chunk_prob_.alloc(total_chunks_);
-        chunk_row_.alloc(total_chunks_);
-        partial_.alloc((size_t)total_chunks_ * PSTRIDE);
-    }
-    if (total_chunks_) {
-        GCI_HIP(hipMemcpyAsync(chunk_prob_.p, hprob_.data(), total_chunks_ * sizeof(int), hipMemcpyHostToDevice,
-                               stream_));
-        GCI_HIP(hipMemcpyAsync(chunk_row_.p, hrow_.data(), total_chunks_ * sizeof(int64_t), hipMemcpyHostToDevice,
-                               stream_));
-    }
     std::vector<SpmmItem> items;
     for (int i = 0; i < cap_; ++i) {
         if (phase_[i] == PH_EMPTY) continue;
@@ -1499,18 +1489,50 @@ void DavidsonEngine::rebuild_chunks() {
             items.push_back(it);
         }
     }
+    total_chunks_ = (int64_t)hprob_.size();
     nitems_ = (int64_t)items.size();
-    if ((size_t)std::max<int64_t>(nitems_, 1) > ditems_.n) ditems_.alloc(std::max<int64_t>(nitems_, 1));
-    if (nitems_)
-        GCI_HIP(hipMemcpyAsync(ditems_.p, items.data(), nitems_ * sizeof(SpmmItem), hipMemcpyHostToDevice, stream_));
-    update_modes();
-    std::vector<ChunkMeta> meta(cap_);
+    const bool grow_chunks = (size_t)std::max<int64_t>(total_chunks_, 1) > chunk_prob_.n;
+    const bool grow_items = (size_t)std::max<int64_t>(nitems_, 1) > ditems_.n;
+    if (grow_chunks || grow_items) GCI_HIP(hipStreamSynchronize(stream_));  // queued work may use the old tables
+    if (grow_chunks) {
+        chunk_prob_.alloc(total_chunks_);
+        chunk_row_.alloc(total_chunks_);
+        partial_.alloc((size_t)total_chunks_ * PSTRIDE);
+    }
+    if (grow_items) ditems_.alloc(std::max<int64_t>(nitems_, 1));
+    // the tables go up from one page-locked arena, stream-ordered: the host
+    // does not wait for them (the arena is reused once the last upload from
+    // it has completed)
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_row = al(total_chunks_ * sizeof(int)), o_item = o_row + al(total_chunks_ * sizeof(int64_t)),
+                 o_meta = o_item + al(nitems_ * sizeof(SpmmItem)), need = o_meta + al(cap_ * sizeof(ChunkMeta));
+    if (chunk_ev_rec_) GCI_HIP(hipEventSynchronize(chunk_ev_));
+    if (need > hchunk_cap_) {
+        if (hchunk_pin_) GCI_HIP(hipHostFree(hchunk_pin_));
+        hchunk_pin_ = nullptr;
+        hchunk_cap_ = 0;
+        GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hchunk_pin_), need, hipHostMallocDefault));
+        hchunk_cap_ = need;
+    }
+    std::copy(hprob_.begin(), hprob_.end(), reinterpret_cast<int *>(hchunk_pin_));
+    std::copy(hrow_.begin(), hrow_.end(), reinterpret_cast<int64_t *>(hchunk_pin_ + o_row));
+    std::copy(items.begin(), items.end(), reinterpret_cast<SpmmItem *>(hchunk_pin_ + o_item));
+    ChunkMeta *meta = reinterpret_cast<ChunkMeta *>(hchunk_pin_ + o_meta);
     for (int i = 0; i < cap_; ++i) meta[i] = ChunkMeta{ch[i].first, ch[i].second, 0};
-    // a persistent staging buffer: a temporary's hipFree would synchronise
-    // the whole device at every admission and release
-    GCI_HIP(hipMemcpyAsync(dmeta_.p, meta.data(), cap_ * sizeof(ChunkMeta), hipMemcpyHostToDevice, stream_));
+    if (total_chunks_) {
+        GCI_HIP(hipMemcpyAsync(chunk_prob_.p, hchunk_pin_, total_chunks_ * sizeof(int), hipMemcpyHostToDevice,
+                               stream_));
+        GCI_HIP(hipMemcpyAsync(chunk_row_.p, hchunk_pin_ + o_row, total_chunks_ * sizeof(int64_t),
+                               hipMemcpyHostToDevice, stream_));
+    }
+    if (nitems_)
+        GCI_HIP(hipMemcpyAsync(ditems_.p, hchunk_pin_ + o_item, nitems_ * sizeof(SpmmItem), hipMemcpyHostToDevice,
+                               stream_));
+    GCI_HIP(hipMemcpyAsync(dmeta_.p, meta, cap_ * sizeof(ChunkMeta), hipMemcpyHostToDevice, stream_));
     launch(k_set_chunks, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, (const ChunkMeta *)dmeta_.p);
-    GCI_HIP(hipStreamSynchronize(stream_));
+    GCI_HIP(hipEventRecord(chunk_ev_, stream_));
+    chunk_ev_rec_ = true;
+    update_modes();
     graph_valid_ = false;
 }
 
@@ -1987,7 +2009,10 @@ void DavidsonEngine::release_all() {
         spec_[i] = DvdSpec{};
         phase_[i] = PH_EMPTY;
     }
-    rebuild_chunks();
+    // no problem left: the tables are rebuilt at the next admission
+    total_chunks_ = 0;
+    nitems_ = 0;
+    graph_valid_ = false;
 }
 
 void DavidsonEngine::release(int slot) {
