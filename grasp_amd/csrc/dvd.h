@@ -183,7 +183,7 @@ class DavidsonEngine {
     // Persistent staging, so admissions and result reads allocate nothing
     // (a temporary's hipFree synchronises the device) and copy from pinned
     // memory (pageable copies block the host one by one).
-    DvdProb *hprob_pin_ = nullptr;  // admitted problems' initial state
+    char *hstage_pin_ = nullptr;    // admitted problems' initial state (DvdProb heads)
     int *hwant_pin_ = nullptr;      // unit-vector start flags
     double *hres_pin_ = nullptr;    // gathered results
     void *hrec_pin_ = nullptr;      // admitted problems' slots and workspace sizes

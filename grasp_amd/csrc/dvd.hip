@@ -32,6 +32,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -1371,21 +1372,32 @@ struct AdmitRec {
     int pad;
 };
 
-__global__ void k_admit_work(const DvdProb *__restrict__ stage, const AdmitRec *__restrict__ rec) {
-    const DvdProb &p = stage[blockIdx.y];
+// Admission stages only the head of DvdProb (every field before the small
+// arrays, which the kernels initialise as they need them), HEAD_STRIDE bytes
+// per problem.
+constexpr size_t HEAD_BYTES = offsetof(DvdProb, eigval);
+constexpr size_t HEAD_STRIDE = (HEAD_BYTES + 15) & ~(size_t)15;
+__device__ __host__ inline const DvdProb &staged(const char *stage, int i) {
+    return *reinterpret_cast<const DvdProb *>(stage + (size_t)i * HEAD_STRIDE);  // head fields only
+}
+
+__global__ void k_admit_work(const char *__restrict__ stage, const AdmitRec *__restrict__ rec) {
+    const DvdProb &p = staged(stage, blockIdx.y);
     const int64_t need = rec[blockIdx.y].need;
     const int64_t d0 = p.diag - p.basis;  // the diagonal's offset in the workspace
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < need; i += (int64_t)gridDim.x * blockDim.x)
         p.basis[i] = (i >= d0 && i < d0 + p.n) ? p.diagH[i - d0] : 0.0;
 }
 
-__global__ void k_admit_state(DvdProb *__restrict__ P, const DvdProb *__restrict__ stage,
+// the staged head, then zeros for the rest of the problem state
+__global__ void k_admit_state(DvdProb *__restrict__ P, const char *__restrict__ stage,
                               const AdmitRec *__restrict__ rec) {
-    constexpr int W = (int)(sizeof(DvdProb) / sizeof(uint32_t));
-    static_assert(sizeof(DvdProb) % sizeof(uint32_t) == 0, "DvdProb is copied as 32-bit words");
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(&stage[blockIdx.x]);
+    constexpr int W = (int)(sizeof(DvdProb) / sizeof(uint32_t)), WH = (int)(HEAD_BYTES / sizeof(uint32_t));
+    static_assert(sizeof(DvdProb) % sizeof(uint32_t) == 0 && HEAD_BYTES % sizeof(uint32_t) == 0,
+                  "DvdProb is copied as 32-bit words");
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(stage + (size_t)blockIdx.x * HEAD_STRIDE);
     uint32_t *dst = reinterpret_cast<uint32_t *>(&P[rec[blockIdx.x].slot]);
-    for (int i = threadIdx.x; i < W; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < W; i += blockDim.x) dst[i] = i < WH ? src[i] : 0u;
 }
 
 DavidsonEngine::DavidsonEngine(int capacity, hipStream_t s) : cap_(capacity), stream_(s) {
@@ -1411,10 +1423,10 @@ DavidsonEngine::DavidsonEngine(int capacity, hipStream_t s) : cap_(capacity), st
     chunk_row_.alloc(1);
     partial_.alloc(PSTRIDE);
     dwant_.alloc(cap_);
-    dstage_.alloc((size_t)cap_ * sizeof(DvdProb));
+    dstage_.alloc((size_t)cap_ * HEAD_STRIDE);
     drec_.alloc((size_t)cap_ * sizeof(AdmitRec));
     dres_.alloc((size_t)cap_ * RSTRIDE);
-    GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hprob_pin_), (size_t)cap_ * sizeof(DvdProb), hipHostMallocDefault));
+    GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hstage_pin_), (size_t)cap_ * HEAD_STRIDE, hipHostMallocDefault));
     GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hwant_pin_), (size_t)cap_ * sizeof(int), hipHostMallocDefault));
     GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&hrec_pin_), (size_t)cap_ * sizeof(AdmitRec),
                           hipHostMallocDefault));
@@ -1444,7 +1456,7 @@ DavidsonEngine::DavidsonEngine(int capacity, hipStream_t s) : cap_(capacity), st
 DavidsonEngine::~DavidsonEngine() {
     if (gexec_) (void)hipGraphExecDestroy(gexec_);
     (void)hipStreamSynchronize(stream_);
-    for (void *p : {(void *)hprob_pin_, (void *)hwant_pin_, (void *)hres_pin_, hrec_pin_, (void *)hphase_pin_,
+    for (void *p : {(void *)hstage_pin_, (void *)hwant_pin_, (void *)hres_pin_, hrec_pin_, (void *)hphase_pin_,
                     (void *)hvoff_pin_, (void *)hsnap_pin_, (void *)hmask_pin_, (void *)hvec_pin_})
         if (p) (void)hipHostFree(p);
     for (hipEvent_t e : ev_)
@@ -1540,6 +1552,7 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
                                        const std::vector<int> &niv) {
     std::vector<int> slots, unit_init;
     int64_t maxneed = 0;
+    std::vector<double *> wbasis(specs.size(), nullptr);  // each problem's workspace (basis first)
     GCI_REQUIRE(specs.size() <= (size_t)cap_, "davidson: more problems than slots");
     // the FP64 operator on a split matrix needs its layout rebuilt with the
     // FP64 values, which only an idle engine may do (running problems hold
@@ -1585,7 +1598,7 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         if (work_[slot].n < need) work_[slot].alloc(need);  // kept across releases for reuse
         double *w = work_[slot].p;
         DvdProb p;
-        std::memset(&p, 0, sizeof(p));
+        std::memset(&p, 0, HEAD_BYTES);  // only the head is staged
         p.n = n;
         p.lim = lim;
         p.nume = nume;
@@ -1626,7 +1639,8 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         p.niv = niv[q];
         // staged in pinned memory; uploaded and applied for the whole batch below
         const size_t nq = slots.size() - 1;
-        hprob_pin_[nq] = p;
+        std::memcpy(hstage_pin_ + nq * HEAD_STRIDE, &p, HEAD_BYTES);
+        wbasis[q] = p.basis;
         reinterpret_cast<AdmitRec *>(hrec_pin_)[nq] = AdmitRec{(int64_t)need, slot, 0};
         maxneed = std::max(maxneed, (int64_t)need);
         if (!init[q]) unit_init.push_back(slot);
@@ -1636,9 +1650,9 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
     }
     if (!slots.empty()) {
         const int nq = (int)slots.size();
-        GCI_HIP(hipMemcpyAsync(dstage_.p, hprob_pin_, nq * sizeof(DvdProb), hipMemcpyHostToDevice, stream_));
+        GCI_HIP(hipMemcpyAsync(dstage_.p, hstage_pin_, nq * HEAD_STRIDE, hipMemcpyHostToDevice, stream_));
         GCI_HIP(hipMemcpyAsync(drec_.p, hrec_pin_, nq * sizeof(AdmitRec), hipMemcpyHostToDevice, stream_));
-        const DvdProb *stage = reinterpret_cast<const DvdProb *>(dstage_.p);
+        const char *stage = dstage_.p;
         const AdmitRec *rec = reinterpret_cast<const AdmitRec *>(drec_.p);
         const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((maxneed + 255) / 256, 1024));
         hipLaunchKernelGGL(k_admit_work, dim3(gx, (unsigned)nq), dim3(256), 0, stream_, stage, rec);
@@ -1648,7 +1662,7 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         // caller-given initial vectors (the single-problem path) after the zeroing
         for (size_t q = 0; q < specs.size(); ++q)
             if (init[q])
-                GCI_HIP(hipMemcpyAsync(hprob_pin_[q].basis, init[q], (size_t)specs[q].h->n * niv[q] * sizeof(double),
+                GCI_HIP(hipMemcpyAsync(wbasis[q], init[q], (size_t)specs[q].h->n * niv[q] * sizeof(double),
                                        hipMemcpyHostToDevice, stream_));
     }
     if (!unit_init.empty()) {
