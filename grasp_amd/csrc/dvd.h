@@ -155,7 +155,7 @@ class DavidsonEngine {
     void launch_iteration();
     void launch_finalize();
     void launch_orth(int passes, int phase);
-    void launch_op(int phase, int M);
+    void launch_op(int phase, int M, int modes = -1);  // modes: mask of operator modes (-1: all in use)
     void update_modes();
     void run_initdvd(const std::vector<int> &slots);
     void drain_profile();

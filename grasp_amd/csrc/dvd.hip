@@ -1683,20 +1683,22 @@ void DavidsonEngine::run_initdvd(const std::vector<int> &slots) {
     // then S = B^T AB.  Each problem's sub-steps are derived on the device
     // from its NIV and NUME, so nothing is copied and nothing waits here.
     const int G = (int)total_chunks_;
-    int init_width = 1, stages = 0;
+    int init_width = 1, stages = 0, modes = 0;  // the operator modes of the problems starting here
     update_modes();
     for (int sl : slots) {
         init_width = std::max(init_width, std::max(niv_[sl], spec_[sl].nume));
         stages = std::max(stages, init_stages(niv_[sl], spec_[sl].nume));
+        const DvdSpec &sp = spec_[sl];
+        modes |= 1 << (sp.mode == GCI_OP_FP64 ? GCI_OP_FP64 : refined_[sl] ? GCI_OP_SPLIT_FULL : sp.mode);
     }
     const int pg = (cap_ + 63) / 64;
     launch(k_init_stage, pg, 64, stream_, dprob_.p, cap_, 0);
-    launch_op(PH_INIT, init_width);
+    launch_op(PH_INIT, init_width, modes);
     for (int st = 1; st <= stages; ++st) {
         launch(k_init_stage, pg, 64, stream_, dprob_.p, cap_, st);
         launch(k_init_copy, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p);
         launch_orth(2, PH_INIT);
-        launch_op(PH_INIT, init_width);
+        launch_op(PH_INIT, init_width, modes);
     }
     launch(k_init_stage, pg, 64, stream_, dprob_.p, cap_, -1);
     launch(k_gram, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p,
@@ -1744,12 +1746,13 @@ void DavidsonEngine::update_modes() {
 // maxcols: upper bound of NNEW over the problems in the stage.  Each
 // operator mode in use gets its own launches, in column blocks as wide as
 // that mode's LDS allows.
-void DavidsonEngine::launch_op(int phase, int maxcols) {
+void DavidsonEngine::launch_op(int phase, int maxcols, int modes) {
     const int G = (int)total_chunks_;
     const int *cp = chunk_prob_.p;
     const int64_t *cr = chunk_row_.p;
+    if (modes < 0) modes = modes_used_;
     for (int mode = 0; mode < 3; ++mode) {
-        if (!(modes_used_ & (1 << mode))) continue;
+        if (!(modes & (1 << mode))) continue;
         const int cb = std::min(8, spmm_max_width(mode));
         for (int c0 = 0; c0 < maxcols; c0 += cb) {
             const int M = spmm_width(std::min(cb, maxcols - c0));
