@@ -222,14 +222,14 @@ void recut_items(HMat &h);
 // the per-workgroup LDS limit.
 size_t spmm_lds_bytes(const SpmmShape &sh, int M, int mode);
 size_t spmm_lds_max();
-// Widest launch width (8, 4, 2, 1) whose LDS fits the shape in `mode`.
+// Widest launch width (8, 5, 4, 2, 1) whose LDS fits the shape in `mode`.
 int spmm_max_width(int mode);
 // Incremented by every set_spmm_shape (engines re-capture their graphs).
 uint64_t spmm_shape_version();
 // Re-cuts h.pc's work items for about `target` entries per item, inside one
 // panel (host only).
 void build_items(HMat &h, int64_t target);
-// Launch width (1, 2, 4 or 8) for m columns.
+// Launch width (1, 2, 4, 5 or 8) for m columns.
 int spmm_width(int m);
 // The panel-slice kernel over a list of work items whose jobs live in device
 // memory (jobs[item.job]); stream-ordered, no synchronisation.

@@ -443,7 +443,10 @@ struct Quad<GCI_OP_FP64> {
 // M consecutive values of type T from global memory, unconverted.
 template <int M, class T>
 __device__ inline void load_raw(const GAS T *p, T (&o)[M]) {
-    if constexpr (M * sizeof(T) >= 16) {
+    if constexpr ((M & (M - 1)) != 0) {  // odd widths: rows are not 16-byte aligned
+#pragma unroll
+        for (int v = 0; v < M; ++v) o[v] = p[v];
+    } else if constexpr (M * sizeof(T) >= 16) {
 #pragma unroll
         for (int q = 0; q < (int)(M * sizeof(T) / 16); ++q) {
             const f4v w = reinterpret_cast<const GAS f4v *>(p)[q];
@@ -753,6 +756,7 @@ void launch_mode(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs,
         case 1: launch_r<1, MODE>(d_items, nitems, d_jobs, sh, s); break;
         case 2: launch_r<2, MODE>(d_items, nitems, d_jobs, sh, s); break;
         case 4: launch_r<4, MODE>(d_items, nitems, d_jobs, sh, s); break;
+        case 5: launch_r<5, MODE>(d_items, nitems, d_jobs, sh, s); break;
         default: launch_r<8, MODE>(d_items, nitems, d_jobs, sh, s); break;
     }
 }
@@ -768,6 +772,7 @@ size_t lds_bytes(int R, int NS, int M, int mode) {
             case 1: return sell_lds<1, float>(R, NS, false).total;
             case 2: return sell_lds<2, float>(R, NS, false).total;
             case 4: return sell_lds<4, float>(R, NS, false).total;
+            case 5: return sell_lds<5, float>(R, NS, false).total;
             default: return sell_lds<8, float>(R, NS, false).total;
         }
     }
@@ -775,6 +780,7 @@ size_t lds_bytes(int R, int NS, int M, int mode) {
         case 1: return sell_lds<1, double>(R, NS, mode == GCI_OP_FP64).total;
         case 2: return sell_lds<2, double>(R, NS, mode == GCI_OP_FP64).total;
         case 4: return sell_lds<4, double>(R, NS, mode == GCI_OP_FP64).total;
+        case 5: return sell_lds<5, double>(R, NS, mode == GCI_OP_FP64).total;
         default: return sell_lds<8, double>(R, NS, mode == GCI_OP_FP64).total;
     }
 }
@@ -858,12 +864,15 @@ uint64_t spmm_shape_version() { return g_shape_version; }
 
 int spmm_max_width(int mode) {
     const SpmmShape sh = spmm_shape();
-    for (int w = 8; w > 1; w /= 2)
+    for (int w : {8, 5, 4, 2})
         if (chunk_slices(sh, w, mode) >= std::min(NS_MIN, sh.nslice)) return w;
     return 1;
 }
 
-int spmm_width(int m) { return m <= 1 ? 1 : m <= 2 ? 2 : m <= 4 ? 4 : 8; }
+// The instantiated launch widths: 1, 2, 4, 5 and 8 columns.  Five serves the
+// Davidson start and restart blocks of NUME + 1 = 5 vectors, and is the
+// widest launch whose LDS holds FP64 x rows at 2048-row panels.
+int spmm_width(int m) { return m <= 1 ? 1 : m <= 2 ? 2 : m <= 4 ? 4 : m == 5 ? 5 : 8; }
 
 void spmm_items(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, int M, int mode, hipStream_t s) {
     if (nitems <= 0) return;

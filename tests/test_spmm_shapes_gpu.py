@@ -46,7 +46,7 @@ def test_spmm_shapes_all_widths(shape_guard, shape):
     csr = W.clustered_block(5200, 31, fill=0.04)
     h32 = HamiltonianBlock.from_csr(csr, precision=L.GCI_SPLIT32)
     h64 = HamiltonianBlock.from_csr(csr)
-    for m in (1, 2, 3, 4, 8):
+    for m in (1, 2, 3, 4, 5, 8, 9):
         _check(h64, csr, m, L.GCI_OP_FP64, 100 + m)
         _check(h32, csr, m, L.GCI_OP_SPLIT_FULL, 200 + m)
         _check(h32, csr, m, L.GCI_OP_SPLIT_BULK, 300 + m)
