@@ -464,8 +464,9 @@ __device__ inline void load_raw(const GAS T *p, T (&o)[M]) {
 // when the launch may meet lower-triangle layouts (FP64 operator),
 // then xs (M x R of T), and the tables of the item's (at most NS) slices:
 // cs (column partial sums, NS x 64 rows of M fp64), xj (the x_j rows,
-// NS x 64 rows of M T), cl (columns, NS x 64 ints) and ss (first steps,
-// NS + 1 ints).
+// NS x 64 rows of M T, again only for lower-triangle layouts, whose
+// entries also update y_i), cl (columns, NS x 64 ints) and ss (first
+// steps, NS + 1 ints).
 struct SellLds {
     size_t xs, cs, xj, cl, ss, total;
 };
@@ -475,7 +476,7 @@ __host__ __device__ constexpr SellLds sell_lds(int R, int NS, bool rows) {
     m.xs = rows ? (size_t)R * M * sizeof(double) : 0;  // ys only for lower-triangle layouts
     m.cs = m.xs + (size_t)R * M * sizeof(T);
     m.xj = m.cs + (size_t)NS * SW * M * sizeof(double);
-    m.cl = m.xj + (size_t)NS * SW * M * sizeof(T);
+    m.cl = m.xj + (rows ? (size_t)NS * SW * M * sizeof(T) : 0);
     m.ss = m.cl + (size_t)NS * SW * sizeof(int);
     m.total = m.ss + (((size_t)(NS + 1) * sizeof(int) + 15) & ~(size_t)15);
     return m;
@@ -568,18 +569,19 @@ __global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items,
         for (int i = t; i < nsl * SW; i += BS) {
             const int c = jb.scol[c0 * SW + i];
             cl[i] = c;
-            T xr[M];
-            if (c >= 0) {
-                load_raw<M, T>(gx + (int64_t)c * M, xr);
-            } else {
+            if constexpr (ROWS) {  // x_j rows only where entries update y_i too
+                T xr[M];
+                if (c >= 0) {
+                    load_raw<M, T>(gx + (int64_t)c * M, xr);
+                } else {
 #pragma unroll
-                for (int v = 0; v < M; ++v) xr[v] = (T)0;
+                    for (int v = 0; v < M; ++v) xr[v] = (T)0;
+                }
+#pragma unroll
+                for (int v = 0; v < M; ++v) xjt[i * M + v] = xr[v];
             }
 #pragma unroll
-            for (int v = 0; v < M; ++v) {
-                xjt[i * M + v] = xr[v];
-                cs[i * M + v] = 0.0;
-            }
+            for (int v = 0; v < M; ++v) cs[i * M + v] = 0.0;
         }
         __syncthreads();
 
@@ -607,7 +609,7 @@ __global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items,
             auto enter = [&]() {  // x_j of slice si from its table
 #pragma unroll
                 for (int v = 0; v < M; ++v) {
-                    xj[v] = (double)xjt[(si * SW + lane) * M + v];
+                    xj[v] = ROWS ? (double)xjt[(si * SW + lane) * M + v] : 0.0;
                     acc[v] = 0.0;
                 }
             };

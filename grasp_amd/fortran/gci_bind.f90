@@ -64,6 +64,20 @@ module gci_bind
          integer(c_int) :: st
       end function gci_init
 
+      ! page-locked host memory for result buffers (hipHostMalloc / hipHostFree)
+      function gci_host_alloc(bytes, ptr) bind(C, name="gci_host_alloc") result(st)
+         import :: c_int, c_size_t, c_ptr
+         integer(c_size_t), value :: bytes
+         type(c_ptr) :: ptr
+         integer(c_int) :: st
+      end function gci_host_alloc
+
+      function gci_host_free(ptr) bind(C, name="gci_host_free") result(st)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ptr
+         integer(c_int) :: st
+      end function gci_host_free
+
       function gci_radgrd(n, rnt, h, r, rp, rpor) bind(C, name="gci_radgrd") result(st)
          import :: c_int, c_double
          integer(c_int), value :: n

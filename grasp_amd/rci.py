@@ -268,11 +268,13 @@ def batch_solve(blocks, nev, tol, eav=None, out=None):
     st = (L.GciDvdStats * nb)()
     check(L.lib().gci_batch_solve(C.c_int(nb), handles, i32p(nev), dp(tol), dp(eav), dp(ev), i64p(eo), dp(vec),
                                   i64p(vo), st))
-    evals = [ev[eo[b]: eo[b] + nev[b]].copy() for b in range(nb)]
+    # plain ints for the per-block slicing (numpy scalars index slowly)
+    eol, vol, nevl = eo.tolist(), vo.tolist(), nev.tolist()
+    evals = [ev[o: o + k] for o, k in zip(eol, nevl)]  # views of this call's own array
     # n x nev column-major views into the one returned buffer (no copy)
-    evecs = [vec[vo[b]: vo[b] + blocks[b].n * nev[b]].reshape(nev[b], blocks[b].n).T for b in range(nb)]
-    stats = [{"nloops": s.nloops, "nmv": s.nmv, "ierr": s.ierr, "refine_loops": s.refine_loops,
-              "seconds": s.seconds} for s in st]
+    evecs = [vec[o: o + b.n * k].reshape(k, b.n).T for o, b, k in zip(vol, blocks, nevl)]
+    keys = [f[0] for f in L.GciDvdStats._fields_]
+    stats = [dict(zip(keys, r)) for r in np.ctypeslib.as_array(st).tolist()]
     return evals, evecs, stats
 
 
