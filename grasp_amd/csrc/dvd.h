@@ -208,6 +208,8 @@ class DavidsonEngine {
     size_t hchunk_cap_ = 0;
     hipEvent_t chunk_ev_ = nullptr; // their last upload
     bool chunk_ev_rec_ = false;
+    hipEvent_t stage_ev_ = nullptr;  // the last admission's staging copies (pinned staging reuse)
+    bool stage_ev_rec_ = false;
     double *hvec_pin_ = nullptr;    // staging of copy_vectors()
     size_t hvec_bytes_ = 0;
     std::vector<hipEvent_t> vec_ev_;

@@ -1192,88 +1192,136 @@ __global__ __launch_bounds__(256) void k_copy_diag(DvdProb *P, const int *chunk_
     for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) p.diag[r] = p.diagH[r];
 }
 
-// Unit initial vectors on the NIV smallest diagonal entries (ties by
-// index) for slots flagged in want[]: NIV argmin passes over the diagonal.
-__global__ __launch_bounds__(256) void k_init_unit(DvdProb *P, const int *want) {
+// Unit initial vectors on the NIV smallest diagonal entries, ascending in
+// (value, index), for the slots flagged in want[].  For NIV <= MAXNUME each
+// chunk of rows first keeps its own NIV smallest (k_unit_chunk), and one
+// workgroup per problem merges the chunks' lists (k_unit_merge); wider
+// starts take NIV argmin passes over the whole diagonal (k_unit_wide).
+
+// (a, ai) before (b, bi): ascending value, ties by index; index -1 is no
+// entry and sorts last
+__device__ inline bool pair_before(double a, int64_t ai, double b, int64_t bi) {
+    return ai >= 0 && (bi < 0 || a < b || (a == b && ai < bi));
+}
+
+// a thread's NIV smallest pairs, ascending: the entry takes the first place
+// it beats and the rest shift down
+__device__ inline void top_insert(int niv, double (&tv)[MAXNUME], int64_t (&ti)[MAXNUME], double d, int64_t di) {
+#pragma unroll
+    for (int q = 0; q < MAXNUME; ++q) {
+        if (q < niv && di >= 0 && pair_before(d, di, tv[q], ti[q])) {
+            const double sv = tv[q];
+            const int64_t si = ti[q];
+            tv[q] = d;
+            ti[q] = di;
+            d = sv;
+            di = si;
+        }
+    }
+}
+
+// NIV rounds of a block minimum (256 threads) over the threads' list heads;
+// round j's winner lands in ov/oi[j] (index -1 once the entries run out)
+__device__ void top_merge_block(int niv, const double (&tv)[MAXNUME], const int64_t (&ti)[MAXNUME], double *bv,
+                                int64_t *bi, double *ov, int64_t *oi) {
+    const int t = threadIdx.x;
+    int head = 0;
+    for (int j = 0; j < niv; ++j) {
+        double hv = 1.0e300;
+        int64_t hi = -1;
+#pragma unroll
+        for (int q = 0; q < MAXNUME; ++q)
+            if (q == head) {
+                hv = tv[q];
+                hi = ti[q];
+            }
+        bv[t] = hv;
+        bi[t] = hi;
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if (t < off && pair_before(bv[t + off], bi[t + off], bv[t], bi[t])) {
+                bv[t] = bv[t + off];
+                bi[t] = bi[t + off];
+            }
+            __syncthreads();
+        }
+        const int64_t win = bi[0];
+        if (win >= 0 && win == hi) ++head;
+        if (t == 0) {
+            ov[j] = bv[0];
+            oi[j] = win;
+        }
+        __syncthreads();
+    }
+}
+
+// each chunk of a flagged problem: its NIV smallest (value, index) pairs into
+// the chunk's partial row, as doubles (index -1 where the chunk has fewer)
+__global__ __launch_bounds__(256) void k_unit_chunk(const DvdProb *P, const int *chunk_prob,
+                                                    const int64_t *chunk_row, const int *want, double *partial) {
+    const int pi = chunk_prob[blockIdx.x];
+    const DvdProb &p = P[pi];
+    if (!want[pi] || p.niv > MAXNUME) return;
+    __shared__ double bv[256], ov[MAXNUME];
+    __shared__ int64_t bi[256], oi[MAXNUME];
+    const int niv = p.niv;
+    double tv[MAXNUME];
+    int64_t ti[MAXNUME];
+#pragma unroll
+    for (int q = 0; q < MAXNUME; ++q) {
+        tv[q] = 1.0e300;
+        ti[q] = -1;
+    }
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) top_insert(niv, tv, ti, p.diagH[r], r);
+    top_merge_block(niv, tv, ti, bv, bi, ov, oi);
+    if ((int)threadIdx.x < niv) {
+        partial[(int64_t)blockIdx.x * PSTRIDE + 2 * threadIdx.x] = ov[threadIdx.x];
+        partial[(int64_t)blockIdx.x * PSTRIDE + 2 * threadIdx.x + 1] = (double)oi[threadIdx.x];
+    }
+}
+
+// one workgroup per flagged problem: the NIV smallest over its chunks' lists,
+// written as unit vectors into basis[:, 0:NIV] (zeroed at admission)
+__global__ __launch_bounds__(256) void k_unit_merge(DvdProb *P, const int *want, const double *partial) {
     DvdProb &p = P[blockIdx.x];
-    if (!want[blockIdx.x]) return;
+    if (!want[blockIdx.x] || p.niv > MAXNUME) return;
+    __shared__ double bv[256], ov[MAXNUME];
+    __shared__ int64_t bi[256], oi[MAXNUME];
+    const int niv = p.niv;
+    double tv[MAXNUME];
+    int64_t ti[MAXNUME];
+#pragma unroll
+    for (int q = 0; q < MAXNUME; ++q) {
+        tv[q] = 1.0e300;
+        ti[q] = -1;
+    }
+    const int64_t cnt = (int64_t)p.nchunks * niv;
+    for (int64_t k = threadIdx.x; k < cnt; k += blockDim.x) {
+        const double *e = partial + (p.chunk0 + k / niv) * PSTRIDE + 2 * (k % niv);
+        top_insert(niv, tv, ti, e[0], (int64_t)e[1]);
+    }
+    top_merge_block(niv, tv, ti, bv, bi, ov, oi);
+    const int j = threadIdx.x;
+    if (j < niv && oi[j] >= 0) p.basis[oi[j] + (int64_t)j * p.n] = 1.0;
+}
+
+// NIV > MAXNUME: NIV argmin passes over the whole diagonal
+__global__ __launch_bounds__(256) void k_unit_wide(DvdProb *P, const int *want) {
+    DvdProb &p = P[blockIdx.x];
+    if (!want[blockIdx.x] || p.niv <= MAXNUME) return;
     __shared__ double bv[256];
     __shared__ int64_t bi[256];
     __shared__ int64_t picked[MAXLIM];
     const int t = threadIdx.x;
-    if (p.niv <= MAXNUME) {
-        // one pass: each thread keeps its NIV smallest (value, index) in
-        // ascending order; then NIV rounds of a block minimum over the
-        // threads' heads, the winner advancing its list
-        const int niv = p.niv;
-        double tv[MAXNUME];
-        int64_t ti[MAXNUME];
-#pragma unroll
-        for (int q = 0; q < MAXNUME; ++q) {
-            tv[q] = 1.0e300;
-            ti[q] = -1;
-        }
-        for (int64_t i = t; i < p.n; i += blockDim.x) {
-            double d = p.diagH[i];
-            int64_t di = i;
-            // insertion: the entry takes the first place it beats, and the
-            // rest shift down (indices ascend along a thread's scan, so ties
-            // keep index order)
-            bool ins = false;
-#pragma unroll
-            for (int q = 0; q < MAXNUME; ++q) {
-                if (q < niv && (ins || ti[q] < 0 || d < tv[q])) {
-                    ins = true;
-                    const double sv = tv[q];
-                    const int64_t si = ti[q];
-                    tv[q] = d;
-                    ti[q] = di;
-                    d = sv;
-                    di = si;
-                    if (di < 0) break;
-                }
-            }
-        }
-        int head = 0;
-        for (int j = 0; j < niv; ++j) {
-            double hv = 1.0e300;
-            int64_t hi = -1;
-#pragma unroll
-            for (int q = 0; q < MAXNUME; ++q)
-                if (q == head) {
-                    hv = tv[q];
-                    hi = ti[q];
-                }
-            bv[t] = hv;
-            bi[t] = hi;
-            __syncthreads();
-            for (int off = 128; off > 0; off >>= 1) {
-                if (t < off) {
-                    const double o = bv[t + off];
-                    const int64_t oi = bi[t + off];
-                    if (oi >= 0 && (bi[t] < 0 || o < bv[t] || (o == bv[t] && oi < bi[t]))) {
-                        bv[t] = o;
-                        bi[t] = oi;
-                    }
-                }
-                __syncthreads();
-            }
-            const int64_t win = bi[0];
-            if (win >= 0 && win == hi) ++head;
-            if (t == 0 && win >= 0) p.basis[win + (int64_t)j * p.n] = 1.0;
-            __syncthreads();
-        }
-        return;
-    }
     for (int j = 0; j < p.niv; ++j) {
         double best = 1.0e300;
         int64_t bidx = -1;
         for (int64_t i = t; i < p.n; i += blockDim.x) {
             bool used = false;
             for (int q = 0; q < j; ++q) used |= (picked[q] == i);
-            const double d = p.diagH[i];
-            if (!used && (d < best || (d == best && (bidx < 0 || i < bidx)))) {
-                best = d;
+            if (!used && pair_before(p.diagH[i], i, best, bidx)) {
+                best = p.diagH[i];
                 bidx = i;
             }
         }
@@ -1281,13 +1329,9 @@ __global__ __launch_bounds__(256) void k_init_unit(DvdProb *P, const int *want) 
         bi[t] = bidx;
         __syncthreads();
         for (int off = 128; off > 0; off >>= 1) {
-            if (t < off) {
-                const double o = bv[t + off];
-                const int64_t oi = bi[t + off];
-                if (oi >= 0 && (bi[t] < 0 || o < bv[t] || (o == bv[t] && oi < bi[t]))) {
-                    bv[t] = o;
-                    bi[t] = oi;
-                }
+            if (t < off && pair_before(bv[t + off], bi[t + off], bv[t], bi[t])) {
+                bv[t] = bv[t + off];
+                bi[t] = bi[t + off];
             }
             __syncthreads();
         }
@@ -1451,6 +1495,7 @@ DavidsonEngine::DavidsonEngine(int capacity, hipStream_t s) : cap_(capacity), st
     restart_seq_.assign(cap_, 0);
     for (int i = 0; i < NEV; ++i) GCI_HIP(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
     GCI_HIP(hipEventCreateWithFlags(&chunk_ev_, hipEventDisableTiming));
+    GCI_HIP(hipEventCreateWithFlags(&stage_ev_, hipEventDisableTiming));
 }
 
 DavidsonEngine::~DavidsonEngine() {
@@ -1463,6 +1508,7 @@ DavidsonEngine::~DavidsonEngine() {
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : vec_ev_) (void)hipEventDestroy(e);
     if (chunk_ev_) (void)hipEventDestroy(chunk_ev_);
+    if (stage_ev_) (void)hipEventDestroy(stage_ev_);
     if (hchunk_pin_) (void)hipHostFree(hchunk_pin_);
 }
 
@@ -1552,6 +1598,7 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
                                        const std::vector<int> &niv) {
     std::vector<int> slots, unit_init;
     int64_t maxneed = 0;
+    if (stage_ev_rec_) GCI_HIP(hipEventSynchronize(stage_ev_));  // the last admission's staging copies
     std::vector<double *> wbasis(specs.size(), nullptr);  // each problem's workspace (basis first)
     GCI_REQUIRE(specs.size() <= (size_t)cap_, "davidson: more problems than slots");
     // the FP64 operator on a split matrix needs its layout rebuilt with the
@@ -1665,14 +1712,29 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
                 GCI_HIP(hipMemcpyAsync(wbasis[q], init[q], (size_t)specs[q].h->n * niv[q] * sizeof(double),
                                        hipMemcpyHostToDevice, stream_));
     }
+    bool wide = false, given = false;
     if (!unit_init.empty()) {
         std::fill(hwant_pin_, hwant_pin_ + cap_, 0);
-        for (int sl : unit_init) hwant_pin_[sl] = 1;
+        for (int sl : unit_init) {
+            hwant_pin_[sl] = 1;
+            wide |= niv_[sl] > MAXNUME;
+        }
         GCI_HIP(hipMemcpyAsync(dwant_.p, hwant_pin_, cap_ * sizeof(int), hipMemcpyHostToDevice, stream_));
-        launch(k_init_unit, cap_, 256, stream_, dprob_.p, (const int *)dwant_.p);
     }
-    GCI_HIP(hipStreamSynchronize(stream_));  // the pinned staging is reused by the next admission
-    rebuild_chunks();
+    for (const double *v : init) given |= v != nullptr;
+    // the pinned staging is rewritten by the next admission once these
+    // copies have completed; caller-given start vectors are read before
+    // returning
+    GCI_HIP(hipEventRecord(stage_ev_, stream_));
+    stage_ev_rec_ = true;
+    if (given) GCI_HIP(hipStreamSynchronize(stream_));
+    rebuild_chunks();  // the chunk tables the unit-vector selection walks
+    if (!unit_init.empty()) {
+        launch(k_unit_chunk, (int)total_chunks_, 256, stream_, (const DvdProb *)dprob_.p,
+               (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p, (const int *)dwant_.p, partial_.p);
+        launch(k_unit_merge, cap_, 256, stream_, dprob_.p, (const int *)dwant_.p, (const double *)partial_.p);
+        if (wide) launch(k_unit_wide, cap_, 256, stream_, dprob_.p, (const int *)dwant_.p);
+    }
     run_initdvd(slots);
     return slots;
 }
