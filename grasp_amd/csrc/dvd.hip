@@ -687,10 +687,11 @@ __global__ __launch_bounds__(256) void k_precond(DvdProb *P, const int *chunk_pr
 //      2 = (basis[:,0:kpass+nnew], ab[:,kpass:+nnew])     (ADDS)
 //      3 = (ab[:,0:nume] - eigval*basis[:,0:nume]) norms  (exit residuals)
 // Sel 0-2 are tall-skinny A^T B products on the FP64 matrix cores: the
-// chunk is staged through LDS 64 rows at a time and each wave accumulates
-// 16 x 16 output tiles (16 columns of A x up to 16 of B) with
-// v_mfma_f64_16x16x4f64, 4 rows per instruction, rows in ascending order,
-// so every output is summed in the same order on every run.
+// chunk is staged through LDS 64 rows at a time; wave w accumulates rows
+// 16w..16w+15 of every staged tile into all 16 x 16 output tiles (16
+// columns of A x up to 16 of B) with v_mfma_f64_16x16x4f64, 4 rows per
+// instruction in ascending order, and the four waves' sums are added in
+// wave order, so every output is summed in the same order on every run.
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
@@ -710,9 +711,12 @@ __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob,
     if (sel != 3) {
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int nti = (ka + 15) >> 4;  // 16-column tiles of A (<= 5); B fits one tile (kb <= 16)
-        d4v acc[2];
-        acc[0] = d4v{0.0, 0.0, 0.0, 0.0};
-        acc[1] = acc[0];
+        constexpr int NT = MAXLIM / 16;
+        // every wave takes 16 of each 64-row tile's rows for all output
+        // tiles, so the chains of dependent MFMAs are 4 long per tile
+        d4v acc[NT];
+#pragma unroll
+        for (int it = 0; it < NT; ++it) acc[it] = d4v{0.0, 0.0, 0.0, 0.0};
         // the next tile's elements are loaded into registers while the
         // matrix cores work on the current one (256 threads, 64 rows)
         constexpr int QA = 64 * MAXLIM / 256, QB = 64 * MAXNUME / 256;
@@ -747,31 +751,36 @@ __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob,
             // operands: lane l holds A^T[i = l%16][r = l/16] and B[r = l/16][j = l%16]
             const int j = lane & 15;
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int it = wave + 4 * t;
-                if (it < nti) {
-                    const int i = it * 16 + (lane & 15);
-#pragma unroll 4
-                    for (int ks = 0; ks < 16; ++ks) {
-                        const int r = ks * 4 + (lane >> 4);
+            for (int ks = 0; ks < 4; ++ks) {
+                const int r = wave * 16 + ks * 4 + (lane >> 4);
+                const double bv = j < kb ? tb[r][j] : 0.0;
+#pragma unroll
+                for (int it = 0; it < NT; ++it) {
+                    if (it < nti) {
+                        const int i = it * 16 + (lane & 15);
                         const double av = i < ka ? ta[r][i] : 0.0;
-                        const double bv = j < kb ? tb[r][j] : 0.0;
-                        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
+                        acc[it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[it], 0, 0, 0);
                     }
                 }
             }
             __syncthreads();
         }
-        // result: lane l, element q holds D[l/16 + 4q][l%16]
+        // the four waves' sums (lane l, element q holds D[l/16 + 4q][l%16])
+        // added in wave order through LDS
+        double *red = &ta[0][0];  // 4 x NT x 4 x 64 doubles fit the 64 x (MAXLIM + 1) tile
+        static_assert(4 * NT * 4 * 64 <= 64 * (MAXLIM + 1), "k_gram: reduction scratch exceeds the tile");
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int it = wave + 4 * t;
-            if (it >= nti) continue;
+        for (int it = 0; it < NT; ++it)
+            if (it < nti)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int i = it * 16 + (lane >> 4) + 4 * q, jj = lane & 15;
-                if (i < ka && jj < kb) partial[(int64_t)blockIdx.x * PSTRIDE + i * kb + jj] = acc[t][q];
-            }
+                for (int q = 0; q < 4; ++q) red[((wave * NT + it) * 4 + q) * 64 + lane] = acc[it][q];
+        __syncthreads();
+        for (int o = threadIdx.x; o < nti * 256; o += 256) {
+            const int it = o >> 8, q = (o >> 6) & 3, l = o & 63;
+            double sum = 0.0;
+            for (int w = 0; w < 4; ++w) sum += red[((w * NT + it) * 4 + q) * 64 + l];
+            const int i = it * 16 + (l >> 4) + 4 * q, jj = l & 15;
+            if (i < ka && jj < kb) partial[(int64_t)blockIdx.x * PSTRIDE + i * kb + jj] = sum;
         }
         return;
     }
