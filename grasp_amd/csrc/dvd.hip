@@ -178,14 +178,14 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
     }
     __syncthreads();
     const double tnorm = tnorm_s, pivmin = pivmin_s;
-    // --- multisection: 32 lanes per eigenvalue, 33-fold shrink per round
-    for (int m0 = 0; m0 < nval; m0 += nt / 32) {
-        const int m = m0 + t / 32, l = t & 31;
+    // --- multisection: a 64-lane wave per eigenvalue, 65-fold shrink per round
+    for (int m0 = 0; m0 < nval; m0 += nt / 64) {
+        const int m = m0 + wave, l = lane;
         double lo = -tnorm - 2.0 * EPS * tnorm - pivmin, hi = tnorm + 2.0 * EPS * tnorm + pivmin;
         for (int it = 0; it < 16; ++it) {
             const bool done = hi - lo <= 2.0 * EPS * fmax(fabs(lo), fabs(hi)) + 2.0 * pivmin;
-            if (__ballot(!done) == 0) break;  // both eigenvalues of this wavefront found
-            const double x = lo + (hi - lo) * (double)(l + 1) / 33.0;
+            if (__ballot(!done) == 0) break;  // this wavefront's eigenvalue found
+            const double x = lo + (hi - lo) * (double)(l + 1) / 65.0;
             int c = 0;
             double q = d[0] - x;
             if (fabs(q) < pivmin) q = -pivmin;
@@ -196,10 +196,9 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
                 c += q < 0.0;
             }
             // points with fewer than m + 1 eigenvalues below them bound m from below
-            const uint64_t below = __ballot(c <= m) >> (t & 32);
-            const int nb = __popcll(below & 0xFFFFFFFFull);
-            const double nlo = nb > 0 ? __shfl(x, (t & 32) + nb - 1, 64) : lo;
-            const double nhi = nb < 32 ? __shfl(x, (t & 32) + nb, 64) : hi;
+            const int nb = __popcll(__ballot(c <= m));
+            const double nlo = nb > 0 ? __shfl(x, nb - 1, 64) : lo;
+            const double nhi = nb < 64 ? __shfl(x, nb, 64) : hi;
             if (!done) {
                 lo = nlo;
                 hi = nhi;
@@ -585,6 +584,7 @@ __global__ __launch_bounds__(256) void k_newvec(DvdProb *P, const int *chunk_pro
         double xb[MAXNUME], xa[MAXNUME];
 #pragma unroll
         for (int t = 0; t < MAXNUME; ++t) { xb[t] = 0.0; xa[t] = 0.0; }
+#pragma unroll 4
         for (int i = 0; i < k; ++i) {
             const double b = p.basis[r + (int64_t)i * p.n];
             const double a = p.ab[r + (int64_t)i * p.n];
@@ -841,6 +841,7 @@ __global__ __launch_bounds__(256) void k_update(DvdProb *P, const int *chunk_pro
         double acc[MAXNUME];
 #pragma unroll
         for (int j = 0; j < MAXNUME; ++j) acc[j] = 0.0;
+#pragma unroll 4
         for (int i = 0; i < k; ++i) {
             const double a = p.basis[r + (int64_t)i * p.n];
 #pragma unroll
