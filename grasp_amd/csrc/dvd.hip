@@ -7,7 +7,7 @@
 //
 //   k_rr            Rayleigh-Ritz: cyclic parallel Jacobi on the packed
 //                   KPASS x KPASS projection S (LDS), sort, TSTSEL, OVFLOW test
-//   k_rotate        basis/AB rotation by the Ritz coefficients (OVFLOW, exit)
+//   k_rotate        basis/AB rotation by the Ritz coefficients (OVFLOW; k_rotate_exit at exit)
 //   k_newvec        Ritz vectors x = B s and residuals r = AB s - lambda x for
 //                   all candidates at once (tall-skinny GEMM) + norms
 //   k_newvec_decide NEWVEC's sequential accept/converge logic, robust shift
@@ -537,6 +537,41 @@ __global__ __launch_bounds__(256) void k_rotate(DvdProb *P, const int *chunk_pro
                 M[rt + rr + (int64_t)j * p.n] = acc;
             }
             __syncthreads();
+        }
+    }
+}
+
+// The exit rotation (mode 1 above) for problems in FINAL, which keep at most
+// NUME + 1 columns: one row per thread with the outputs in registers and the
+// same sums in the same order, so the small LDS leaves room for several
+// workgroups per CU to hide the loads' latency.
+__global__ __launch_bounds__(256) void k_rotate_exit(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row) {
+    DvdProb &p = P[chunk_prob[blockIdx.x]];
+    if (p.phase != PH_FINAL) return;
+    constexpr int KO = MAXNUME + 1;
+    __shared__ double sv[MAXLIM * KO];
+    const int kin = p.kpass;
+    const int kout = bulk_exit(p) ? min(p.kpass, p.nume + 1) : p.nume;
+    for (int i = threadIdx.x; i < kin * kout; i += blockDim.x) sv[i] = p.svec[i];
+    __syncthreads();
+    const int nrot = bulk_exit(p) ? 1 : 2;
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    for (int which = 0; which < nrot; ++which) {
+        double *M = which == 0 ? p.basis : p.ab;
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+            double acc[KO];
+#pragma unroll
+            for (int j = 0; j < KO; ++j) acc[j] = 0.0;
+#pragma unroll 4
+            for (int i = 0; i < kin; ++i) {
+                const double a = M[r + (int64_t)i * p.n];
+#pragma unroll
+                for (int j = 0; j < KO; ++j)
+                    if (j < kout) acc[j] = fma(a, sv[j * kin + i], acc[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < KO; ++j)
+                if (j < kout) M[r + (int64_t)j * p.n] = acc[j];
         }
     }
 }
@@ -1852,7 +1887,7 @@ void DavidsonEngine::launch_finalize() {
     const int G = (int)total_chunks_;
     const int *cp = chunk_prob_.p;
     const int64_t *cr = chunk_row_.p;
-    launch(k_rotate, G, 256, stream_, dprob_.p, cp, cr, 1);
+    launch(k_rotate_exit, G, 256, stream_, dprob_.p, cp, cr);
     launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 3, (int)PH_FINAL);
     launch(k_final_post, cap_, 256, stream_, dprob_.p, (const double *)partial_.p);
 }
