@@ -953,33 +953,25 @@ __global__ void k_chol(DvdProb *P, const double *partial, int want_phase, int pa
     else if (brk) p.ierr |= 1 << 20;  // orthogonalisation failed twice
 }
 
-// pseudo-random replacement for dropped columns (after the first pass)
-__global__ __launch_bounds__(256) void k_fill_random(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
-                                                     int want_phase) {
-    DvdProb &p = P[chunk_prob[blockIdx.x]];
-    if (p.phase != want_phase || !p.brk_mask) return;
-    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
-    for (int j = 0; j < p.nnew; ++j) {
-        if (!(p.brk_mask & (1 << j))) continue;
-        double *W = p.basis + (int64_t)(p.kpass + j) * p.n;
-        for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-            uint64_t x = (uint64_t)r * 0x9E3779B97F4A7C15ull ^ ((uint64_t)(p.nloops * 131 + j) << 32) ^ (uint64_t)p.slot;
-            x ^= x >> 33;
-            x *= 0xff51afd7ed558ccdull;
-            x ^= x >> 33;
-            x *= 0xc4ceb9fe1a85ec53ull;
-            x ^= x >> 33;
-            W[r] = (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-        }
-    }
+// pseudo-random replacement for a column dropped in the first pass, at row r
+__device__ inline double fill_random(const DvdProb &p, int64_t r, int j) {
+    uint64_t x = (uint64_t)r * 0x9E3779B97F4A7C15ull ^ ((uint64_t)(p.nloops * 131 + j) << 32) ^ (uint64_t)p.slot;
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
-// W = W R^{-1}
+// W = W R^{-1}; after the first pass, columns dropped by k_chol take a
+// pseudo-random direction instead (the second pass orthonormalises it)
 __global__ __launch_bounds__(256) void k_scale(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
-                                               int want_phase) {
+                                               int want_phase, int pass) {
     DvdProb &p = P[chunk_prob[blockIdx.x]];
     if (p.phase != want_phase) return;
     const int nn = p.nnew;
+    const int brk = pass == 0 ? p.brk_mask : 0;
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
     double *W = p.basis + (int64_t)p.kpass * p.n;
     for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
@@ -991,7 +983,7 @@ __global__ __launch_bounds__(256) void k_scale(DvdProb *P, const int *chunk_prob
             if (j < nn) {
                 double s = 0.0;
                 for (int i = 0; i <= j; ++i) s = fma(w[i], p.rinv[i * nn + j], s);
-                W[r + (int64_t)j * p.n] = s;
+                W[r + (int64_t)j * p.n] = (brk >> j) & 1 ? fill_random(p, r, j) : s;
             }
         }
     }
@@ -1829,8 +1821,7 @@ void DavidsonEngine::launch_orth(int passes, int phase) {
         launch(k_update, G, 256, stream_, dprob_.p, cp, cr, phase);
         launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 1, phase);
         launch(k_chol, cap_, 64, stream_, dprob_.p, (const double *)partial_.p, phase, pass);
-        launch(k_scale, G, 256, stream_, dprob_.p, cp, cr, phase);
-        if (pass == 0) launch(k_fill_random, G, 256, stream_, dprob_.p, cp, cr, phase);
+        launch(k_scale, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
     }
 }
 
