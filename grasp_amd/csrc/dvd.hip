@@ -61,27 +61,39 @@ __device__ inline double block_sum(double v, double *sh) {
 // out[o] = sum over the problem's chunks of partial[chunk][o], o < nout.
 // The block is split into G groups of nout threads (G = blockDim / nout,
 // at most the chunk count); group g sums chunks g, g + G, ... and the group
-// sums are added in group order, so every output is summed in one fixed
-// order.  Called by every thread of the block; ends with a barrier.
+// sums are added in group order.  Within a group the chunks are dealt in
+// turn to RU running sums (so RU loads are in flight), which are added in
+// order at the end: every output is summed in one fixed order.  Called by
+// every thread of the block; ends with a barrier.
+constexpr int RU = 8;
+__device__ inline double sum_chunks(const DvdProb &p, const double *partial, int o, int g, int G) {
+    double s[RU];
+#pragma unroll
+    for (int u = 0; u < RU; ++u) s[u] = 0.0;
+    for (int c = g; c < p.nchunks; c += RU * G) {
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            const int cc = c + u * G;
+            if (cc < p.nchunks) s[u] += partial[(p.chunk0 + cc) * PSTRIDE + o];
+        }
+    }
+    double r = s[0];
+#pragma unroll
+    for (int u = 1; u < RU; ++u) r += s[u];
+    return r;
+}
+
 __device__ void reduce_partials(const DvdProb &p, const double *partial, int nout, double *out) {
     __shared__ double red[256];
     const int nt = blockDim.x, t = threadIdx.x;
     const int G = max(1, min(min(nt, 256) / max(nout, 1), p.nchunks));
     if (G == 1) {
-        for (int o = t; o < nout; o += nt) {
-            double s = 0.0;
-            for (int c = 0; c < p.nchunks; ++c) s += partial[(p.chunk0 + c) * PSTRIDE + o];
-            out[o] = s;
-        }
+        for (int o = t; o < nout; o += nt) out[o] = sum_chunks(p, partial, o, 0, 1);
         __syncthreads();
         return;
     }
     const int o = t % nout, g = t / nout;
-    if (g < G) {
-        double s = 0.0;
-        for (int c = g; c < p.nchunks; c += G) s += partial[(p.chunk0 + c) * PSTRIDE + o];
-        red[g * nout + o] = s;
-    }
+    if (g < G) red[g * nout + o] = sum_chunks(p, partial, o, g, G);
     __syncthreads();
     if (t < nout) {
         double s = 0.0;
@@ -1820,7 +1832,7 @@ void DavidsonEngine::launch_orth(int passes, int phase) {
         launch(k_coef, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, phase);
         launch(k_update, G, 256, stream_, dprob_.p, cp, cr, phase);
         launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 1, phase);
-        launch(k_chol, cap_, 64, stream_, dprob_.p, (const double *)partial_.p, phase, pass);
+        launch(k_chol, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, phase, pass);
         launch(k_scale, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
     }
 }
