@@ -5,18 +5,20 @@
 // MGS_NRM :1417-1508) and the GDVD preconditioner (libdvd90/gdvd.f90:93-134)
 // as a sequence of device kernels over an array of problems:
 //
-//   k_rr            Rayleigh-Ritz: cyclic parallel Jacobi on the packed
-//                   KPASS x KPASS projection S (LDS), sort, TSTSEL, OVFLOW test
+//   k_rr            Rayleigh-Ritz on the packed KPASS x KPASS projection S
+//                   (LDS): DSPEVX's steps below LIM, cyclic parallel Jacobi
+//                   at LIM; sort, TSTSEL, OVFLOW test
 //   k_rotate        basis/AB rotation by the Ritz coefficients (OVFLOW; k_rotate_exit at exit)
 //   k_newvec        Ritz vectors x = B s and residuals r = AB s - lambda x for
 //                   all candidates at once (tall-skinny GEMM) + norms
 //   k_newvec_decide NEWVEC's sequential accept/converge logic, robust shift
 //   k_precond       (r - eps x) / (D - lambda) with GDVD's 1e-6 guard and its
 //                   in-place DIAG drift, reproduced exactly
-//   k_gram/k_update two-pass Gram-Schmidt against the basis (classical GS
-//   k_chol/k_scale  twice) and Cholesky-QR among the new vectors
-//   k_spmm_batch    the operator on the new vectors (all problems in one launch)
-//   k_gram + k_adds new columns of S = B^T AB_new
+//   k_gram/k_reduce two-pass Gram-Schmidt against the basis (classical GS
+//   k_update        twice) and Cholesky-QR among the new vectors
+//   k_chol/k_scale
+//   k_sell          the operator on the new vectors (all problems in one launch)
+//   k_gram/k_reduce new columns of S = B^T AB_new, then k_adds' bookkeeping
 //
 // Every kernel reads the per-problem state from device memory and skips
 // problems that are not in the stage, so one iteration over many problems
@@ -866,11 +868,33 @@ __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob,
 
 // Sum chunk partials of problem blockIdx.x into out (fixed chunk order).
 
-// projection coefficients for the classical Gram-Schmidt pass
-__global__ void k_coef(DvdProb *P, const double *partial, int want_phase) {
+// Fixed-order sums of the chunk partials where a problem has many outputs:
+// sel 0, the classical Gram-Schmidt coefficients (KPASS x NNEW) into COEF;
+// sel 1, ADDS's new columns of S ((KPASS + NNEW) x NNEW, upper triangle into
+// packed S, dvdson.f90:804-817).  Grid (problems, NRB): block y takes 64
+// outputs and its four waves a quarter of the chunks each (chunks w, w + 4,
+// ...), added in wave order, so a problem's outputs spread over up to NRB
+// workgroups.
+constexpr int NRB = (PSTRIDE + 63) / 64;
+__global__ __launch_bounds__(256) void k_reduce(DvdProb *P, const double *partial, int want_phase, int sel) {
     DvdProb &p = P[blockIdx.x];
     if (p.phase != want_phase) return;
-    reduce_partials(p, partial, p.kpass * p.nnew, p.coef);
+    const int k = p.kpass, nn = p.nnew;
+    const int nout = sel == 0 ? k * nn : (k + nn) * nn;
+    const int o0 = blockIdx.y * 64;
+    if (o0 >= nout) return;
+    __shared__ double red[4][64];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6, o = o0 + l;
+    red[w][l] = o < nout ? sum_chunks(p, partial, o, w, 4) : 0.0;
+    __syncthreads();
+    if (w != 0 || o >= nout) return;
+    const double v = ((red[0][l] + red[1][l]) + red[2][l]) + red[3][l];
+    if (sel == 0) {
+        p.coef[o] = v;
+    } else {
+        const int i = o / nn, j = k + o % nn;  // absolute column j
+        if (i <= j) p.s[i + j * (j + 1) / 2] = v;
+    }
 }
 
 // B[r, j] -= sum_i A[r, i] coef[i, j]
@@ -1002,19 +1026,10 @@ __global__ __launch_bounds__(256) void k_scale(DvdProb *P, const int *chunk_prob
 }
 
 // ADDS: new columns of packed S, then KPASS += NNEW, NLOOPS += 1 (dvdson.f90:804-817)
-__global__ void k_adds(DvdProb *P, const double *partial, int want_phase, int count_loop) {
+__global__ void k_adds(DvdProb *P, int want_phase, int count_loop) {
     DvdProb &p = P[blockIdx.x];
     if (p.phase != want_phase) return;
-    __shared__ double G[MAXLIM * MAXNUME];
-    const int k = p.kpass, nn = p.nnew;
-    reduce_partials(p, partial, (k + nn) * nn, G);
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < (k + nn) * nn; idx += blockDim.x) {
-        const int i = idx / nn, iv = idx % nn;
-        const int j = k + iv;  // absolute column
-        if (i <= j) p.s[i + j * (j + 1) / 2] = G[idx];
-    }
-    __syncthreads();
+    const int k = p.kpass, nn = p.nnew;  // S's new columns came from k_reduce
     if (threadIdx.x != 0) return;
     p.kpass = k + nn;
     p.nmv += nn;
@@ -1452,6 +1467,11 @@ void launch(K kern, int grid, int block, hipStream_t s, Args... args) {
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, args...);
 }
 
+// the many-output partial sums of problems in `phase` (k_reduce, sel 0 or 1)
+void launch_reduce(DvdProb *P, int cap, const double *partial, int phase, int sel, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)cap, (unsigned)NRB), dim3(256), 0, s, P, partial, phase, sel);
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------- host
@@ -1814,7 +1834,8 @@ void DavidsonEngine::run_initdvd(const std::vector<int> &slots) {
     launch(k_init_stage, pg, 64, stream_, dprob_.p, cap_, -1);
     launch(k_gram, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p,
            partial_.p, 2, (int)PH_INIT);
-    launch(k_adds, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, (int)PH_INIT, 0);
+    launch_reduce(dprob_.p, cap_, (const double *)partial_.p, (int)PH_INIT, 1, stream_);
+    launch(k_adds, cap_, 64, stream_, dprob_.p, (int)PH_INIT, 0);
     launch(k_init_done, pg, 64, stream_, dprob_.p, cap_);
     GCI_HIP(hipGetLastError());
     for (int sl : slots) {
@@ -1829,7 +1850,7 @@ void DavidsonEngine::launch_orth(int passes, int phase) {
     const int64_t *cr = chunk_row_.p;
     for (int pass = 0; pass < passes; ++pass) {
         launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 0, phase);
-        launch(k_coef, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, phase);
+        launch_reduce(dprob_.p, cap_, (const double *)partial_.p, phase, 0, stream_);
         launch(k_update, G, 256, stream_, dprob_.p, cp, cr, phase);
         launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 1, phase);
         launch(k_chol, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, phase, pass);
@@ -1908,7 +1929,8 @@ void DavidsonEngine::launch_iteration() {
     launch_orth(2, PH_ITER);
     launch_op(PH_ITER, mwidth_iter_);
     launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 2, (int)PH_ITER);
-    launch(k_adds, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, (int)PH_ITER, 1);
+    launch_reduce(dprob_.p, cap_, (const double *)partial_.p, (int)PH_ITER, 1, stream_);
+    launch(k_adds, cap_, 64, stream_, dprob_.p, (int)PH_ITER, 1);
     launch_finalize();
     launch(k_snapshot, 1, 256, stream_, (const DvdProb *)dprob_.p, cap_, dseq_.p, dsnap_host_);
 }
