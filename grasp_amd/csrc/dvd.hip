@@ -462,6 +462,20 @@ __global__ __launch_bounds__(512) void k_rr(DvdProb *P) {
         }
         __syncthreads();
     }
+    // TSTSEL reads LDS copies of the Ritz values and of the coefficients it
+    // tests (one parallel load instead of thread 0's chain of global reads);
+    // A is free once the Ritz pairs are written
+    __shared__ double evl[MAXNUME], ovl[MAXNUME];
+    double *svl = &A[0][0];  // k x NUME, ld k (k NUME <= MAXLIM MAXNUME)
+    {
+        const int nu = p.nume;
+        for (int idx = t; idx < k * nu; idx += nt) svl[idx] = p.svec[idx];
+        for (int i = t; i < nu; i += nt) {
+            evl[i] = p.eigval[i];
+            ovl[i] = p.oldval[i];
+        }
+    }
+    __syncthreads();
     if (t != 0) return;
     // TSTSEL (dvdson.f90:1328-1411); ISELEC = 1..NUME.  A CRITE below the
     // rounding unit of the eigenvalue (MANEIG's 1e-17) can only be met by a
@@ -470,10 +484,11 @@ __global__ __launch_bounds__(512) void k_rr(DvdProb *P) {
     // DSPEVX's tridiagonal rounding moves them, so such a CRITE is not
     // taken as met.
     const int nume = p.nume;
+    const double crite = p.crite, critc = p.critc;
     int nnce = 0;
     for (int i = 0; i < nume; ++i) {
-        const bool resolvable = p.crite >= 2.220446049250313e-16 * fabs(p.eigval[i]);
-        if (!(resolvable && fabs(p.oldval[i] - p.eigval[i]) < p.crite)) ++nnce;
+        const bool resolvable = crite >= 2.220446049250313e-16 * fabs(evl[i]);
+        if (!(resolvable && fabs(ovl[i] - evl[i]) < crite)) ++nnce;
     }
     if (nnce == 0) {
         p.phase = PH_FINAL;
@@ -482,11 +497,12 @@ __global__ __launch_bounds__(512) void k_rr(DvdProb *P) {
     int icnt = 0;
     int ind[MAXNUME];
     double rowlast[MAXNUME];
+    const int nncv = p.nncv;
     for (int i = 0; i < nume; ++i) {
         const int icur = k * (i + 1) - 1;
-        double tmax = fabs(p.svec[icur]);
-        for (int l = 1; l <= p.nncv - 1; ++l) tmax = fmax(tmax, fabs(p.svec[icur - l]));
-        if (tmax < p.critc) {
+        double tmax = fabs(svl[icur]);
+        for (int l = 1; l <= nncv - 1; ++l) tmax = fmax(tmax, fabs(svl[icur - l]));
+        if (tmax < critc) {
             p.icv[i] = 1;
         } else {
             p.icv[i] = 0;
@@ -665,18 +681,29 @@ __global__ void k_newvec_decide(DvdProb *P, const double *partial) {
     DvdProb &p = P[blockIdx.x];
     if (p.phase != PH_ITER) return;
     __shared__ double nrm2[MAXNUME];
+    // LDS copies of what the sequential decision below reads (one parallel
+    // load instead of thread 0's chain of global reads); the candidates'
+    // INCV entries are read before any of them is overwritten
+    __shared__ double evl[MAXLIM], ovl[MAXLIM];
+    __shared__ int icl[MAXNUME];
     const int nc = p.ncand, k = p.kpass;
-    reduce_partials(p, partial, nc, nrm2);
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        evl[i] = p.eigval[i];
+        ovl[i] = p.oldval[i];
+    }
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) icl[i] = p.incv[i];
+    reduce_partials(p, partial, nc, nrm2);  // ends with a barrier
     if (threadIdx.x != 0) return;
     const int limadd = min(p.lim, p.mblock + k);
+    const double tol_rel = p.tol_rel, eshift = p.eshift, critr = p.critr;
     int nadded = 0, icvc = 0;
     for (int t = 0; t < nc; ++t) {
         const double s = nrm2[t];
-        const int indx = p.incv[t];
+        const int indx = icl[t];
         const double res = sqrt(s);
         p.resn[indx - 1] = res;
-        const double lam = p.eigval[indx - 1];
-        const double thr = p.tol_rel > 0.0 ? p.tol_rel * fabs(lam + p.eshift) : p.critr;
+        const double lam = evl[indx - 1];
+        const double thr = tol_rel > 0.0 ? tol_rel * fabs(lam + eshift) : critr;
         if (res < thr) {
             ++icvc;
             p.icv[indx - 1] = 1;
@@ -687,20 +714,20 @@ __global__ void k_newvec_decide(DvdProb *P, const double *partial) {
         }
         double gap;
         if (indx == 1) {
-            gap = fabs(p.eigval[1] - p.eigval[0]);
+            gap = fabs(evl[1] - evl[0]);
         } else {
-            const double gl = fabs(p.eigval[indx - 1] - p.eigval[indx - 2]);
-            const double gu = fabs(p.eigval[indx] - p.eigval[indx - 1]);
+            const double gl = fabs(evl[indx - 1] - evl[indx - 2]);
+            const double gu = fabs(evl[indx] - evl[indx - 1]);
             gap = fmin(gl, gu);
         }
-        const double dl = fabs(p.oldval[indx - 1] - p.eigval[indx - 1]);
+        const double dl = fabs(ovl[indx - 1] - evl[indx - 1]);
         double rhs;
         if (gap > res) rhs = sqrt(dl * res);
         else rhs = fmin(dl, sqrt(dl));
         p.epsil[indx - 1] = 0.0;
         p.slot_cand[nadded] = t;
         p.rhseps[nadded] = rhs;
-        p.lam_add[nadded] = lam - p.epsil[indx - 1];
+        p.lam_add[nadded] = lam - 0.0;  // EPSIL(INDX) was just zeroed
         p.incv[nadded] = indx;
         ++nadded;
         if (nadded + k == limadd) break;
