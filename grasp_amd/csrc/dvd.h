@@ -222,6 +222,7 @@ class DavidsonEngine {
     int64_t nitems_ = 0;
     int modes_used_ = 0;           // bitmask of operator modes over active slots
     int mwidth_iter_ = 1;          // launch width for iterations
+    int nume_max_ = MAXNUME;       // largest NUME over active slots (k_newvec's accumulators)
     int64_t total_chunks_ = 0;
     std::vector<int> hprob_;
     std::vector<int64_t> hrow_;

@@ -626,6 +626,9 @@ __global__ void k_ovf_fix(DvdProb *P, int nprob) {
 }
 
 // ------------------------------------------------------------- NEWVEC
+// NC: the largest candidate count of the launch (NUME <= NC for every
+// problem), which sizes the per-row accumulators and so the occupancy
+template <int NC>
 __global__ __launch_bounds__(256) void k_newvec(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
                                                 double *partial) {
     const int pi = chunk_prob[blockIdx.x];
@@ -642,19 +645,19 @@ __global__ __launch_bounds__(256) void k_newvec(DvdProb *P, const int *chunk_pro
     if (threadIdx.x < nc) lam[threadIdx.x] = p.eigval[p.incv[threadIdx.x] - 1];
     __syncthreads();
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
-    double nrm[MAXNUME];
+    double nrm[NC];
 #pragma unroll
-    for (int t = 0; t < MAXNUME; ++t) nrm[t] = 0.0;
+    for (int t = 0; t < NC; ++t) nrm[t] = 0.0;
     for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        double xb[MAXNUME], xa[MAXNUME];
+        double xb[NC], xa[NC];
 #pragma unroll
-        for (int t = 0; t < MAXNUME; ++t) { xb[t] = 0.0; xa[t] = 0.0; }
+        for (int t = 0; t < NC; ++t) { xb[t] = 0.0; xa[t] = 0.0; }
 #pragma unroll 4
         for (int i = 0; i < k; ++i) {
             const double b = p.basis[r + (int64_t)i * p.n];
             const double a = p.ab[r + (int64_t)i * p.n];
 #pragma unroll
-            for (int t = 0; t < MAXNUME; ++t) {
+            for (int t = 0; t < NC; ++t) {
                 if (t < nc) {
                     xb[t] = fma(b, sv[t][i], xb[t]);
                     xa[t] = fma(a, sv[t][i], xa[t]);
@@ -662,7 +665,7 @@ __global__ __launch_bounds__(256) void k_newvec(DvdProb *P, const int *chunk_pro
             }
         }
 #pragma unroll
-        for (int t = 0; t < MAXNUME; ++t) {
+        for (int t = 0; t < NC; ++t) {
             if (t < nc) {
                 const double rr = fma(-lam[t], xb[t], xa[t]);
                 p.xc[r + (int64_t)t * p.n] = xb[t];
@@ -671,7 +674,9 @@ __global__ __launch_bounds__(256) void k_newvec(DvdProb *P, const int *chunk_pro
             }
         }
     }
-    for (int t = 0; t < nc; ++t) {
+#pragma unroll
+    for (int t = 0; t < NC; ++t) {
+        if (t >= nc) break;
         const double s = block_sum(nrm[t], red);
         if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * PSTRIDE + t] = s;
     }
@@ -1886,18 +1891,21 @@ void DavidsonEngine::launch_orth(int passes, int phase) {
 }
 
 void DavidsonEngine::update_modes() {
-    int modes = 0, mw = 1;
+    int modes = 0, mw = 1, nmax = 1;
     for (int i = 0; i < cap_; ++i) {
         if (phase_[i] == PH_EMPTY) continue;
         const DvdSpec &sp = spec_[i];
+        nmax = std::max(nmax, sp.nume);
         if (sp.mode == GCI_OP_FP64) modes |= 1 << GCI_OP_FP64;
         else if (refined_[i]) modes |= 1 << GCI_OP_SPLIT_FULL;
         else modes |= 1 << sp.mode;
         mw = std::max(mw, sp.mblock);
     }
-    if (modes != modes_used_ || mw != mwidth_iter_ || spmm_shape_version() != shape_version_) graph_valid_ = false;
+    if (modes != modes_used_ || mw != mwidth_iter_ || nmax != nume_max_ || spmm_shape_version() != shape_version_)
+        graph_valid_ = false;
     modes_used_ = modes;
     mwidth_iter_ = mw;
+    nume_max_ = nmax;
     shape_version_ = spmm_shape_version();
 }
 
@@ -1950,7 +1958,8 @@ void DavidsonEngine::launch_iteration() {
     launch(k_rr, cap_, 512, stream_, dprob_.p);
     launch(k_rotate, G, 256, stream_, dprob_.p, cp, cr, 0);
     launch(k_ovf_fix, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_);
-    launch(k_newvec, G, 256, stream_, dprob_.p, cp, cr, partial_.p);
+    if (nume_max_ <= 8) launch(k_newvec<8>, G, 256, stream_, dprob_.p, cp, cr, partial_.p);
+    else launch(k_newvec<MAXNUME>, G, 256, stream_, dprob_.p, cp, cr, partial_.p);
     launch(k_newvec_decide, cap_, 256, stream_, dprob_.p, (const double *)partial_.p);
     launch(k_precond, G, 256, stream_, dprob_.p, cp, cr);
     launch_orth(2, PH_ITER);
