@@ -60,6 +60,7 @@ struct DvdProb {
     int refine_loops;
     int bulk_loops, bulk_nmv;       // iterations/products of the bulk phase
     int brk_mask;                   // new columns that lost rank in the first GS pass
+    int skip2;                      // the first GS pass kept every column's norm (DGKS): no second pass
     int init_ist, init_iend;        // INITDVD progress: next AB column to copy, columns built
     int init_src;                   // AB column copied by the current INITDVD sub-step
     // --- small arrays (1-based reference arrays stored 0-based)

@@ -776,9 +776,9 @@ __global__ __launch_bounds__(256) void k_precond(DvdProb *P, const int *chunk_pr
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
-                                              double *partial, int sel, int want_phase) {
+                                              double *partial, int sel, int want_phase, int pass2) {
     DvdProb &p = P[chunk_prob[blockIdx.x]];
-    if (p.phase != want_phase || (sel == 3 && bulk_exit(p))) return;
+    if (p.phase != want_phase || (sel == 3 && bulk_exit(p)) || (pass2 && p.skip2)) return;
     __shared__ double ta[64][MAXLIM + 1];
     __shared__ double tb[64][MAXNUME + 1];
     const int k = p.kpass, nn = p.nnew;
@@ -908,9 +908,10 @@ __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob,
 // ...), added in wave order, so a problem's outputs spread over up to NRB
 // workgroups.
 constexpr int NRB = (PSTRIDE + 63) / 64;
-__global__ __launch_bounds__(256) void k_reduce(DvdProb *P, const double *partial, int want_phase, int sel) {
+__global__ __launch_bounds__(256) void k_reduce(DvdProb *P, const double *partial, int want_phase, int sel,
+                                                int pass2) {
     DvdProb &p = P[blockIdx.x];
-    if (p.phase != want_phase) return;
+    if (p.phase != want_phase || (pass2 && p.skip2)) return;
     const int k = p.kpass, nn = p.nnew;
     const int nout = sel == 0 ? k * nn : (k + nn) * nn;
     const int o0 = blockIdx.y * 64;
@@ -931,9 +932,9 @@ __global__ __launch_bounds__(256) void k_reduce(DvdProb *P, const double *partia
 
 // B[r, j] -= sum_i A[r, i] coef[i, j]
 __global__ __launch_bounds__(256) void k_update(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
-                                                int want_phase) {
+                                                int want_phase, int pass2) {
     DvdProb &p = P[chunk_prob[blockIdx.x]];
-    if (p.phase != want_phase) return;
+    if (p.phase != want_phase || (pass2 && p.skip2)) return;
     __shared__ double cf[MAXLIM * MAXNUME];
     const int k = p.kpass, nn = p.nnew;
     for (int i = threadIdx.x; i < k * nn; i += blockDim.x) cf[i] = p.coef[i];
@@ -966,7 +967,7 @@ __global__ __launch_bounds__(256) void k_update(DvdProb *P, const int *chunk_pro
 // orthonormalises.
 __global__ void k_chol(DvdProb *P, const double *partial, int want_phase, int pass) {
     DvdProb &p = P[blockIdx.x];
-    if (p.phase != want_phase) return;
+    if (p.phase != want_phase || (pass == 1 && p.skip2)) return;
     __shared__ double G[MAXNUME * MAXNUME];
     __shared__ double R[MAXNUME][MAXNUME + 1];
     __shared__ double w0[MAXNUME];  // squared norms of the new columns before this pass's projection
@@ -1017,8 +1018,18 @@ __global__ void k_chol(DvdProb *P, const double *partial, int want_phase, int pa
         }
     }
     for (int i = 0; i < nn * nn; ++i) p.rinv[i] = Ri[i];
-    if (pass == 0) p.brk_mask = brk;
-    else if (brk) p.ierr |= 1 << 20;  // orthogonalisation failed twice
+    if (pass == 0) {
+        p.brk_mask = brk;
+        // DGKS: when every new column kept at least half its squared norm
+        // through the projections (basis and earlier new columns) and none
+        // broke down, one pass leaves the block orthonormal to a few units
+        // of rounding and the second pass is skipped
+        int keep = brk == 0;
+        for (int j = 0; j < nn; ++j) keep &= R[j][j] * R[j][j] >= 0.5 * w0[j];
+        p.skip2 = keep;
+    } else if (brk) {
+        p.ierr |= 1 << 20;  // orthogonalisation failed twice
+    }
 }
 
 // pseudo-random replacement for a column dropped in the first pass, at row r
@@ -1037,7 +1048,7 @@ __device__ inline double fill_random(const DvdProb &p, int64_t r, int j) {
 __global__ __launch_bounds__(256) void k_scale(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
                                                int want_phase, int pass) {
     DvdProb &p = P[chunk_prob[blockIdx.x]];
-    if (p.phase != want_phase) return;
+    if (p.phase != want_phase || (pass == 1 && p.skip2)) return;
     const int nn = p.nnew;
     const int brk = pass == 0 ? p.brk_mask : 0;
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
@@ -1500,8 +1511,8 @@ void launch(K kern, int grid, int block, hipStream_t s, Args... args) {
 }
 
 // the many-output partial sums of problems in `phase` (k_reduce, sel 0 or 1)
-void launch_reduce(DvdProb *P, int cap, const double *partial, int phase, int sel, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce, dim3((unsigned)cap, (unsigned)NRB), dim3(256), 0, s, P, partial, phase, sel);
+void launch_reduce(DvdProb *P, int cap, const double *partial, int phase, int sel, hipStream_t s, int pass2 = 0) {
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)cap, (unsigned)NRB), dim3(256), 0, s, P, partial, phase, sel, pass2);
 }
 
 }  // namespace
@@ -1865,7 +1876,7 @@ void DavidsonEngine::run_initdvd(const std::vector<int> &slots) {
     }
     launch(k_init_stage, pg, 64, stream_, dprob_.p, cap_, -1);
     launch(k_gram, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p,
-           partial_.p, 2, (int)PH_INIT);
+           partial_.p, 2, (int)PH_INIT, 0);
     launch_reduce(dprob_.p, cap_, (const double *)partial_.p, (int)PH_INIT, 1, stream_);
     launch(k_adds, cap_, 64, stream_, dprob_.p, (int)PH_INIT, 0);
     launch(k_init_done, pg, 64, stream_, dprob_.p, cap_);
@@ -1881,10 +1892,10 @@ void DavidsonEngine::launch_orth(int passes, int phase) {
     const int *cp = chunk_prob_.p;
     const int64_t *cr = chunk_row_.p;
     for (int pass = 0; pass < passes; ++pass) {
-        launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 0, phase);
-        launch_reduce(dprob_.p, cap_, (const double *)partial_.p, phase, 0, stream_);
-        launch(k_update, G, 256, stream_, dprob_.p, cp, cr, phase);
-        launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 1, phase);
+        launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 0, phase, pass);
+        launch_reduce(dprob_.p, cap_, (const double *)partial_.p, phase, 0, stream_, pass);
+        launch(k_update, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
+        launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 1, phase, pass);
         launch(k_chol, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, phase, pass);
         launch(k_scale, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
     }
@@ -1947,7 +1958,7 @@ void DavidsonEngine::launch_finalize() {
     const int *cp = chunk_prob_.p;
     const int64_t *cr = chunk_row_.p;
     launch(k_rotate_exit, G, 256, stream_, dprob_.p, cp, cr);
-    launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 3, (int)PH_FINAL);
+    launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 3, (int)PH_FINAL, 0);
     launch(k_final_post, cap_, 256, stream_, dprob_.p, (const double *)partial_.p);
 }
 
@@ -1964,7 +1975,7 @@ void DavidsonEngine::launch_iteration() {
     launch(k_precond, G, 256, stream_, dprob_.p, cp, cr);
     launch_orth(2, PH_ITER);
     launch_op(PH_ITER, mwidth_iter_);
-    launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 2, (int)PH_ITER);
+    launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 2, (int)PH_ITER, 0);
     launch_reduce(dprob_.p, cap_, (const double *)partial_.p, (int)PH_ITER, 1, stream_);
     launch(k_adds, cap_, 64, stream_, dprob_.p, (int)PH_ITER, 1);
     launch_finalize();
