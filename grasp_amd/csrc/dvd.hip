@@ -931,6 +931,7 @@ __global__ __launch_bounds__(256) void k_reduce(DvdProb *P, const double *partia
 }
 
 // B[r, j] -= sum_i A[r, i] coef[i, j]
+template <int NC>  // NNEW <= NC for every problem of the launch
 __global__ __launch_bounds__(256) void k_update(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
                                                 int want_phase, int pass2) {
     DvdProb &p = P[chunk_prob[blockIdx.x]];
@@ -942,18 +943,18 @@ __global__ __launch_bounds__(256) void k_update(DvdProb *P, const int *chunk_pro
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
     double *B = p.basis + (int64_t)k * p.n;
     for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        double acc[MAXNUME];
+        double acc[NC];
 #pragma unroll
-        for (int j = 0; j < MAXNUME; ++j) acc[j] = 0.0;
+        for (int j = 0; j < NC; ++j) acc[j] = 0.0;
 #pragma unroll 4
         for (int i = 0; i < k; ++i) {
             const double a = p.basis[r + (int64_t)i * p.n];
 #pragma unroll
-            for (int j = 0; j < MAXNUME; ++j)
+            for (int j = 0; j < NC; ++j)
                 if (j < nn) acc[j] = fma(a, cf[i * nn + j], acc[j]);
         }
 #pragma unroll
-        for (int j = 0; j < MAXNUME; ++j)
+        for (int j = 0; j < NC; ++j)
             if (j < nn) B[r + (int64_t)j * p.n] -= acc[j];
     }
 }
@@ -1045,6 +1046,7 @@ __device__ inline double fill_random(const DvdProb &p, int64_t r, int j) {
 
 // W = W R^{-1}; after the first pass, columns dropped by k_chol take a
 // pseudo-random direction instead (the second pass orthonormalises it)
+template <int NC>  // NNEW <= NC for every problem of the launch
 __global__ __launch_bounds__(256) void k_scale(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
                                                int want_phase, int pass) {
     DvdProb &p = P[chunk_prob[blockIdx.x]];
@@ -1054,11 +1056,11 @@ __global__ __launch_bounds__(256) void k_scale(DvdProb *P, const int *chunk_prob
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
     double *W = p.basis + (int64_t)p.kpass * p.n;
     for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        double w[MAXNUME];
+        double w[NC];
 #pragma unroll
-        for (int j = 0; j < MAXNUME; ++j) w[j] = (j < nn) ? W[r + (int64_t)j * p.n] : 0.0;
+        for (int j = 0; j < NC; ++j) w[j] = (j < nn) ? W[r + (int64_t)j * p.n] : 0.0;
 #pragma unroll
-        for (int j = 0; j < MAXNUME; ++j) {
+        for (int j = 0; j < NC; ++j) {
             if (j < nn) {
                 double s = 0.0;
                 for (int i = 0; i <= j; ++i) s = fma(w[i], p.rinv[i * nn + j], s);
@@ -1894,10 +1896,12 @@ void DavidsonEngine::launch_orth(int passes, int phase) {
     for (int pass = 0; pass < passes; ++pass) {
         launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 0, phase, pass);
         launch_reduce(dprob_.p, cap_, (const double *)partial_.p, phase, 0, stream_, pass);
-        launch(k_update, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
+        if (nume_max_ <= 8) launch(k_update<8>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
+        else launch(k_update<MAXNUME>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
         launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 1, phase, pass);
         launch(k_chol, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, phase, pass);
-        launch(k_scale, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
+        if (nume_max_ <= 8) launch(k_scale<8>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
+        else launch(k_scale<MAXNUME>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
     }
 }
 
