@@ -77,6 +77,7 @@ struct DvdProb {
     double lam_add[MAXNUME];
     double coef[MAXLIM * MAXNUME];
     double rinv[MAXNUME * MAXNUME];
+    int unit_row[MAXNUME];          // rows of the unit start vectors (k_unit_merge)
 };
 
 // Unit start vectors of a batched or served solve.  NUME + 1 of them replace
@@ -158,7 +159,9 @@ class DavidsonEngine {
     void launch_orth(int passes, int phase);
     void launch_op(int phase, int M, int modes = -1);  // modes: mask of operator modes (-1: all in use)
     void update_modes();
-    void run_initdvd(const std::vector<int> &slots);
+    // unit_op: every starting problem has unit start vectors (k_unit_merge),
+    // whose first product is read from H's columns instead of an SpMM pass
+    void run_initdvd(const std::vector<int> &slots, bool unit_op = false);
     void drain_profile();
     struct ProfEv {
         int mode;

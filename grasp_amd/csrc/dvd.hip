@@ -1413,6 +1413,51 @@ __global__ __launch_bounds__(256) void k_unit_merge(DvdProb *P, const int *want,
     top_merge_block(niv, tv, ti, bv, bi, ov, oi);
     const int j = threadIdx.x;
     if (j < niv && oi[j] >= 0) p.basis[oi[j] + (int64_t)j * p.n] = 1.0;
+    if (j < niv) p.unit_row[j] = (int)oi[j];
+}
+
+// position of column c in the off-diagonal row `row` (columns ascend), or -1
+__device__ inline int64_t find_col(const int64_t *rp, const int32_t *col, int64_t row, int64_t c) {
+    int64_t lo = rp[row], hi = rp[row + 1];
+    while (lo < hi) {
+        const int64_t m = (lo + hi) >> 1;
+        if (col[m] < c) lo = m + 1;
+        else hi = m;
+    }
+    return (lo < rp[row + 1] && col[lo] == c) ? lo : -1;
+}
+
+// INITDVD's first product for unit start vectors: AB[:, j] = H e_u (u the
+// j-th unit row) is column u of H, read from the off-diagonal rows by binary
+// search, with the value the problem's operator mode applies (fp32 hi for
+// the bulk operator, hi + lo for the split FP64 one, the FP64 value
+// otherwise) and the FP64 diagonal.  Each output of the operator on e_u is a
+// single term, so these equal the operator's results bit for bit, without a
+// pass over the whole matrix.
+__global__ __launch_bounds__(256) void k_unit_op(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
+                                                 const int *want) {
+    const int pi = chunk_prob[blockIdx.x];
+    const DvdProb &p = P[pi];
+    if (!want[pi] || p.phase != PH_INIT || p.niv > MAXNUME) return;
+    const int mode = op_mode(p);
+    const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
+    for (int j = 0; j < p.niv; ++j) {
+        const int64_t u = p.unit_row[j];
+        double *Y = p.ab + (int64_t)j * p.n;
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+            double y = 0.0;
+            if (r == u) {
+                y = p.diagH[r];
+            } else if (u >= 0) {
+                const int64_t t = r > u ? find_col(p.rp, p.col, r, u) : find_col(p.rp, p.col, u, r);
+                if (t >= 0)
+                    y = mode == GCI_OP_SPLIT_BULK   ? (double)p.hi[t]
+                        : mode == GCI_OP_SPLIT_FULL ? (double)p.hi[t] + (double)p.lo[t]
+                                                    : p.val[t];
+            }
+            Y[r] = y;
+        }
+    }
 }
 
 // NIV > MAXNUME: NIV argmin passes over the whole diagonal
@@ -1849,11 +1894,11 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         launch(k_unit_merge, cap_, 256, stream_, dprob_.p, (const int *)dwant_.p, (const double *)partial_.p);
         if (wide) launch(k_unit_wide, cap_, 256, stream_, dprob_.p, (const int *)dwant_.p);
     }
-    run_initdvd(slots);
+    run_initdvd(slots, !wide && !slots.empty() && unit_init.size() == slots.size());
     return slots;
 }
 
-void DavidsonEngine::run_initdvd(const std::vector<int> &slots) {
+void DavidsonEngine::run_initdvd(const std::vector<int> &slots, bool unit_op) {
     // INITDVD (dvdson.f90:865-946) for the problems in PH_INIT: OP on the
     // NIV initial vectors, then orthonormalised AB vectors up to NUME+1,
     // then S = B^T AB.  Each problem's sub-steps are derived on the device
@@ -1869,7 +1914,11 @@ void DavidsonEngine::run_initdvd(const std::vector<int> &slots) {
     }
     const int pg = (cap_ + 63) / 64;
     launch(k_init_stage, pg, 64, stream_, dprob_.p, cap_, 0);
-    launch_op(PH_INIT, init_width, modes);
+    if (unit_op)
+        launch(k_unit_op, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p,
+               (const int *)dwant_.p);
+    else
+        launch_op(PH_INIT, init_width, modes);
     for (int st = 1; st <= stages; ++st) {
         launch(k_init_stage, pg, 64, stream_, dprob_.p, cap_, st);
         launch(k_init_copy, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p);
