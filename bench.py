@@ -298,7 +298,7 @@ def main():
         achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(),
-                "kernel": "k_sell (panel-slice symmetric SpMM, bulk fp32 hi: 4-vector iterations and the 5-vector start)",
+                "kernel": "k_sell (panel-slice symmetric SpMM, bulk fp32 hi: the 4-vector iterations)",
                 "launches": int(launches[bulk]), "avg_ms": round(avg_ms, 4),
                 "alg_bytes_per_launch": round(avg_bytes),
                 "refine_fp64_achieved_GBs": round(byts[2] / (ms[2] * 1e-3) / 1e9, 1) if ms[2] > 0 else None}
