@@ -1434,17 +1434,22 @@ __device__ inline int64_t find_col(const int64_t *rp, const int32_t *col, int64_
 // otherwise) and the FP64 diagonal.  Each output of the operator on e_u is a
 // single term, so these equal the operator's results bit for bit, without a
 // pass over the whole matrix.
-__global__ __launch_bounds__(256) void k_unit_op(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
-                                                 const int *want) {
+__global__ __launch_bounds__(1024) void k_unit_op(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
+                                                  const int *want) {
     const int pi = chunk_prob[blockIdx.x];
     const DvdProb &p = P[pi];
     if (!want[pi] || p.phase != PH_INIT || p.niv > MAXNUME) return;
     const int mode = op_mode(p);
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
-    for (int j = 0; j < p.niv; ++j) {
+    const int rows = (int)(r1 - r0);
+    // one (row, vector) pair per thread: each is a chain of dependent loads
+    // (a binary search), so the pairs run side by side
+    for (int idx = threadIdx.x; idx < rows * p.niv; idx += blockDim.x) {
+        const int j = idx / rows;
+        const int64_t r = r0 + idx % rows;
         const int64_t u = p.unit_row[j];
         double *Y = p.ab + (int64_t)j * p.n;
-        for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        {
             double y = 0.0;
             if (r == u) {
                 y = p.diagH[r];
@@ -1915,7 +1920,7 @@ void DavidsonEngine::run_initdvd(const std::vector<int> &slots, bool unit_op) {
     const int pg = (cap_ + 63) / 64;
     launch(k_init_stage, pg, 64, stream_, dprob_.p, cap_, 0);
     if (unit_op)
-        launch(k_unit_op, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p,
+        launch(k_unit_op, G, 1024, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p,
                (const int *)dwant_.p);
     else
         launch_op(PH_INIT, init_width, modes);
