@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libgraspci.so")
 _LIB = None
 
 GCI_OK = 0
-GCI_ERR_BUSY = -6
+GCI_ERR_ARG, GCI_ERR_DEVICE, GCI_ERR_NOMEM, GCI_ERR_NOTFOUND, GCI_ERR_NOCONV, GCI_ERR_BUSY = -1, -2, -3, -4, -5, -6
 GCI_FP64, GCI_SPLIT32 = 0, 1
 GCI_OP_FP64, GCI_OP_SPLIT_BULK, GCI_OP_SPLIT_FULL = 0, 1, 2
 

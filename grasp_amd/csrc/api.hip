@@ -54,10 +54,13 @@ HMat *upload(int64_t n, const int64_t *iendc, const int32_t *irow, const double 
                                              std::to_string(i + 1));
             GCI_REQUIRE(c > prev, "hmat_upload: columns not ascending in row " + std::to_string(i + 1));
             prev = c;
+            GCI_REQUIRE(std::isfinite(emt[t]), "hmat_upload: non-finite element in row " + std::to_string(i + 1));
             col[k] = (int32_t)c;
             val[k] = emt[t];
             ++k;
         }
+        GCI_REQUIRE(std::isfinite(emt[e - 1]) && std::isfinite(shift),
+                    "hmat_upload: non-finite diagonal in row " + std::to_string(i + 1));
         diag[i] = emt[e - 1] + shift;
         rp[i + 1] = k;
     }

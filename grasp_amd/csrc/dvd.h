@@ -6,8 +6,13 @@
 
 namespace gci {
 
-constexpr int MAXLIM = 80;    // largest LIM (2*nvec+60 for nvec <= 10)
 constexpr int MAXNUME = 16;   // largest number of wanted eigenpairs
+// largest LIM: MANEIG's 2*NVECMX+60 for every NVECMX <= MAXNUME
+// (rci90/maneig.f90:389), rounded up to whole 16-column MFMA tiles
+constexpr int MAXLIM = 96;
+static_assert(MAXLIM >= 2 * MAXNUME + 60 && MAXLIM % 16 == 0, "MAXLIM must hold 2*MAXNUME+60 in 16-column tiles");
+// NINCOR of MANEIG (rci90/maneig.f90:93)
+constexpr int64_t NINCOR = 268435456;
 constexpr int CHUNK = 512;    // rows per tall-skinny work item
 constexpr int RESULT_STRIDE = 5 + 2 * MAXNUME;
 
@@ -150,6 +155,8 @@ class DavidsonEngine {
     int capacity() const { return cap_; }
     int phase(int slot) const { return phase_[slot]; }
     static size_t workspace_bytes(const DvdSpec &s);
+    // admission rules of one problem started from niv vectors (GCI_ERR_ARG)
+    static void check_spec(const DvdSpec &s, int niv);
     bool use_graph = true;
 
   private:
@@ -157,6 +164,7 @@ class DavidsonEngine {
     void launch_iteration();
     void launch_finalize();
     void launch_orth(int passes, int phase);
+    void launch_gram(int G, int sel, int phase, int pass2);
     void launch_op(int phase, int M, int modes = -1);  // modes: mask of operator modes (-1: all in use)
     void update_modes();
     // unit_op: every starting problem has unit start vectors (k_unit_merge),
@@ -227,6 +235,7 @@ class DavidsonEngine {
     int modes_used_ = 0;           // bitmask of operator modes over active slots
     int mwidth_iter_ = 1;          // launch width for iterations
     int nume_max_ = MAXNUME;       // largest NUME over active slots (k_newvec's accumulators)
+    int lim_max_ = MAXLIM;         // largest LIM over active slots (k_gram's tile width)
     int64_t total_chunks_ = 0;
     std::vector<int> hprob_;
     std::vector<int64_t> hrow_;

@@ -775,11 +775,15 @@ __global__ __launch_bounds__(256) void k_precond(DvdProb *P, const int *chunk_pr
 // wave order, so every output is summed in the same order on every run.
 typedef double d4v __attribute__((ext_vector_type(4)));
 
+// LIMC: the largest LIM among the resident problems the launch serves (80
+// or MAXLIM); the A tile and the accumulators are sized for it.
+template <int LIMC>
 __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
                                               double *partial, int sel, int want_phase, int pass2) {
+    static_assert(LIMC % 16 == 0 && LIMC <= MAXLIM, "k_gram: LIMC must be a multiple of 16 up to MAXLIM");
     DvdProb &p = P[chunk_prob[blockIdx.x]];
     if (p.phase != want_phase || (sel == 3 && bulk_exit(p)) || (pass2 && p.skip2)) return;
-    __shared__ double ta[64][MAXLIM + 1];
+    __shared__ double ta[64][LIMC + 1];
     __shared__ double tb[64][MAXNUME + 1];
     const int k = p.kpass, nn = p.nnew;
     const double *A, *B;
@@ -791,8 +795,8 @@ __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob,
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
     if (sel != 3) {
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int nti = (ka + 15) >> 4;  // 16-column tiles of A (<= 5); B fits one tile (kb <= 16)
-        constexpr int NT = MAXLIM / 16;
+        const int nti = (ka + 15) >> 4;  // 16-column tiles of A (<= LIMC / 16); B fits one tile (kb <= 16)
+        constexpr int NT = LIMC / 16;
         // every wave takes 16 of each 64-row tile's rows for all output
         // tiles, so the chains of dependent MFMAs are 4 long per tile
         d4v acc[NT];
@@ -800,7 +804,7 @@ __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob,
         for (int it = 0; it < NT; ++it) acc[it] = d4v{0.0, 0.0, 0.0, 0.0};
         // the next tile's elements are loaded into registers while the
         // matrix cores work on the current one (256 threads, 64 rows)
-        constexpr int QA = 64 * MAXLIM / 256, QB = 64 * MAXNUME / 256;
+        constexpr int QA = 64 * LIMC / 256, QB = 64 * MAXNUME / 256;
         double ra[QA], rb[QB];
         auto fetch = [&](int64_t rt) {
             const int rows = (int)min<int64_t>(64, r1 - rt);
@@ -848,8 +852,8 @@ __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob,
         }
         // the four waves' sums (lane l, element q holds D[l/16 + 4q][l%16])
         // added in wave order through LDS
-        double *red = &ta[0][0];  // 4 x NT x 4 x 64 doubles fit the 64 x (MAXLIM + 1) tile
-        static_assert(4 * NT * 4 * 64 <= 64 * (MAXLIM + 1), "k_gram: reduction scratch exceeds the tile");
+        double *red = &ta[0][0];  // 4 x NT x 4 x 64 doubles fit the 64 x (LIMC + 1) tile
+        static_assert(4 * NT * 4 * 64 <= 64 * (LIMC + 1), "k_gram: reduction scratch exceeds the tile");
 #pragma unroll
         for (int it = 0; it < NT; ++it)
             if (it < nti)
@@ -1758,10 +1762,34 @@ void DavidsonEngine::rebuild_chunks() {
     graph_valid_ = false;
 }
 
+// The admission rules of one problem (DVDSON's input checks, the engine's
+// LIM and NUME limits); throws GCI_ERR_ARG with the reason.
+void DavidsonEngine::check_spec(const DvdSpec &sp, int niv) {
+    GCI_REQUIRE(sp.h && sp.h->n >= 2, "davidson: matrix order < 2");
+    const int64_t n = sp.h->n;
+    const int nume = sp.nume;
+    GCI_REQUIRE(nume >= 1 && nume <= MAXNUME, "davidson: nvec must be in 1.." + std::to_string(MAXNUME));
+    const int lim = sp.lim > 0 ? sp.lim : (int)std::min<int64_t>(n, 2 * nume + 60);
+    GCI_REQUIRE(lim >= 2 && lim <= MAXLIM, "davidson: LIM must be in 2.." + std::to_string(MAXLIM));
+    if (sp.mode != GCI_OP_FP64 || sp.refine)
+        GCI_REQUIRE(sp.h->precision == GCI_SPLIT32, "davidson: split mode needs a split matrix");
+    // DVDSON input checks (dvdson.f90:319-398), ILOW = 1, IHIGH = NUME
+    int ierr = 0;
+    if (lim > n) ierr += 1;
+    if (nume > lim || (nume == lim && nume != n)) ierr += 128;
+    if (nume + 1 > lim) ierr += 128;  // INITDVD builds NUME+1 vectors
+    if (sp.mblock < 1 || sp.mblock > nume) ierr += 256;
+    GCI_REQUIRE(ierr == 0, "davidson: DVDSON input check failed, IERR = " + std::to_string(ierr));
+    GCI_REQUIRE(niv >= 1 && niv <= lim, "davidson: NIV out of range");
+}
+
 std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const std::vector<const double *> &init,
                                        const std::vector<int> &niv) {
     std::vector<int> slots, unit_init;
     int64_t maxneed = 0;
+    GCI_REQUIRE(specs.size() == niv.size() && specs.size() == init.size(), "davidson: admit lists differ in length");
+    // every problem is checked before any engine state changes
+    for (size_t q = 0; q < specs.size(); ++q) check_spec(specs[q], niv[q]);
     if (stage_ev_rec_) GCI_HIP(hipEventSynchronize(stage_ev_));  // the last admission's staging copies
     std::vector<double *> wbasis(specs.size(), nullptr);  // each problem's workspace (basis first)
     GCI_REQUIRE(specs.size() <= (size_t)cap_, "davidson: more problems than slots");
@@ -1780,18 +1808,6 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         const int nume = sp.nume;
         const int lim = sp.lim > 0 ? sp.lim : (int)std::min<int64_t>(n, 2 * nume + 60);
         const int maxiter = sp.maxiter > 0 ? sp.maxiter : (int)std::max<int64_t>(200 * nume, n / 10);
-        GCI_REQUIRE(nume >= 1 && nume <= MAXNUME, "davidson: nvec must be in 1..16");
-        GCI_REQUIRE(lim >= 2 && lim <= MAXLIM, "davidson: LIM must be in 2..80");
-        if (sp.mode != GCI_OP_FP64 || sp.refine)
-            GCI_REQUIRE(sp.h->precision == GCI_SPLIT32, "davidson: split mode needs a split matrix");
-        // DVDSON input checks (dvdson.f90:319-398), ILOW = 1, IHIGH = NUME
-        int ierr = 0;
-        if (lim > n) ierr += 1;
-        if (nume > lim || (nume == lim && nume != n)) ierr += 128;
-        if (nume + 1 > lim) ierr += 128;  // INITDVD builds NUME+1 vectors
-        if (sp.mblock < 1 || sp.mblock > nume) ierr += 256;
-        GCI_REQUIRE(ierr == 0, "davidson: DVDSON input check failed, IERR = " + std::to_string(ierr));
-        GCI_REQUIRE(niv[q] >= 1 && niv[q] <= lim, "davidson: NIV out of range");
         int slot = -1;
         for (int i = 0; i < cap_; ++i)
             if (phase_[i] == PH_EMPTY && std::find(slots.begin(), slots.end(), i) == slots.end()) {
@@ -1931,8 +1947,7 @@ void DavidsonEngine::run_initdvd(const std::vector<int> &slots, bool unit_op) {
         launch_op(PH_INIT, init_width, modes);
     }
     launch(k_init_stage, pg, 64, stream_, dprob_.p, cap_, -1);
-    launch(k_gram, G, 256, stream_, dprob_.p, (const int *)chunk_prob_.p, (const int64_t *)chunk_row_.p,
-           partial_.p, 2, (int)PH_INIT, 0);
+    launch_gram(G, 2, (int)PH_INIT, 0);
     launch_reduce(dprob_.p, cap_, (const double *)partial_.p, (int)PH_INIT, 1, stream_);
     launch(k_adds, cap_, 64, stream_, dprob_.p, (int)PH_INIT, 0);
     launch(k_init_done, pg, 64, stream_, dprob_.p, cap_);
@@ -1943,16 +1958,25 @@ void DavidsonEngine::run_initdvd(const std::vector<int> &slots, bool unit_op) {
     }
 }
 
+// k_gram sized for the largest LIM in the engine: 80-column A tiles while
+// every resident problem has LIM <= 80 (NUME <= 10), else MAXLIM
+void DavidsonEngine::launch_gram(int G, int sel, int phase, int pass2) {
+    const int *cp = chunk_prob_.p;
+    const int64_t *cr = chunk_row_.p;
+    if (lim_max_ <= 80) launch(k_gram<80>, G, 256, stream_, dprob_.p, cp, cr, partial_.p, sel, phase, pass2);
+    else launch(k_gram<MAXLIM>, G, 256, stream_, dprob_.p, cp, cr, partial_.p, sel, phase, pass2);
+}
+
 void DavidsonEngine::launch_orth(int passes, int phase) {
     const int G = (int)total_chunks_;
     const int *cp = chunk_prob_.p;
     const int64_t *cr = chunk_row_.p;
     for (int pass = 0; pass < passes; ++pass) {
-        launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 0, phase, pass);
+        launch_gram(G, 0, phase, pass);
         launch_reduce(dprob_.p, cap_, (const double *)partial_.p, phase, 0, stream_, pass);
         if (nume_max_ <= 8) launch(k_update<8>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
         else launch(k_update<MAXNUME>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
-        launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 1, phase, pass);
+        launch_gram(G, 1, phase, pass);
         launch(k_chol, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, phase, pass);
         if (nume_max_ <= 8) launch(k_scale<8>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
         else launch(k_scale<MAXNUME>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
@@ -1960,21 +1984,24 @@ void DavidsonEngine::launch_orth(int passes, int phase) {
 }
 
 void DavidsonEngine::update_modes() {
-    int modes = 0, mw = 1, nmax = 1;
+    int modes = 0, mw = 1, nmax = 1, lmax = 2;
     for (int i = 0; i < cap_; ++i) {
         if (phase_[i] == PH_EMPTY) continue;
         const DvdSpec &sp = spec_[i];
         nmax = std::max(nmax, sp.nume);
+        lmax = std::max(lmax, sp.lim);
         if (sp.mode == GCI_OP_FP64) modes |= 1 << GCI_OP_FP64;
         else if (refined_[i]) modes |= 1 << GCI_OP_SPLIT_FULL;
         else modes |= 1 << sp.mode;
         mw = std::max(mw, sp.mblock);
     }
-    if (modes != modes_used_ || mw != mwidth_iter_ || nmax != nume_max_ || spmm_shape_version() != shape_version_)
+    if (modes != modes_used_ || mw != mwidth_iter_ || nmax != nume_max_ || (lmax > 80) != (lim_max_ > 80) ||
+        spmm_shape_version() != shape_version_)
         graph_valid_ = false;
     modes_used_ = modes;
     mwidth_iter_ = mw;
     nume_max_ = nmax;
+    lim_max_ = lmax;
     shape_version_ = spmm_shape_version();
 }
 
@@ -2016,7 +2043,7 @@ void DavidsonEngine::launch_finalize() {
     const int *cp = chunk_prob_.p;
     const int64_t *cr = chunk_row_.p;
     launch(k_rotate_exit, G, 256, stream_, dprob_.p, cp, cr);
-    launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 3, (int)PH_FINAL, 0);
+    launch_gram(G, 3, (int)PH_FINAL, 0);
     launch(k_final_post, cap_, 256, stream_, dprob_.p, (const double *)partial_.p);
 }
 
@@ -2033,7 +2060,7 @@ void DavidsonEngine::launch_iteration() {
     launch(k_precond, G, 256, stream_, dprob_.p, cp, cr);
     launch_orth(2, PH_ITER);
     launch_op(PH_ITER, mwidth_iter_);
-    launch(k_gram, G, 256, stream_, dprob_.p, cp, cr, partial_.p, 2, (int)PH_ITER, 0);
+    launch_gram(G, 2, (int)PH_ITER, 0);
     launch_reduce(dprob_.p, cap_, (const double *)partial_.p, (int)PH_ITER, 1, stream_);
     launch(k_adds, cap_, 64, stream_, dprob_.p, (int)PH_ITER, 1);
     launch_finalize();

@@ -63,7 +63,7 @@ DvdSpec resolve(const HMat &h, const gci_dvd_params *p) {
     DvdSpec s;
     s.h = &h;
     s.nume = p->nvec;
-    GCI_REQUIRE(s.nume >= 1 && s.nume <= MAXNUME, "davidson: nvec must be in 1..16");
+    GCI_REQUIRE(s.nume >= 1 && s.nume <= MAXNUME, "davidson: nvec must be in 1.." + std::to_string(MAXNUME));
     s.lim = p->lim > 0 ? p->lim : (int)std::min<int64_t>(h.n, 2 * s.nume + 60);
     s.mblock = p->mblock > 0 ? p->mblock : 1;
     s.maxiter = p->maxiter > 0 ? p->maxiter : (int)std::max<int64_t>(200 * s.nume, h.n / 10);
@@ -177,9 +177,16 @@ int gci_maneig(gci_hmat *h, int nvec, double *eval, double *evec, gci_dvd_stats 
             std::copy(w.begin(), w.end(), eval);
             std::copy(z.begin(), z.end(), evec);
         } else {
-            // DVDSON branch, in-core sparse (maneig.f90:302-451)
+            // DVDSON branch (maneig.f90:262-451).  The reference keeps H in
+            // core while NSTORE = NELMNT + NELMNT/2 + (NCF+1)/2 is at most
+            // NINCOR = 2^28 and runs GDVD with MBLOCK = 1; above it H stays
+            // on disk and MBLOCK = NVEX (maneig.f90:93, 262-266, 416-420).
+            // The device holds H either way; the block size follows the rule.
+            const int64_t nelmnt = h->nnz + h->n;
+            const int64_t nstore = nelmnt + nelmnt / 2 + (h->n + 1) / 2;
             gci_dvd_params p{};
             p.nvec = nvec;
+            p.mblock = nstore > NINCOR ? nvec : 1;
             p.mode = GCI_OP_FP64;
             DvdSpec sp = resolve(*h, &p);
             solve_one(*h, sp, nullptr, 0, eval, evec, nullptr, &local);

@@ -247,9 +247,21 @@ namespace {
 int submit(gci_server *s, gci_hmat *h, int nev, double tol, double eav, int64_t *req, bool own) {
     return guarded([&] {
         GCI_REQUIRE(s && h && req, "gci_server_submit: null argument");
-        GCI_REQUIRE(nev >= 1 && nev <= MAXNUME && h->n >= 2 && 2 * nev + 60 <= MAXLIM + 60,
-                    "gci_server_submit: bad nev or block order");
         GCI_REQUIRE(tol > 0.0, "gci_server_submit: tolerance must be positive");
+        {
+            // the engine's admission rules, checked here so that a request
+            // the worker could not admit is refused to its caller alone
+            Request probe;
+            probe.h = h;
+            probe.nev = nev;
+            probe.tol = tol;
+            const DvdSpec sp = spec_for(probe);
+            try {
+                DavidsonEngine::check_spec(sp, start_vectors(nev, sp.lim));
+            } catch (const Error &e) {
+                throw Error{GCI_ERR_ARG, "gci_server_submit: " + e.msg};
+            }
+        }
         const Clock::time_point t0 = Clock::now();
         // the SpMM layout is built here (once per matrix), on the caller's
         // thread and the matrix's stream, so the worker's engine steps never

@@ -209,6 +209,19 @@ def syev(a):
     return w, a
 
 
+def spevx_lowest(a, nev, nthreads: int = 1):
+    """Lowest nev eigenpairs of a dense symmetric matrix by DSPEVX's steps
+    (tridiagonal reduction, bisection, inverse iteration, back-transform)."""
+    a = np.array(a, dtype=np.float64, order="F")
+    n = a.shape[0]
+    w = np.zeros(nev)
+    z = np.zeros((n, nev), order="F")
+    rc = lib().orc_spevx_lowest(C.c_int(n), dp(a), C.c_int(nev), dp(w), dp(z), C.c_int(nthreads))
+    if rc:
+        raise RuntimeError("spevx_lowest failed")
+    return w, z
+
+
 def maneig(row_end, col, val, eav, nvec, nthreads: int = 1):
     """MANEIG for one block: (eval of H - EAV, evec, nloops, nmv, rc)."""
     ncf = len(row_end) - 1

@@ -764,6 +764,210 @@ int orc_syev(int n, double *a, double *w)
     return rc;
 }
 
+/* Tridiagonal solve (T - lam I) y = b by Gaussian elimination with row
+ * interchanges (the factorisation of LAPACK's DGTTRF/DGTTS2, restated);
+ * pivots smaller than `tiny` are replaced by +-tiny, as inverse iteration
+ * perturbs an exactly singular shift (DSTEIN/DLAGTF). */
+static void tri_shift_solve(int n, const double *d, const double *e, double lam, double tiny,
+                            double *dd, double *du, double *du2, double *dl, int *piv, double *b)
+{
+    for (int i = 0; i < n; ++i) dd[i] = d[i] - lam;
+    for (int i = 0; i < n - 1; ++i) {
+        du[i] = e[i];
+        dl[i] = e[i];
+        du2[i] = 0.0;
+    }
+    for (int i = 0; i < n - 1; ++i) {
+        if (fabs(dd[i]) >= fabs(dl[i])) {
+            if (fabs(dd[i]) < tiny) dd[i] = dd[i] < 0.0 ? -tiny : tiny;
+            double fact = dl[i] / dd[i];
+            dl[i] = fact;
+            dd[i + 1] -= fact * du[i];
+            piv[i] = 0;
+        } else {
+            double fact = dd[i] / dl[i];
+            dd[i] = dl[i];
+            dl[i] = fact;
+            double temp = du[i];
+            du[i] = dd[i + 1];
+            dd[i + 1] = temp - fact * dd[i + 1];
+            if (i < n - 2) {
+                du2[i] = du[i + 1];
+                du[i + 1] = -fact * du[i + 1];
+            }
+            piv[i] = 1;
+        }
+    }
+    if (fabs(dd[n - 1]) < tiny) dd[n - 1] = dd[n - 1] < 0.0 ? -tiny : tiny;
+    for (int i = 0; i < n - 1; ++i) {
+        if (!piv[i]) {
+            b[i + 1] -= dl[i] * b[i];
+        } else {
+            double temp = b[i];
+            b[i] = b[i + 1];
+            b[i + 1] = temp - dl[i] * b[i];
+        }
+    }
+    b[n - 1] /= dd[n - 1];
+    if (n > 1) b[n - 2] = (b[n - 2] - du[n - 2] * b[n - 1]) / dd[n - 2];
+    for (int i = n - 3; i >= 0; --i) b[i] = (b[i] - du[i] * b[i + 1] - du2[i] * b[i + 2]) / dd[i];
+}
+
+/* Number of eigenvalues of the tridiagonal (d, e2 = e^2) below x: the
+ * Sturm sequence count of DSTEBZ (negative pivots of T - x I). */
+static int sturm_count(int n, const double *d, const double *e2, double x, double pivmin)
+{
+    double q = d[0] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    int cnt = q < 0.0;
+    for (int i = 1; i < n; ++i) {
+        q = d[i] - x - e2[i - 1] / q;
+        if (fabs(q) < pivmin) q = -pivmin;
+        cnt += q < 0.0;
+    }
+    return cnt;
+}
+
+/* Lowest nev eigenpairs of a dense symmetric matrix by the steps of
+ * LAPACK's DSPEVX with RANGE = 'I', IL = 1, IU = nev (the call of
+ * rci90/maneig.f90:220 and lib9290/iniest2.f90): Householder reduction to
+ * tridiagonal form (DSPTRD), bisection on Sturm counts to 2 ulp (DSTEBZ),
+ * inverse iteration with reorthogonalisation inside clusters of
+ * eigenvalues closer than 1e-3 |T| (DSTEIN), and back-transformation by
+ * the reflectors (DOPMTR).  The O(n^3) reduction runs on nthreads OpenMP
+ * threads.  a: n x n column-major holding both triangles (destroyed);
+ * w[nev] ascending; z: n x nev column-major. */
+int orc_spevx_lowest(int n, double *a, int nev, double *w, double *z, int nthreads)
+{
+    if (n <= 0 || nev <= 0) return 0;
+    if (nev > n) nev = n;
+    if (nthreads < 1) nthreads = 1;
+#define A(i, j) a[(size_t)(j) * n + (i)]
+    double *d = malloc(sizeof(double) * n), *e = calloc(n, sizeof(double)), *tau = calloc(n, sizeof(double));
+    double *p = malloc(sizeof(double) * n), *wv = malloc(sizeof(double) * n);
+    /* reduction: step k annihilates A(k+2:n, k) by H_k = I - tau v v^T,
+     * v = (1, A(k+2:n, k)) acting on rows/columns k+1..n-1 */
+    for (int k = 0; k < n - 1; ++k) {
+        const int m = n - k - 1;
+        double *x = &A(k + 1, k);
+        d[k] = A(k, k);
+        if (m == 1) {
+            e[k] = x[0];
+            tau[k] = 0.0;
+            continue;
+        }
+        double xnorm = 0.0;
+        for (int i = 1; i < m; ++i) xnorm = hypot(xnorm, x[i]);
+        const double alpha = x[0];
+        if (xnorm == 0.0) {
+            e[k] = alpha;
+            tau[k] = 0.0;
+            continue;
+        }
+        const double beta = -copysign(hypot(alpha, xnorm), alpha);
+        const double t = (beta - alpha) / beta, sc = 1.0 / (alpha - beta);
+        for (int i = 1; i < m; ++i) x[i] *= sc;
+        e[k] = beta;
+        tau[k] = t;
+        x[0] = 1.0;  /* v in place while the trailing block is updated */
+        /* p = tau A22 v (column i of A22 is contiguous and equals row i) */
+#pragma omp parallel for num_threads(nthreads) schedule(static) if (m > 256)
+        for (int i = 0; i < m; ++i) {
+            const double *c = &A(k + 1, k + 1 + i);
+            double sum = 0.0;
+            for (int r = 0; r < m; ++r) sum += c[r] * x[r];
+            p[i] = t * sum;
+        }
+        double pv = 0.0;
+        for (int i = 0; i < m; ++i) pv += p[i] * x[i];
+        const double al = -0.5 * t * pv;
+        for (int i = 0; i < m; ++i) wv[i] = p[i] + al * x[i];
+        /* A22 -= v w^T + w v^T, both triangles */
+#pragma omp parallel for num_threads(nthreads) schedule(static) if (m > 256)
+        for (int j = 0; j < m; ++j) {
+            double *c = &A(k + 1, k + 1 + j);
+            const double vj = x[j], wj = wv[j];
+            for (int r = 0; r < m; ++r) c[r] -= x[r] * wj + wv[r] * vj;
+        }
+        x[0] = beta;
+    }
+    d[n - 1] = A(n - 1, n - 1);
+    /* bisection for the nev lowest eigenvalues */
+    const double eps = 2.220446049250313e-16, safmin = 2.2250738585072014e-308;
+    double *e2 = malloc(sizeof(double) * n);
+    double gl = d[0], gu = d[0], emax2 = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double off = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < n - 1 ? fabs(e[i]) : 0.0);
+        if (d[i] - off < gl) gl = d[i] - off;
+        if (d[i] + off > gu) gu = d[i] + off;
+        if (i < n - 1) {
+            e2[i] = e[i] * e[i];
+            if (e2[i] > emax2) emax2 = e2[i];
+        }
+    }
+    const double pivmin = safmin * (emax2 > 1.0 ? emax2 : 1.0);
+    const double tnorm = fabs(gl) > fabs(gu) ? fabs(gl) : fabs(gu);
+    gl -= 2.0 * eps * tnorm + 2.0 * pivmin;
+    gu += 2.0 * eps * tnorm + 2.0 * pivmin;
+    for (int j = 0; j < nev; ++j) {
+        double lo = j > 0 ? w[j - 1] - 4.0 * eps * tnorm : gl, hi = gu;
+        if (lo < gl) lo = gl;
+        for (int it = 0; it < 400; ++it) {
+            const double big = fabs(lo) > fabs(hi) ? fabs(lo) : fabs(hi);
+            if (hi - lo <= 2.0 * eps * big + 2.0 * pivmin) break;
+            const double mid = 0.5 * (lo + hi);
+            if (sturm_count(n, d, e2, mid, pivmin) > j) hi = mid;
+            else lo = mid;
+        }
+        w[j] = 0.5 * (lo + hi);
+    }
+    /* inverse iteration */
+    double *dd = malloc(sizeof(double) * n), *du = malloc(sizeof(double) * n), *du2 = malloc(sizeof(double) * n);
+    double *dl = malloc(sizeof(double) * n), *y = malloc(sizeof(double) * n);
+    int *piv = malloc(sizeof(int) * n);
+    const double tiny = eps * (tnorm > 0.0 ? tnorm : 1.0), ortol = 1.0e-3 * tnorm;
+    uint64_t seed = 0x9E3779B97F4A7C15ull;
+    for (int j = 0; j < nev; ++j) {
+        int j0 = j;  /* first member of j's cluster */
+        while (j0 > 0 && w[j] - w[j0 - 1] <= ortol) --j0;
+        for (int i = 0; i < n; ++i) {
+            seed = seed * 6364136223846793005ull + 1442695040888963407ull;
+            y[i] = (double)(seed >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+        }
+        for (int it = 0; it < 4; ++it) {
+            tri_shift_solve(n, d, e, w[j], tiny, dd, du, du2, dl, piv, y);
+            for (int q = j0; q < j; ++q) {  /* against the cluster's earlier vectors */
+                const double *zq = z + (size_t)q * n;
+                double c = 0.0;
+                for (int i = 0; i < n; ++i) c += zq[i] * y[i];
+                for (int i = 0; i < n; ++i) y[i] -= c * zq[i];
+            }
+            double nrm = 0.0;
+            for (int i = 0; i < n; ++i) nrm = hypot(nrm, y[i]);
+            for (int i = 0; i < n; ++i) y[i] /= nrm;
+        }
+        memcpy(z + (size_t)j * n, y, sizeof(double) * n);
+    }
+    /* back-transformation: z = H_0 H_1 ... H_{n-2} y */
+#pragma omp parallel for num_threads(nthreads) schedule(static) if (nev > 1)
+    for (int j = 0; j < nev; ++j) {
+        double *zj = z + (size_t)j * n;
+        for (int k = n - 2; k >= 0; --k) {
+            if (tau[k] == 0.0) continue;
+            const double *vk = &A(k + 2, k);
+            double sum = zj[k + 1];
+            for (int i = 0; i < n - k - 2; ++i) sum += vk[i] * zj[k + 2 + i];
+            sum *= tau[k];
+            zj[k + 1] -= sum;
+            for (int i = 0; i < n - k - 2; ++i) zj[k + 2 + i] -= sum * vk[i];
+        }
+    }
+#undef A
+    free(d); free(e); free(tau); free(p); free(wv); free(e2);
+    free(dd); free(du); free(du2); free(dl); free(y); free(piv);
+    return 0;
+}
+
 /* Lowest ifind eigenpairs of a packed-upper symmetric matrix (the DSPEVX
  * 'V','I','U' contract used at dvdson.f90:724-726). */
 static int packed_lowest(int k, const double *sp, int ifind, double *eig, double *vec)
@@ -1114,9 +1318,10 @@ out:
     return ierr;
 }
 
-/* lib9290/iniest2.f90:55-95 */
-int orc_iniest2(int nmax, int ncf, int niv, const int64_t *row_end, const int32_t *col,
-                const double *val, double *basis)
+/* lib9290/iniest2.f90:55-95: the lowest niv eigenvectors of the leading
+ * min(nmax, ncf) block (DSPEVX), zero-padded to ncf rows */
+static int iniest2_mt(int nmax, int ncf, int niv, const int64_t *row_end, const int32_t *col,
+                      const double *val, double *basis, int nthreads)
 {
     int ns = nmax < ncf ? nmax : ncf;
     double *a = calloc((size_t)ns * ns, sizeof(double)), *w = malloc(sizeof(double) * ns);
@@ -1127,12 +1332,21 @@ int orc_iniest2(int nmax, int ncf, int niv, const int64_t *row_end, const int32_
             a[(size_t)(j - 1) * ns + (i - 1)] = v;
             a[(size_t)(i - 1) * ns + (j - 1)] = v;
         }
-    int rc = orc_syev(ns, a, w);
+    double *zv = malloc(sizeof(double) * (size_t)ns * niv);
+    int rc = orc_spevx_lowest(ns, a, niv, w, zv, nthreads);
     memset(basis, 0, sizeof(double) * (size_t)ncf * niv);
-    for (int j = 0; j < niv; ++j) memcpy(basis + (size_t)j * ncf, a + (size_t)j * ns, sizeof(double) * ns);
+    for (int j = 0; j < niv; ++j) memcpy(basis + (size_t)j * ncf, zv + (size_t)j * ns, sizeof(double) * ns);
+    free(zv);
     free(a);
     free(w);
     return rc;
+}
+
+/* lib9290/iniest2.f90:55-95 (on the OpenMP default thread count) */
+int orc_iniest2(int nmax, int ncf, int niv, const int64_t *row_end, const int32_t *col,
+                const double *val, double *basis)
+{
+    return iniest2_mt(nmax, ncf, niv, row_end, col, val, basis, omp_get_max_threads());
 }
 
 /* rci90/maneig.f90:157-242 (dense, NCF <= IOLPCK = 2000) and :262-555
@@ -1163,19 +1377,21 @@ int orc_maneig(int ncf, const int64_t *row_end, const int32_t *col, const double
                 a[(size_t)(j - 1) * ncf + (i - 1)] = F1(sval, ir);
                 a[(size_t)(i - 1) * ncf + (j - 1)] = F1(sval, ir);
             }
-        rc = orc_syev(ncf, a, w);
-        for (int j = 0; j < nvec; ++j) {
-            eval[j] = w[j];
-            memcpy(evec + (size_t)j * ncf, a + (size_t)j * ncf, sizeof(double) * ncf);
-        }
+        rc = orc_spevx_lowest(ncf, a, nvec, w, evec, nthreads);
+        for (int j = 0; j < nvec; ++j) eval[j] = w[j];
         free(a);
         free(w);
     } else {
         int lim = ncf < 2 * nvec + 60 ? ncf : 2 * nvec + 60;
         int maxitr = nvec * 200 > ncf / 10 ? nvec * 200 : ncf / 10;
+        /* storage rule (maneig.f90:262-266, 416-420): NSTORE = NELMNT +
+         * NELMNT/2 + (NCF+1)/2 above NINCOR = 2^28 keeps H on disk, and GDVD
+         * then runs with MBLOCK = NVEX; in core MBLOCK = 1 */
+        int64_t nstore = nnz + nnz / 2 + (ncf + 1) / 2;
+        int mblock = nstore > ORC_NINCOR ? nvec : 1;
         double *work = calloc((size_t)orc_gdvd_worksize(ncf, lim, nvec), sizeof(double));
-        orc_iniest2(2000, ncf, nvec, row_end, col, sval, work);
-        rc = orc_gdvd(ncf, row_end, col, sval, lim, diag, nvec, nvec, 1, 1.0e-17, 1.0e-9, 1.0e-9,
+        iniest2_mt(2000, ncf, nvec, row_end, col, sval, work, nthreads);
+        rc = orc_gdvd(ncf, row_end, col, sval, lim, diag, nvec, nvec, mblock, 1.0e-17, 1.0e-9, 1.0e-9,
                       maxitr, work, nthreads, nloops, nmv);
         for (int j = 0; j < nvec; ++j) eval[j] = work[(size_t)ncf * nvec + j];
         memcpy(evec, work, sizeof(double) * (size_t)ncf * nvec);

@@ -121,6 +121,12 @@ void orc_spicmv2_mt(int n, int m, const int64_t *row_end, const int32_t *col,
  * maneig.f90:220, iniest2.f90).  Returns 0 on success. */
 int orc_syev(int n, double *a, double *w);
 
+/* Lowest nev eigenpairs by the steps of LAPACK's DSPEVX (RANGE='I', IL=1,
+ * IU=nev; called at rci90/maneig.f90:220 and in lib9290/iniest2.f90):
+ * tridiagonal reduction, bisection, inverse iteration, back-transformation.
+ * a: n x n column-major, both triangles (destroyed); w[nev]; z n x nev. */
+int orc_spevx_lowest(int n, double *a, int nev, double *w, double *z, int nthreads);
+
 /* GDVD + DVDSON (libdvd90/gdvd.f90, libdvd90/dvdson.f90) for the lowest
  * ILOW=1..IHIGH eigenpairs with the CSR operator above.  work must hold
  * n*nume initial vectors (niv of them filled) on entry and is sized by
@@ -137,6 +143,10 @@ int orc_gdvd(int n, const int64_t *row_end, const int32_t *col, const double *va
  * min(nmax, ncf) block, zero padded, written to basis (ncf x niv). */
 int orc_iniest2(int nmax, int ncf, int niv, const int64_t *row_end,
                 const int32_t *col, const double *val, double *basis);
+
+/* NINCOR of MANEIG (rci90/maneig.f90:93): matrices whose NSTORE exceeds it
+ * are kept on disk, and GDVD then runs with MBLOCK = NVEX. */
+#define ORC_NINCOR 268435456LL
 
 /* MANEIG (rci90/maneig.f90:157-555) for one block: CSR is the unshifted H
  * as SETHAM writes it; the diagonal is shifted by -eav here.  eval gets

@@ -49,3 +49,21 @@ def test_radgrd_matches_oracle():
     r, rp, rpor = rci.radgrd(590, 2e-6 / 26, 0.05)
     g = O.Grid(590, 2e-6 / 26, 0.05)
     assert np.array_equal(r, g.r) and np.array_equal(rp, g.rp) and np.array_equal(rpor[1:], g.rpor[1:])
+
+
+@pytest.mark.parametrize("bad", [float("nan"), float("inf"), -float("inf")])
+@pytest.mark.parametrize("where", ["offdiag", "diag"])
+def test_upload_rejects_non_finite_values(bad, where):
+    # .res rows of a 3x3 block, diagonal last; the check runs before any
+    # device call
+    import numpy as np
+
+    from grasp_amd import rci
+
+    iendc = np.array([0, 1, 3, 6], dtype=np.int64)
+    irow = np.array([1, 1, 2, 1, 2, 3], dtype=np.int32)
+    emt = np.array([-1.0, 0.1, -2.0, 0.2, 0.3, -3.0])
+    emt[1 if where == "offdiag" else 5] = bad
+    with pytest.raises(L.GciError) as e:
+        rci.HamiltonianBlock.from_rows(iendc, irow, emt)
+    assert e.value.code == L.GCI_ERR_ARG and "non-finite" in str(e.value)

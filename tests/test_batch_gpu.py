@@ -30,6 +30,20 @@ def _reference(csr, eav, nev):
     return w[:nev]
 
 
+def _assert_residuals(csr, eav, V, ev, tol):
+    """Eigenvector quality with the FP64 operator: ||H v - lambda v|| <=
+    tol |lambda + EAV| per pair (the oracle's SPICMV2 on the held block,
+    diagonal shifted by -EAV), and V^T V = I."""
+    diag = np.zeros(csr.nnz, dtype=bool)
+    diag[csr.row_end[1:] - 1] = True
+    held = np.where(diag, csr.val - eav, csr.val)
+    V = np.asfortranarray(V)
+    r = np.linalg.norm(O.spicmv2(csr.row_end, csr.col, held, V, nthreads=8) - V * ev, axis=0)
+    assert np.all(r <= tol * np.abs(ev + eav)), (r, tol * np.abs(ev + eav))
+    k = V.shape[1]
+    assert np.max(np.abs(V.T @ V - np.eye(k))) < 1e-8
+
+
 def test_batch_solve_mixed_sizes(gpu):
     sizes = [120, 350, 900, 1600, 2400, 240, 3000, 700]
     fills = [0.1, 0.08, 0.05, 0.04, 0.03, 0.1, 0.02, 0.06]
@@ -44,8 +58,7 @@ def test_batch_solve_mixed_sizes(gpu):
         assert np.all(rel < tol), (b, rel)
         assert stats[b]["ierr"] == 0
         assert stats[b]["refine_loops"] >= 1
-        V = evecs[b]
-        assert np.max(np.abs(V.T @ V - np.eye(k))) < 1e-8
+        _assert_residuals(csr, eav, evecs[b], evals[b], tol)
 
 
 def test_batch_solve_fp64_blocks(gpu):
@@ -55,6 +68,7 @@ def test_batch_solve_fp64_blocks(gpu):
     for b, (csr, eav) in enumerate(data):
         want = _reference(csr, eav, 3)
         assert np.all(np.abs(evals[b] - want) / np.abs(want + eav) < 1e-10)
+        _assert_residuals(csr, eav, evecs[b], evals[b], 1e-10)
 
 
 def test_server_alone_vs_mix(gpu):
@@ -70,7 +84,7 @@ def test_server_alone_vs_mix(gpu):
         h = rci.HamiltonianBlock.from_csr(csr, shift=-eav, precision=L.GCI_SPLIT32)
         rid = srv.submit(h, k, tol, eav)
         srv.drain()
-        alone.append(srv.poll(rid)[0])
+        alone.append(srv.poll(rid))
     # in the mix: all submitted together, staggered arrivals
     rids = []
     for (csr, eav), k in zip(data, nev):
@@ -82,9 +96,13 @@ def test_server_alone_vs_mix(gpu):
     srv.close()
     for i, ((csr, eav), k) in enumerate(zip(data, nev)):
         want = _reference(csr, eav, k)
-        for got in (alone[i], mix[i][0]):
-            assert np.all(np.abs(got - want) / np.abs(want + eav) < tol), i
-        assert np.all(np.abs(alone[i] - mix[i][0]) / np.abs(want + eav) < tol)
+        for got in (alone[i], mix[i]):
+            assert np.all(np.abs(got[0] - want) / np.abs(want + eav) < tol), i
+            _assert_residuals(csr, eav, got[1], got[0], tol)
+        assert np.all(np.abs(alone[i][0] - mix[i][0]) / np.abs(want + eav) < tol)
+        # the same eigenvectors alone and in the mix (up to sign)
+        ov = np.abs(np.sum(alone[i][1] * mix[i][1], axis=0))
+        assert np.all(np.abs(ov - 1.0) < 1e-6), ov
         assert mix[i][2] > 0.0
 
 
@@ -109,11 +127,12 @@ def test_server_shared_matrix(gpu):
     h = rci.HamiltonianBlock.from_csr(csr, shift=-eav, precision=L.GCI_SPLIT32)
     rids = [srv.submit(h, k, tol, eav, shared=True) for k in (3, 2, 3)]
     srv.drain()
-    res = [srv.poll(r)[0] for r in rids]
+    res = [srv.poll(r) for r in rids]
     srv.close()
-    for got in res:
+    for got, vec, _ in res:
         k = len(got)
         assert np.all(np.abs(got - want[:k]) / np.abs(want[:k] + eav) < tol)
+        _assert_residuals(csr, eav, vec, got, tol)
     # still the caller's: the operator runs on it after the server is gone
     B = np.asfortranarray(np.random.default_rng(4).standard_normal((csr.n, 2)))
     assert np.all(np.isfinite(h.spicmv2(B, L.GCI_OP_SPLIT_FULL)))
@@ -158,4 +177,60 @@ def test_batch_solve_decoupled_block(gpu):
         assert st[0]["ierr"] == 0 and st[0]["nloops"] < 50, st[0]
         rel = np.abs(ev[0] - want[:nev]) / np.abs(want[:nev] + eav)
         assert np.all(rel < 1e-8), rel
+        Vb = np.asfortranarray(V[0])
+        r = np.linalg.norm(O.spicmv2(iendc, irow, emt, Vb) - Vb * ev[0], axis=0)
+        assert np.all(r <= 1e-8 * np.abs(ev[0] + eav)), r
     h.free()
+
+
+def test_batch_unit_start_matches_spmm_start(gpu):
+    """The batched solve's first product of its unit start vectors is read
+    from H's columns (no SpMM pass); a solve of the same block whose first
+    product goes through the SpMM (gci_davidson with the same unit vectors
+    given as the initial estimates, same split operator and refinement)
+    reaches the same eigenpairs."""
+    (csr, eav), = _blocks([2600], 800, [0.04])
+    nev, tol = 4, 1e-9
+    h = rci.HamiltonianBlock.from_csr(csr, shift=-eav, precision=L.GCI_SPLIT32)
+    ev_b, V_b, st_b = rci.batch_solve([h], [nev], tol, [eav])
+    # unit vectors on the NEV smallest diagonal entries (ascending value, then index)
+    d = csr.val[csr.row_end[1:] - 1]
+    rows = np.lexsort((np.arange(csr.n), d))[:nev]
+    init = np.zeros((csr.n, nev), order="F")
+    init[rows, np.arange(nev)] = 1.0
+    ev_g, V_g, res_g, st_g = rci.gdvd(h, nev, mblock=nev, init=init, mode=L.GCI_OP_SPLIT_BULK, refine=True)
+    want = _reference(csr, eav, nev)
+    for ev, V in ((ev_b[0], V_b[0]), (ev_g, V_g)):
+        assert np.all(np.abs(ev - want) / np.abs(want + eav) < tol)
+        _assert_residuals(csr, eav, V, ev, tol)
+    assert np.all(np.abs(ev_b[0] - ev_g) / np.abs(want + eav) < tol)
+    ov = np.abs(np.sum(V_b[0] * V_g, axis=0))
+    assert np.all(np.abs(ov - 1.0) < 1e-6), ov
+    h.free()
+
+
+def test_server_refuses_bad_request_and_keeps_serving(gpu):
+    """A request the engine cannot admit is refused at submit with
+    GCI_ERR_ARG; requests around it (12 eigenpairs, LIM = 84, included)
+    complete."""
+    tol = 1e-9
+    data = _blocks([2200, 40], 900, [0.04, 0.2])
+    srv = rci.Server(8 << 30, max_active=4)
+    (csr, eav), (small, seav) = data
+    good = rci.HamiltonianBlock.from_csr(csr, shift=-eav, precision=L.GCI_SPLIT32)
+    r12 = srv.submit(good, 12, tol, eav, shared=True)
+    bad = rci.HamiltonianBlock.from_csr(small, shift=-seav, precision=L.GCI_SPLIT32)
+    for nev in (40, 17):  # NEV = NCF (DVDSON IERR 128), NEV above 16
+        with pytest.raises(L.GciError) as e:
+            srv.submit(bad, nev, tol, seav, shared=True)
+        assert e.value.code == L.GCI_ERR_ARG
+    r2 = srv.submit(good, 2, tol, eav, shared=True)
+    srv.drain()
+    want = _reference(csr, eav, 12)
+    for rid, k in ((r12, 12), (r2, 2)):
+        ev, V, lat = srv.poll(rid)
+        assert np.all(np.abs(ev - want[:k]) / np.abs(want[:k] + eav) < tol)
+        _assert_residuals(csr, eav, V, ev, tol)
+    srv.close()
+    good.free()
+    bad.free()

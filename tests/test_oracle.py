@@ -143,6 +143,56 @@ def test_syev_matches_numpy():
     assert np.allclose(v.T @ a @ v, np.diag(w), atol=1e-10)
 
 
+@pytest.mark.parametrize("kind", ["random", "clustered", "degenerate", "diagonal", "block"])
+def test_spevx_lowest_matches_numpy(kind):
+    """The DSPEVX-style selected solver against numpy's eigh (LAPACK
+    dsyevd), on spectra that stress each step: random, a tight cluster at
+    the bottom (inverse iteration's reorthogonalisation), exactly repeated
+    eigenvalues, a diagonal matrix (every reflector is the identity) and a
+    decoupled block diagonal matrix (zero off-diagonals in T)."""
+    rng = np.random.default_rng(7)
+    n, nev = 257, 12
+    if kind == "random":
+        a = rng.standard_normal((n, n))
+        a = a + a.T
+    else:
+        q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        lam = np.sort(rng.uniform(-5, 5, n))
+        if kind == "clustered":
+            lam[:6] = -10.0 + 1e-9 * np.arange(6)
+        elif kind == "degenerate":
+            lam[:4] = -10.0
+            lam[4:7] = -9.0
+        if kind == "diagonal":
+            a = np.diag(rng.permutation(lam))
+        elif kind == "block":
+            b = rng.standard_normal((n // 2, n // 2))
+            a = np.zeros((n, n))
+            a[: n // 2, : n // 2] = b + b.T
+            a[n // 2:, n // 2:] = np.diag(lam[n // 2:])
+        else:
+            a = (q * lam) @ q.T
+            a = 0.5 * (a + a.T)
+    want = np.linalg.eigvalsh(a)[:nev]
+    for threads in (1, 4):
+        w, z = O.spevx_lowest(a, nev, nthreads=threads)
+        scale = np.max(np.abs(np.linalg.eigvalsh(a)))
+        assert np.max(np.abs(w - want)) <= 1e-13 * scale, kind
+        # eigenvectors: residuals and orthonormality (bases of repeated
+        # eigenvalues are not unique)
+        assert np.max(np.abs(a @ z - z * w)) <= 1e-11 * scale, kind
+        assert np.max(np.abs(z.T @ z - np.eye(nev))) <= 1e-11, kind
+
+
+def test_spevx_lowest_matches_syev():
+    csr = W.clustered_block(400, 6, fill=0.1)
+    a = csr.dense()
+    w1, z1 = O.spevx_lowest(a, 5)
+    w2, z2 = O.syev(a)
+    assert np.allclose(w1, w2[:5], rtol=0, atol=1e-12 * np.max(np.abs(w2)))
+    assert np.allclose(np.abs(np.sum(z1 * z2[:, :5], axis=0)), 1.0, atol=1e-9)
+
+
 def test_gdvd_matches_eigh():
     n, nvec = 400, 3
     csr = W.clustered_block(n, 3, fill=0.1, sorted_diag=True)

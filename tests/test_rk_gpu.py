@@ -2,7 +2,11 @@
 GPU vs the oracle.  Keys, packing and KSTART must match exactly; values
 within 1e-13 of the oracle relative to max(|R^k|, 1e-3 hartree) -- the GPU
 evaluates the Newton-Cotes recurrences as prefix scans and the quadrature as
-a tree sum, so only the summation order differs."""
+a tree sum, so only the summation order differs.  The true relative error
+|GPU - oracle| / |oracle| of every integral is also measured and recorded
+(pytest record_property): integrals far below 1e-3 hartree come from
+cancellation inside the quadrature, where a different summation order moves
+the last bits of a sum whose terms are much larger than the result."""
 import numpy as np
 import pytest
 
@@ -17,6 +21,9 @@ CASES = {
     "C_III": (6.0, 4, 3, {1: 2, 2: 2}, 12.0),
     # Fe XVII (Z=26): n <= 5, l <= 4 (25 orbitals)
     "Fe_XVII": (26.0, 5, 4, {1: 2, 2: 8}, 56.0),
+    # Fe XVII at the CI configuration's orbital set: n <= 6, l <= 5 (36
+    # orbitals, 151447 integrals)
+    "Fe_XVII_n6": (26.0, 6, 5, {1: 2, 2: 8}, 56.0),
 }
 
 
@@ -32,7 +39,7 @@ def _setup(z, nmax, lmax, occ):
 
 
 @pytest.mark.parametrize("case", list(CASES))
-def test_rk_table_matches_oracle(gpu, case):
+def test_rk_table_matches_oracle(gpu, case, record_property):
     z, nmax, lmax, occ, amass = CASES[case]
     n, h, r, rp, rpor, g, o, orb = _setup(z, nmax, lmax, occ)
     kst, keys, vals = rci.genintrk(r, rp, rpor, n, h, orb)
@@ -42,8 +49,21 @@ def test_rk_table_matches_oracle(gpu, case):
     err = np.abs(vals - ovals)
     scale = np.maximum(np.abs(ovals), 1e-3)
     rel = err / scale
-    print(f"{case}: {len(keys)} integrals, max rel err {rel.max():.2e}, bit-exact {np.mean(err == 0):.3f}")
+    true_rel = err / np.abs(ovals)
+    big = np.abs(ovals) >= 1e-3
+    rec = {"integrals": int(len(keys)), "bit_exact_frac": float(np.mean(err == 0)),
+           "max_rel_floored": float(rel.max()), "max_true_rel": float(true_rel.max()),
+           "max_true_rel_ge_1e-3": float(true_rel[big].max()),
+           "p999_true_rel": float(np.quantile(true_rel, 0.999)),
+           "smallest_abs": float(np.abs(ovals).min()), "max_abs_err_lt_1e-3": float(err[~big].max(initial=0.0))}
+    for k, v in rec.items():
+        record_property(k, v)
+    print(f"{case}: {rec}")
     assert np.all(rel <= 1e-13)
+    # integrals of 1e-3 hartree and more: the bar holds as a true relative error
+    assert rec["max_true_rel_ge_1e-3"] <= 1e-13
+    # below that: absolute agreement at the rounding level of the O(1) terms
+    assert rec["max_abs_err_lt_1e-3"] <= 1e-16
 
 
 @pytest.mark.parametrize("case", list(CASES))
