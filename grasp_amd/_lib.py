@@ -74,7 +74,9 @@ EXPORTS = [
     "gci_spmm_layout",
     "gci_server_submit_shared",
     "gci_host_alloc", "gci_host_free",
+    "gci_spmm_device_noalloc",
 ]
+GCI_NOALLOC_SCRATCH_BYTES = 48 << 20
 
 
 def lib():

@@ -10,6 +10,32 @@ namespace gci {
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
 
+// Scratch of the no-allocation SpMM entry point: one array per device,
+// resident from module load until the process ends.  Host code never uses
+// the array's name as an address: its device address is read once per
+// device with hipGetSymbolAddress (gci_init, or the entry point's first
+// call on a device) and kept in g_scratch_ptr.
+constexpr size_t NOALLOC_SCRATCH_BYTES = GCI_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static char *g_scratch_ptr[MAX_DEVICES] = {};
+
+char *scratch_of_current_device() {
+    int dev = 0;
+    GCI_HIP(hipGetDevice(&dev));
+    GCI_REQUIRE(dev >= 0 && dev < MAX_DEVICES, "scratch: device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            throw Error{GCI_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+        g_scratch_ptr[dev] = static_cast<char *>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
 namespace {
 
 __global__ void k_split(int64_t nnz, const double *__restrict__ v, float *__restrict__ hi,
@@ -119,6 +145,7 @@ int gci_init(int device) {
     return guarded([&] {
         GCI_HIP(hipSetDevice(device));
         GCI_HIP(hipFree(nullptr));
+        (void)scratch_of_current_device();  // the no-allocation entry point's array on this device
     });
 }
 
@@ -207,6 +234,14 @@ int gci_spmm_device(gci_hmat *h, int m, const double *dB, int ldb, double *dC, i
     return guarded([&] {
         GCI_REQUIRE(h, "gci_spmm_device: null handle");
         spmm(*h, m, dB, ldb, dC, ldc, mode, h->stream);
+    });
+}
+
+int gci_spmm_device_noalloc(gci_hmat *h, int m, const double *dB, int ldb, double *dC, int ldc, int mode) {
+    return guarded([&] {
+        GCI_REQUIRE(h && dB && dC, "gci_spmm_device_noalloc: null argument");
+        spmm_noalloc(*h, m, dB, ldb, dC, ldc, mode, h->stream, scratch_of_current_device(),
+                     NOALLOC_SCRATCH_BYTES);
     });
 }
 

@@ -236,6 +236,15 @@ int spmm_width(int m);
 void spmm_items(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, int M, int mode, hipStream_t s);
 // Single-matrix SpMM on device pointers (column-major x, y).
 void spmm(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode, hipStream_t s);
+// The matrix's work items in device memory (allocated and uploaded if missing).
+void ensure_device_items(HMat &h, hipStream_t s);
+// Bytes of caller-provided scratch spmm_noalloc needs for m columns in `mode`.
+size_t spmm_scratch_need(const HMat &h, int m, int mode);
+// spmm with its staging buffers and job descriptor in `scratch` (cap bytes,
+// 256-byte aligned device memory): allocates nothing; the matrix's layout
+// and device items must already exist for the shape in force.
+void spmm_noalloc(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode, hipStream_t s,
+                  char *scratch, size_t cap);
 
 }  // namespace gci
 

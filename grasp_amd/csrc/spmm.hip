@@ -1099,6 +1099,84 @@ void ensure_pcsc(HMat &h) {
     build_items(h, sh.target);
 }
 
+// The matrix's work items in device memory (uploaded if missing or stale).
+void ensure_device_items(HMat &h, hipStream_t s) {
+    Pcsc &pc = h.pc;
+    if (!pc.items.empty() && pc.d_items.n < pc.items.size()) {
+        GCI_HIP(hipStreamSynchronize(s));
+        pc.d_items.alloc(pc.items.size());
+        GCI_HIP(hipMemcpy(pc.d_items.p, pc.items.data(), pc.items.size() * sizeof(SpmmItem),
+                          hipMemcpyHostToDevice));
+    }
+}
+
+namespace {
+__global__ void k_put_job(SpmmJob *dst, SpmmJob j) { *dst = j; }
+
+size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+size_t spmm_scratch_need(const HMat &h, int m, int mode) {
+    const int W = spmm_max_width(mode);
+    const int Mmax = spmm_width(std::min(W, m));
+    const size_t xt = mode == GCI_OP_SPLIT_BULK ? sizeof(float) : sizeof(double);
+    return round256((size_t)h.n * Mmax * xt) + round256((size_t)h.n * Mmax * sizeof(double)) +
+           round256(sizeof(SpmmJob));
+}
+
+void spmm_noalloc(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode, hipStream_t s,
+                  char *scratch, size_t cap) {
+    GCI_REQUIRE(m >= 1, "spmm: m < 1");
+    GCI_REQUIRE(ldx >= h.n && ldy >= h.n, "spmm: leading dimension < n");
+    GCI_REQUIRE(mode == GCI_OP_FP64 || mode == GCI_OP_SPLIT_BULK || mode == GCI_OP_SPLIT_FULL, "spmm: bad mode");
+    if (mode != GCI_OP_FP64)
+        GCI_REQUIRE(h.precision == GCI_SPLIT32, "spmm: split operator on an FP64-only matrix");
+    GCI_REQUIRE(mode != GCI_OP_FP64 || h.val.p != nullptr, "spmm: FP64 operator on a matrix without FP64 values");
+    GCI_REQUIRE(mode != GCI_OP_FP64 || h.precision != GCI_SPLIT32 || h.layout_fp64,
+                "spmm_noalloc: the FP64 operator on a split matrix needs its FP64 layout; apply gci_spmm_device "
+                "with GCI_OP_FP64 once first");
+    if (h.n == 0) return;
+    const SpmmShape sh = spmm_shape();
+    Pcsc &pc = h.pc;
+    GCI_REQUIRE(pc.R == sh.R && pc.target == sh.target && (pc.items.empty() || pc.d_items.n >= pc.items.size()),
+                "spmm_noalloc: the matrix's SpMM layout is not prepared for the shape in force; call "
+                "gci_spmm_layout first");
+    const size_t need = spmm_scratch_need(h, m, mode);
+    GCI_REQUIRE(need <= cap, "spmm_noalloc: n x launch width needs " + std::to_string(need) +
+                                 " bytes of scratch, above the " + std::to_string(cap) + " bytes available");
+    const int W = spmm_max_width(mode);
+    const int Mmax = spmm_width(std::min(W, m));
+    const size_t xt = mode == GCI_OP_SPLIT_BULK ? sizeof(float) : sizeof(double);
+    char *xs = scratch;
+    double *ys = reinterpret_cast<double *>(scratch + round256((size_t)h.n * Mmax * xt));
+    SpmmJob *job = reinterpret_cast<SpmmJob *>(reinterpret_cast<char *>(ys) +
+                                               round256((size_t)h.n * Mmax * sizeof(double)));
+    for (int c0 = 0; c0 < m; c0 += W) {
+        const int mc = std::min(W, m - c0);
+        const int M = spmm_width(mc);
+        const double *xc = x + (int64_t)c0 * ldx;
+        double *yc = y + (int64_t)c0 * ldy;
+        // stream order keeps the previous column block's launch ahead of
+        // this descriptor write
+        hipLaunchKernelGGL(k_put_job, dim3(1), dim3(1), 0, s, job,
+                           SpmmJob{pc.scol.p, pc.sstep.p, pc.lrow.p, pc.val.p, pc.hi.p, pc.lo.p, xs, ys, h.n, mc,
+                                   mode, 1, pc.full ? 1 : 0});
+        const unsigned pb = grid_for(h.n * M);
+        if (mode == GCI_OP_SPLIT_BULK)
+            hipLaunchKernelGGL(k_pack<float>, dim3(pb), dim3(256), 0, s, h.n, mc, M, xc, (int64_t)ldx,
+                               (float *)xs, ys);
+        else
+            hipLaunchKernelGGL(k_pack<double>, dim3(pb), dim3(256), 0, s, h.n, mc, M, xc, (int64_t)ldx,
+                               (double *)xs, ys);
+        GCI_HIP(hipGetLastError());
+        spmm_items(pc.d_items.p, (int64_t)pc.items.size(), job, M, mode, s);
+        GCI_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_unpack, dim3(grid_for(h.n * mc)), dim3(256), 0, s, h.n, mc, M, (const double *)ys,
+                           (const double *)h.diag.p, xc, (int64_t)ldx, yc, (int64_t)ldy);
+        GCI_HIP(hipGetLastError());
+    }
+}
+
 void spmm(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode, hipStream_t s) {
     GCI_REQUIRE(m >= 1, "spmm: m < 1");
     GCI_REQUIRE(ldx >= h.n && ldy >= h.n, "spmm: leading dimension < n");
@@ -1129,12 +1207,7 @@ void spmm(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode
         pc.d_jobs.alloc(jt.size());
         GCI_HIP(hipMemcpy(pc.d_jobs.p, jt.data(), jt.size() * sizeof(SpmmJob), hipMemcpyHostToDevice));
     }
-    if (!pc.items.empty() && pc.d_items.n < pc.items.size()) {
-        GCI_HIP(hipStreamSynchronize(s));
-        pc.d_items.alloc(pc.items.size());
-        GCI_HIP(hipMemcpy(pc.d_items.p, pc.items.data(), pc.items.size() * sizeof(SpmmItem),
-                          hipMemcpyHostToDevice));
-    }
+    ensure_device_items(h, s);
     for (int c0 = 0; c0 < m; c0 += W) {
         const int mc = std::min(W, m - c0);
         const int M = spmm_width(mc);

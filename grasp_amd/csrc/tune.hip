@@ -163,6 +163,7 @@ int gci_spmm_layout(gci_hmat *h, int64_t *nseg, int64_t *nslice, int64_t *stored
     return guarded([&] {
         GCI_REQUIRE(h && nseg && nslice && stored, "gci_spmm_layout: null argument");
         ensure_items(*h);
+        ensure_device_items(*h, h->stream);
         *nseg = h->pc.nseg;
         *nslice = h->pc.nslice;
         *stored = h->pc.nstep * 256;

@@ -183,6 +183,19 @@ module gci_bind
          integer(c_int) :: st
       end function gci_spmm_device
 
+      ! the same operator with no device allocation (library-owned scratch)
+      function gci_spmm_device_noalloc(h, m, db, ldb, dc, ldc, mode) &
+            bind(C, name="gci_spmm_device_noalloc") result(st)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int), value :: m
+         type(c_ptr), value :: db
+         integer(c_int), value :: ldb
+         type(c_ptr), value :: dc
+         integer(c_int), value :: ldc, mode
+         integer(c_int) :: st
+      end function gci_spmm_device_noalloc
+
       function gci_davidson(h, p, eval, evec, resid, stats) bind(C, name="gci_davidson") result(st)
          import :: c_int, c_double, c_ptr, gci_dvd_params, gci_dvd_stats
          type(c_ptr), value :: h

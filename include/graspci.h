@@ -158,6 +158,23 @@ int gci_spmm(gci_hmat *h, int m, const double *B, double *C, int mode);
  * (or be waited for) before the call. */
 int gci_spmm_device(gci_hmat *h, int m, const double *dB, int ldb, double *dC, int ldc, int mode);
 
+/* No counterpart in the reference (SPICMV2 allocates nothing, and the GDVD
+ * contract has no workspace for the operator): the same operator as
+ * gci_spmm_device for callers that cannot allocate device memory -- work
+ * captured into a HIP graph, tight serving loops, harnesses that forbid
+ * hipMalloc.  It calls no hipMalloc/hipFree: its staging (n x launch width
+ * rows of x and y, launch width <= 8) and its launch descriptor live in one
+ * library-owned array of GCI_NOALLOC_SCRATCH_BYTES (48 MiB) per device, so
+ * n x width may be at most about 3.1e6 (n <= 393k rows at 8 columns);
+ * larger calls return GCI_ERR_ARG.  The matrix must be prepared
+ * beforehand: gci_spmm_layout (or one gci_spmm_device call) for the launch
+ * shape in force, and for GCI_OP_FP64 on a split matrix one gci_spmm_device
+ * call in that mode.  Enqueued on the handle's stream, no synchronisation.
+ * Every call on a device shares that device's array, which keeps the last
+ * call's contents: calls that overlap on one device race. */
+#define GCI_NOALLOC_SCRATCH_BYTES (48u << 20)
+int gci_spmm_device_noalloc(gci_hmat *h, int m, const double *dB, int ldb, double *dC, int ldc, int mode);
+
 /* --------------------------------------------- piece 4: Davidson
  * GDVD/DVDSON contract (libdvd90/gdvd.f90:4-147, dvdson.f90): lowest nvec
  * eigenpairs of H (as held, i.e. already shifted), LIM, CRITE, CRITC, CRITR,
