@@ -8,16 +8,18 @@
 // key packing and KSTART).  Entries sharing a Hartree function Y^k(b,d) are
 // grouped; one workgroup per (k, b, d):
 //   * wave 0 builds Y^k on the exponential grid in FP64: the Newton-Cotes
-//     marches of YZK are four interleaved linear recurrences, evaluated as
-//     segmented prefix scans across the 64 lanes (outward for Z^k, inward
-//     for W^k), with YZK's K = 0 asymptotic correction applied per chain;
+//     marches of YZK are four interleaved linear recurrences, one lane per
+//     chain in the reference's order (outward for Z^k, inward for W^k),
+//     with YZK's K = 0 asymptotic correction applied per chain;
 //   * the densities P_b P_d + Q_b Q_d and Y^k live in LDS;
 //   * every wave then integrates entries (a, c) against Y^k with QUAD's
 //     power-law head and strided 5-point Newton-Cotes sum, reduced across
 //     the wave.
 // Elementwise arithmetic follows the reference operation by operation with
-// floating-point contraction disabled; only the order of the recurrence and
-// quadrature sums differs (parallel scans and tree reductions).
+// floating-point contraction disabled, and the recurrences and QUAD's sum
+// run in the reference's order, so a table entry differs from the
+// reference's only where the device log() (QUAD's power-law exponent)
+// rounds differently from the host's.
 #include <algorithm>
 #include <cmath>
 #include <map>
@@ -51,33 +53,10 @@ __device__ inline double ipow_d(double a, int k) {
     return res;
 }
 
-// Inclusive prefix sum over 64 lanes of a wave (Hillis-Steele; lanes in
-// 4 independent groups of 16 when seg == 16).
-__device__ inline double wave_scan(double v, int seg) {
-    const int lane = threadIdx.x & 63;
-    const int sl = lane % seg;
-    for (int off = 1; off < seg; off <<= 1) {
-        const double u = __shfl_up(v, off, 64);
-        if (sl >= off) v += u;
-    }
-    return v;
-}
-
-// Exclusive prefix within groups of `seg` lanes (first lane of a group gets 0).
-__device__ inline double wave_excl_scan(double v, int seg) {
-    const double incl = wave_scan(v, seg);
-    const double up = __shfl_up(incl, 1, 64);
-    return ((threadIdx.x & 63) % seg == 0) ? 0.0 : up;
-}
-
-__device__ inline double wave_sum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// QUAD (quad.f90:388-437) for TA in LDS (1-based view ta[i-1]), one wave.
-// TA(mtp+1..mtp+3) must be zero.
-__device__ double wave_quad(const double *ta, const double *r, const double *rp, int mtp, const RkConst &k) {
+// QUAD (quad.f90:388-437) for TA in LDS (1-based view ta[i-1]), one wave;
+// grp: LDS scratch of (mtp + 3) / 4 doubles.  TA(mtp+1..mtp+3) must be zero.
+__device__ double wave_quad(const double *ta, const double *r, const double *rp, int mtp, const RkConst &k,
+                           double *grp) {
     const int lane = threadIdx.x & 63;
     // first admissible start i in [2, mtp-1]
     int found = 1 << 30;
@@ -99,11 +78,21 @@ __device__ double wave_quad(const double *ta, const double *r, const double *rp,
     }
     if (found == (1 << 30)) return 0.0;
     const int i = found, ip1 = i + 1;
-    // strided closed Newton-Cotes sum over groups a = ip1, ip1+4, ... <= mtp
+    // closed Newton-Cotes sum over groups a = ip1, ip1+4, ... <= mtp: the
+    // group terms in parallel, their sum sequential in group order as
+    // QUAD's SUM runs (quad.f90:84-85)
+    const int ngrp = (mtp - ip1) / 4 + 1;
+    for (int q = lane; q < ngrp; q += 64) {
+        const int a = ip1 + 4 * q;
+        grp[q] = k.c2 * (ta[a - 1] + ta[a + 1]) + k.c3 * ta[a] + k.c4 * ta[a + 2];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     double s = 0.0;
-    for (int a = ip1 + 4 * lane; a <= mtp; a += 256)
-        s += k.c2 * (ta[a - 1] + ta[a + 1]) + k.c3 * ta[a] + k.c4 * ta[a + 2];
-    s = wave_sum(s);
+    if (lane == 0)
+        for (int q = 0; q < ngrp; ++q) s += grp[q];
+    s = __shfl(s, 0, 64);
     const double tai = ta[i - 1], taip1 = ta[ip1 - 1];
     const double frip1 = taip1 / rp[ip1 - 1];
     double fri = tai / rp[i - 1];
@@ -135,6 +124,7 @@ __global__ __launch_bounds__(256) void k_rk_table(RkConst kc, const YDesc *__res
     __shared__ double zk[NMAXG + 8];
     __shared__ double wk[NMAXG + 8];
     __shared__ double tas[4][NMAXG + 8];
+    __shared__ double grps[4][NMAXG / 4 + 8];  // QUAD's group terms, per wave
     __shared__ int ymtp;
     const YDesc y = ys[blockIdx.x];
     const int n = kc.n;
@@ -160,45 +150,27 @@ __global__ __launch_bounds__(256) void k_rk_table(RkConst kc, const YDesc *__res
         temp[ii - 1] = v;
     }
     __syncthreads();
-    if (wave == 0) {
-        // semi-open start ZK(2..4) (yzk.f90:128-135)
-        if (lane == 0) {
+    if (wave == 0 && lane < 4) {
+        // Lane c runs the chain of ii = c+1, c+5, c+9, ... in YZK's order:
+        // its start ZK(c+1) (ZK(1) = 0, ZK(2..4) by the semi-open rule,
+        // yzk.f90:128-135), then the closed march ZK(ii) = ZK(ii-4) + RTMP
+        // for ii = 5..MTP+3 (yzk.f90:138-144).  Sequential chains give the
+        // reference's rounding: near the origin Y^k can be pure rounding
+        // noise (a density whose inward moment cancels), and QUAD's
+        // power-law head then starts where that noise says, which fixes the
+        // Newton-Cotes panel alignment of the whole integral.
+        const int c = lane;
+        if (c == 0) {
             zk[0] = 0.0;
-            for (int ii = 2; ii <= 4; ++ii) {
-                double sum = 0.0;
-                for (int kk = 2; kk <= 5; ++kk) sum = sum + kc.cnc5c[ii - 2][kk - 2] * temp[kk - 1];
-                zk[ii - 1] = sum;
-            }
+        } else {
+            double sum = 0.0;
+            for (int kk = 2; kk <= 5; ++kk) sum = sum + kc.cnc5c[c - 1][kk - 2] * temp[kk - 1];
+            zk[c] = sum;
         }
-        // closed march ZK(ii) = ZK(ii-4) + RTMP(ii), ii = 5..MTP+3: chain
-        // c = (ii-1) % 4 starts at ZK(c+1); 16 lanes per chain scan the
-        // increments in segments of up to `per` elements.
-        const int chain = lane >> 4, sl = lane & 15;
-        const int first = chain + 5;  // first ii of this chain >= 5 with (ii-1)%4 == chain
-        const int cnt = (mtp + 3 >= first) ? (mtp + 3 - first) / 4 + 1 : 0;
-        const int per = (cnt + 15) / 16;
-        auto zinc = [&](int idx) {
-            const int ii = first + 4 * idx;
-            return kc.c1 * (temp[ii - 5] + temp[ii - 1]) + kc.c2 * (temp[ii - 4] + temp[ii - 2]) + kc.c3 * temp[ii - 3];
-        };
-        // pass 1: segment totals; scan across the 16 lanes of the chain
-        double run = 0.0;
-        for (int q = 0; q < per; ++q) {
-            const int idx = sl * per + q;
-            if (idx < cnt) run += zinc(idx);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        double acc = wave_excl_scan(run, 16);
-        const double z0 = zk[chain];  // ZK(chain+1)
-        // pass 2: running sums from the segment offset
-        for (int q = 0; q < per; ++q) {
-            const int idx = sl * per + q;
-            if (idx < cnt) {
-                acc += zinc(idx);
-                zk[first + 4 * idx - 1] = z0 + acc;
-            }
+        for (int ii = c + 5; ii <= mtp + 3; ii += 4) {
+            const double rtmp = kc.c1 * (temp[ii - 5] + temp[ii - 1]) + kc.c2 * (temp[ii - 4] + temp[ii - 2]) +
+                                kc.c3 * temp[ii - 3];
+            zk[ii - 1] = zk[ii - 5] + rtmp;
         }
     }
     __syncthreads();
@@ -246,28 +218,16 @@ __global__ __launch_bounds__(256) void k_rk_table(RkConst kc, const YDesc *__res
     }
     __syncthreads();
     // inward march WK(ii) = WK(ii+4) + (...), ii = MTP..2, WK(MTP+1..) = 0
-    if (wave == 0) {
-        const int chain = lane >> 4, sl = lane & 15;
-        // chain c holds ii with (mtp - ii) % 4 == c, ii = mtp - c - 4*idx >= 2
-        const int start = mtp - chain;
-        const int cnt = (start >= 2) ? (start - 2) / 4 + 1 : 0;
-        const int per = (cnt + 15) / 16;
-        auto winc = [&](int idx) {
-            const int ii = start - 4 * idx;
-            return kc.c1 * (temp[ii - 1] + temp[ii + 3]) + kc.c2 * (temp[ii] + temp[ii + 2]) + kc.c3 * temp[ii + 1];
-        };
-        double run = 0.0;
-        for (int q = 0; q < per; ++q) {
-            const int idx = sl * per + q;
-            if (idx < cnt) run += winc(idx);
-        }
-        double acc = wave_excl_scan(run, 16);
-        for (int q = 0; q < per; ++q) {
-            const int idx = sl * per + q;
-            if (idx < cnt) {
-                acc += winc(idx);
-                wk[start - 4 * idx - 1] = acc;
-            }
+    // (yzk.f90:191-209): lane c runs the chain ii = MTP-c, MTP-c-4, ... in
+    // the reference's order and association
+    if (wave == 0 && lane < 4) {
+        const int c = lane;
+        wk[mtp + 3 - c] = 0.0;  // WK(MTP+4-c): the chain's first WK(ii+4), so a lane touches its own chain only
+        for (int ii = mtp - c; ii >= 2; ii -= 4) {
+            double w = wk[ii + 3] + kc.c1 * (temp[ii - 1] + temp[ii + 3]);
+            w = w + kc.c2 * (temp[ii] + temp[ii + 2]);
+            w = w + kc.c3 * temp[ii + 1];
+            wk[ii - 1] = w;
         }
     }
     __syncthreads();
@@ -296,7 +256,7 @@ __global__ __launch_bounds__(256) void k_rk_table(RkConst kc, const YDesc *__res
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const double res = wave_quad(ta, r, rp, m, kc);
+        const double res = wave_quad(ta, r, rp, m, kc, grps[wave]);
         if (lane == 0) vals[ent_out[e]] = res;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -313,6 +273,7 @@ __global__ __launch_bounds__(64) void k_rinti(RkConst kc, const int *__restrict_
                                               double *__restrict__ out, int nw) {
     __shared__ double r[NMAXG], rp[NMAXG], rpor[NMAXG];
     __shared__ double dp[NMAXG], dq[NMAXG], ta[NMAXG + 8];
+    __shared__ double qgrp[NMAXG / 4 + 8];  // QUAD's group terms
     __shared__ double A[13][13];
     const int pi = blockIdx.x;
     if (pi >= npairs) return;
@@ -364,7 +325,7 @@ __global__ __launch_bounds__(64) void k_rinti(RkConst kc, const int *__restrict_
             ta[i - 1] = v;
         }
         __syncthreads();
-        piece[term] = wave_quad(ta, r, rp, mtp, kc);
+        piece[term] = wave_quad(ta, r, rp, mtp, kc, qgrp);
         __syncthreads();
     }
     const double p1 = cl * piece[0] / h;

@@ -29,3 +29,5 @@ $FC $FFLAGS -c "$HERE/ref_driver.f90" -o ref_driver.o
 $FC -shared -o "$OUT/libgrasp_ref.so" $OBJS ref_driver.o
 rm -rf "$OUT/obj"
 echo "build_ref: $OUT/libgrasp_ref.so"
+# the angular-coefficient producer (librang90) behind the list fixtures
+"$HERE/build_rang.sh"

@@ -1,8 +1,8 @@
 """R^k tables (GENINTRK/SLATER/YZK/QUAD) and one-body I(ab) (RINTI) on the
 GPU vs the oracle.  Keys, packing and KSTART must match exactly; values
 within 1e-13 of the oracle relative to max(|R^k|, 1e-3 hartree) -- the GPU
-evaluates the Newton-Cotes recurrences as prefix scans and the quadrature as
-a tree sum, so only the summation order differs.  The true relative error
+runs YZK's Newton-Cotes recurrences and QUAD's sum in the reference's
+order, so the tables come out bit-identical.  The true relative error
 |GPU - oracle| / |oracle| of every integral is also measured and recorded
 (pytest record_property): integrals far below 1e-3 hartree come from
 cancellation inside the quadrature, where a different summation order moves
@@ -60,6 +60,9 @@ def test_rk_table_matches_oracle(gpu, case, record_property):
         record_property(k, v)
     print(f"{case}: {rec}")
     assert np.all(rel <= 1e-13)
+    # the marches and QUAD's sum run in the reference's order, so the table
+    # is bit-identical to the oracle's (and so to the reference build's)
+    assert rec["bit_exact_frac"] == 1.0
     # integrals of 1e-3 hartree and more: the bar holds as a true relative error
     assert rec["max_true_rel_ge_1e-3"] <= 1e-13
     # below that: absolute agreement at the rounding level of the O(1) terms
@@ -67,13 +70,14 @@ def test_rk_table_matches_oracle(gpu, case, record_property):
 
 
 @pytest.mark.parametrize("case", list(CASES))
-def test_one_body_matches_oracle(gpu, case):
+def test_one_body_matches_oracle(gpu, case, record_property):
     z, nmax, lmax, occ, amass = CASES[case]
     n, h, r, rp, rpor, g, o, orb = _setup(z, nmax, lmax, occ)
     parm = W.fermi_parameters(amass)
     zz = O.nucpot(g, z, parm)
     iab = rci.one_body(r, rp, rpor, n, h, orb, zz, W.C_LIGHT)
     nw = o.nw
+    exact = []
     for a in range(1, nw + 1):
         for b in range(a, nw + 1):
             if orb.nak[a - 1] != orb.nak[b - 1]:
@@ -81,6 +85,8 @@ def test_one_body_matches_oracle(gpu, case):
                 continue
             want = O.rinti(g, o, zz, W.C_LIGHT, a, b, 0)
             assert abs(iab[a - 1, b - 1] - want) <= 1e-13 * max(abs(want), 1e-3), (a, b)
+            exact.append(iab[a - 1, b - 1] == want)
+    record_property("bit_exact_frac", float(np.mean(exact)))
 
 
 def test_rk_table_rejects_bad_orbitals(gpu):
