@@ -191,6 +191,66 @@ def serve_sweep(args, L, rci, synth, sync):
                       "rates": out}), flush=True)
 
 
+def pieces(args, L, rci, sync):
+    """Pieces 1 and 2 of the path on their own, for kernel profiles:
+    the R^k table of Fe XVII with n <= 6, l <= 5 (NW = 36, gci_rk_table)
+    and H assembly (gci_h_assemble) from a synthetic angular list of
+    --asm-ncf CSFs.  Prints one JSON line with the wall time per call and the
+    algorithmic work per call: R^k flops = 12 MTP per distinct Y^k(b, d) +
+    10 MTP per integral; assembly bytes = 12 per list entry + 12 per stored
+    element (the kernel times come from rocprofv3 on this command)."""
+    from grasp_amd import workloads as W
+
+    z, n, h = 26.0, 590, 0.05
+    orb = W.screened_orbitals(z, W.subshells(6, 5), occupied={1: 2, 2: 8})
+    r, rp, rpor = rci.radgrd(n, 2e-6 / z, h)
+    kst, keys, vals = rci.genintrk(r, rp, rpor, n, h, orb)  # warm
+    key = orb.P.shape[0] + 1
+    ks = np.zeros(len(keys), dtype=np.int64)
+    for k in range(len(kst) - 1):
+        ks[kst[k] - 1: kst[k + 1] - 1] = k
+    kk = keys.astype(np.int64)
+    ia, ib, ic, id_ = kk // key**3, (kk // key**2) % key, (kk // key) % key, kk % key
+    mf = orb.mf.astype(np.int64)
+    ymtp = np.minimum(mf[ib - 1], mf[id_ - 1])
+    ydist = {}
+    for kv, b, d, m in zip(ks, ib, id_, ymtp):
+        ydist[(int(kv), int(b), int(d))] = int(m)
+    mtp_int = np.where((ks == 0) & (ib == id_), np.minimum(mf[ia - 1], mf[ic - 1]),
+                       np.minimum(ymtp, np.minimum(mf[ia - 1], mf[ic - 1])))
+    rk_flops = 12.0 * sum(ydist.values()) + 10.0 * float(np.sum(mtp_int))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rci.genintrk(r, rp, rpor, n, h, orb)
+    t_rk = (time.perf_counter() - t0) / args.steps
+    log(f"R^k table: {len(keys)} integrals, {len(ydist)} distinct Y^k, {1e3 * t_rk:.2f} ms per table (wall)")
+    # piece 2: assembly
+    pattern = W.clustered_block(args.asm_ncf, args.seed, fill=args.asm_fill)
+    labels = np.stack([ia, ib, ic, id_], axis=1).astype(np.int32)
+    lists = W.synthetic_lists(pattern, orb.P.shape[0], orb.nkl, labels, ks.astype(np.int32), args.seed)
+    zz = np.full(n, z)  # point nucleus: -r V = Z
+    iab = rci.one_body(r, rp, rpor, n, h, orb, zz, W.C_LIGHT)
+    blk, eav, nel = rci.setham(args.asm_ncf, orb.P.shape[0], lists, iab, kst, keys, vals)  # warm
+    blk.free()
+    nent = len(lists["coeff"]) + len(lists["tshell"])
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        blk, eav, nel = rci.setham(args.asm_ncf, orb.P.shape[0], lists, iab, kst, keys, vals)
+        blk.free()
+    t_asm = (time.perf_counter() - t0) / args.steps
+    log(f"assembly: {len(lists['pair_ic'])} pairs, {nent} list entries, {nel} stored, {1e3 * t_asm:.1f} ms (wall)")
+    print(json.dumps({
+        "metric": "pieces 1-2: R^k table and H assembly", "n_gpus": 1, "steps": args.steps,
+        "rk_table": {"workload": "Fe XVII n<=6 l<=5, NW=36, N=590", "integrals": int(len(keys)),
+                     "distinct_yk": len(ydist), "alg_flops": rk_flops, "wall_ms": round(1e3 * t_rk, 3),
+                     "kernel": "k_rk_table"},
+        "assembly": {"workload": f"synthetic lists on a {args.asm_ncf}-CSF clustered block at "
+                                 f"{args.asm_fill:.0%} fill", "pairs": int(len(lists["pair_ic"])),
+                     "list_entries": int(nent), "stored": int(nel), "alg_bytes": 12.0 * nent + 12.0 * nel,
+                     "wall_ms": round(1e3 * t_asm, 3), "kernel": "k_elements"},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +278,10 @@ def main():
                          "prepared once), nev 1..8, tol 1e-8 or 1e-10")
     ap.add_argument("--sweep-requests", type=int, default=10000)
     ap.add_argument("--sweep-rates", type=str, default="100,200,400,800")
+    ap.add_argument("--pieces", action="store_true",
+                    help="run pieces 1-2 instead: the NW=36 R^k table and H assembly from a synthetic list")
+    ap.add_argument("--asm-ncf", type=int, default=16000)
+    ap.add_argument("--asm-fill", type=float, default=0.02)
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain elementwise kernel that reads and writes 1 GiB (the scale for the "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -251,6 +315,9 @@ def main():
 
     if args.precision:
         precision_form(args, L, rci, synth, sync)
+        return
+    if args.pieces:
+        pieces(args, L, rci, sync)
         return
     if args.serve_sweep:
         serve_sweep(args, L, rci, synth, sync)

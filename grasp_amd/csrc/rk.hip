@@ -120,7 +120,8 @@ __global__ __launch_bounds__(256) void k_rk_table(RkConst kc, const YDesc *__res
                                                   double *__restrict__ vals) {
     __shared__ double r[NMAXG], rp[NMAXG], rpor[NMAXG];
     __shared__ double yk[NMAXG];
-    __shared__ double temp[NMAXG + 8];
+    __shared__ double temp[NMAXG + 8];   // r^k RHOP, the outward march's integrand
+    __shared__ double tempw[NMAXG + 8];  // RHOP / r^(k+1), the inward march's
     __shared__ double zk[NMAXG + 8];
     __shared__ double wk[NMAXG + 8];
     __shared__ double tas[4][NMAXG + 8];
@@ -140,16 +141,21 @@ __global__ __launch_bounds__(256) void k_rk_table(RkConst kc, const YDesc *__res
     const double *Qb = Q + (size_t)y.b * ld, *Qd = Q + (size_t)y.d * ld;
     const int mtp = min(mf[y.b], mf[y.d]);
     // ---- YZK (yzk.f90:99-223); 1-based index ii lives at [ii-1]
-    // TEMP = r^k * RP (Pb Pd + Qb Qd) on 2..MTP, zero on 1 and MTP+1..N+4
+    // TEMP = r^k * RP (Pb Pd + Qb Qd) on 2..MTP and, for the inward march,
+    // RHOP / r^(k+1) (yzk.f90:197-202); zero on 1 and MTP+1..N+4
     for (int ii = 1 + t; ii <= n + 4; ii += blockDim.x) {
-        double v = 0.0;
+        double v = 0.0, vw = 0.0;
         if (ii >= 2 && ii <= mtp) {
             const double rhop = rp[ii - 1] * (Pb[ii - 1] * Pd[ii - 1] + Qb[ii - 1] * Qd[ii - 1]);
             v = (K == 0) ? rhop : ipow_d(r[ii - 1], K) * rhop;
+            vw = (K == 0) ? rhop * (1.0 / r[ii - 1]) : rhop * (1.0 / (ipow_d(r[ii - 1], K) * r[ii - 1]));
         }
         temp[ii - 1] = v;
+        tempw[ii - 1] = vw;
     }
     __syncthreads();
+    // The outward (Z^k) and inward (W^k) marches are independent: wave 0
+    // runs the first, wave 1 the second, at the same time.
     if (wave == 0 && lane < 4) {
         // Lane c runs the chain of ii = c+1, c+5, c+9, ... in YZK's order:
         // its start ZK(c+1) (ZK(1) = 0, ZK(2..4) by the semi-open rule,
@@ -171,6 +177,19 @@ __global__ __launch_bounds__(256) void k_rk_table(RkConst kc, const YDesc *__res
             const double rtmp = kc.c1 * (temp[ii - 5] + temp[ii - 1]) + kc.c2 * (temp[ii - 4] + temp[ii - 2]) +
                                 kc.c3 * temp[ii - 3];
             zk[ii - 1] = zk[ii - 5] + rtmp;
+        }
+    }
+    // inward march WK(ii) = WK(ii+4) + (...), ii = MTP..2, WK(MTP+1..) = 0
+    // (yzk.f90:205-209): lane c of wave 1 runs the chain ii = MTP-c,
+    // MTP-c-4, ... in the reference's order and association
+    if (wave == 1 && lane < 4) {
+        const int c = lane;
+        wk[mtp + 3 - c] = 0.0;  // WK(MTP+4-c): the chain's first WK(ii+4), so a lane touches its own chain only
+        for (int ii = mtp - c; ii >= 2; ii -= 4) {
+            double w = wk[ii + 3] + kc.c1 * (tempw[ii - 1] + tempw[ii + 3]);
+            w = w + kc.c2 * (tempw[ii] + tempw[ii + 2]);
+            w = w + kc.c3 * tempw[ii + 1];
+            wk[ii - 1] = w;
         }
     }
     __syncthreads();
@@ -205,30 +224,6 @@ __global__ __launch_bounds__(256) void k_rk_table(RkConst kc, const YDesc *__res
             z = (K == 0) ? zklim_s : zklim_s * (1.0 / ipow_d(r[ii - 1], K));
         }
         zk[ii - 1] = z;
-    }
-    // TEMP = RHOP / r^(k+1) on 2..MTP (yzk.f90:197-202)
-    __syncthreads();
-    for (int ii = 1 + t; ii <= n + 4; ii += blockDim.x) {
-        double v = 0.0;
-        if (ii >= 2 && ii <= mtp) {
-            const double rhop = rp[ii - 1] * (Pb[ii - 1] * Pd[ii - 1] + Qb[ii - 1] * Qd[ii - 1]);
-            v = (K == 0) ? rhop * (1.0 / r[ii - 1]) : rhop * (1.0 / (ipow_d(r[ii - 1], K) * r[ii - 1]));
-        }
-        temp[ii - 1] = v;
-    }
-    __syncthreads();
-    // inward march WK(ii) = WK(ii+4) + (...), ii = MTP..2, WK(MTP+1..) = 0
-    // (yzk.f90:191-209): lane c runs the chain ii = MTP-c, MTP-c-4, ... in
-    // the reference's order and association
-    if (wave == 0 && lane < 4) {
-        const int c = lane;
-        wk[mtp + 3 - c] = 0.0;  // WK(MTP+4-c): the chain's first WK(ii+4), so a lane touches its own chain only
-        for (int ii = mtp - c; ii >= 2; ii -= 4) {
-            double w = wk[ii + 3] + kc.c1 * (temp[ii - 1] + temp[ii + 3]);
-            w = w + kc.c2 * (temp[ii] + temp[ii + 2]);
-            w = w + kc.c3 * temp[ii + 1];
-            wk[ii - 1] = w;
-        }
     }
     __syncthreads();
     // YK = ZK + WK r^(k+1) (yzk.f90:214-223)

@@ -294,28 +294,26 @@ def synthetic_lists(csr_pattern: LowerCSR, nw: int, nkl, rk_labels, rk_k, seed: 
     same = [(a, b) for a in range(1, nw + 1) for b in range(a + 1, nw + 1) if nkl[a - 1] == nkl[b - 1]]
     one_ia = np.zeros(npairs, dtype=np.int32)
     one_ib = np.zeros(npairs, dtype=np.int32)
-    tsh_off = np.zeros(npairs, dtype=np.int64)
-    tsh = []
-    off = 0
     diag = rows == cols
     choose = rng.random(npairs)
-    for p in range(npairs):
-        if diag[p]:
-            one_ia[p] = one_ib[p] = 1
-            occ = rng.integers(0, 3, size=nw).astype(np.float64)
-            occ[rng.random(nw) < 0.1] = 1e-13
-            tsh.append(occ)
-            tsh_off[p] = off
-            off += nw
-        elif same and choose[p] < 0.3:
-            a, b = same[int(rng.integers(len(same)))]
-            if rng.random() < 0.5:
-                a, b = b, a
-            one_ia[p], one_ib[p] = a, b
-            tsh.append(np.array([rng.normal(0, 0.3)]))
-            tsh_off[p] = off
-            off += 1
-    tshell = np.concatenate(tsh) if tsh else np.zeros(1)
+    sel = (~diag) & (choose < 0.3) if same else np.zeros(npairs, dtype=bool)
+    # TSHELL values per pair: NW on the diagonal, one for a selected
+    # off-diagonal one-body pair, none otherwise
+    ntsh = np.where(diag, nw, np.where(sel, 1, 0)).astype(np.int64)
+    tsh_off = (np.cumsum(ntsh) - ntsh).astype(np.int64)
+    tshell = np.zeros(max(1, int(ntsh.sum())))
+    di = np.nonzero(diag)[0]
+    one_ia[di] = one_ib[di] = 1
+    occ = rng.integers(0, 3, size=(len(di), nw)).astype(np.float64)
+    occ[rng.random(occ.shape) < 0.1] = 1e-13
+    tshell[(tsh_off[di][:, None] + np.arange(nw)[None, :]).ravel()] = occ.ravel()
+    si = np.nonzero(sel)[0]
+    if len(si):
+        pairs = np.asarray(same, dtype=np.int32)[rng.integers(len(same), size=len(si))]
+        swap = rng.random(len(si)) < 0.5
+        one_ia[si] = np.where(swap, pairs[:, 1], pairs[:, 0])
+        one_ib[si] = np.where(swap, pairs[:, 0], pairs[:, 1])
+        tshell[tsh_off[si]] = rng.normal(0, 0.3, size=len(si))
     nt = rng.integers(1, max_terms + 1, size=npairs)
     tb_end = np.cumsum(nt).astype(np.int64)
     tb_beg = (tb_end - nt).astype(np.int64)
