@@ -126,3 +126,31 @@ def test_gpu_pipeline_on_reference_lists(gpu, he, b, record_property):
     record_property("max_rel_err_levels", float(rel.max()))
     assert np.all(rel < 1e-10), rel
     blk.free()
+
+
+def test_gpu_levels_written_as_cm(gpu, he, tmp_path):
+    """The GPU pipeline's levels for the three He blocks, written as a .cm
+    mixing file (grasp_amd.grasp_io, rci90/matrix.f90:209-212), have the
+    reference-held 3Pn6.cm's layout and symmetries, and its levels to
+    5 mhartree (the reference's come from MCDHF orbitals with Breit)."""
+    from grasp_amd import grasp_io as GI
+
+    T = he
+    ref = GI.read_cm(os.path.join(os.path.dirname(__file__), "golden", "he_3pn6.cm"))
+    kst, keys, vals = rci.genintrk(T["r"], T["rp"], T["rpor"], T["n"], T["h"], T["orb"])
+    iab = rci.one_body(T["r"], T["rp"], T["rpor"], T["n"], T["h"], T["orb"], T["zz"], W.C_LIGHT)
+    cm = GI.CmFile(nelec=2, nw=T["nw"])
+    for rb in ref.blocks:
+        ncf = int(G[f"b{rb.jblock}_ncf"])
+        blk, eav, nel = rci.setham(ncf, T["nw"], lists(rb.jblock), iab, kst, keys, vals)
+        ev, evec, st = rci.maneig(blk, rb.nvec)
+        cm.blocks.append(GI.CmBlock(rb.jblock, ncf, rb.iatjpo, rb.iaspar,
+                                    np.arange(1, rb.nvec + 1, dtype=np.int32), eav, ev, evec))
+        blk.free()
+    out = tmp_path / "gpu.cm"
+    GI.write_cm(out, cm)
+    back = GI.read_cm(out)
+    assert (back.ncftot, back.ncmin, back.nvecsiz) == (ref.ncftot, ref.ncmin, ref.nvecsiz)
+    for b, r in zip(back.blocks, ref.blocks):
+        assert (b.jblock, b.ncf, b.nvec, b.iatjpo, b.iaspar) == (r.jblock, r.ncf, r.nvec, r.iatjpo, r.iaspar)
+        assert np.all(np.abs(b.energies - r.energies) < 5e-3)
