@@ -46,6 +46,8 @@ struct DvdProb {
     const float *phi, *plo;
     int64_t pnnz;                   // stored off-diagonal entries (lower triangle)
     int pfull;                      // the layout holds both triangles
+    const uint8_t *pdrow;           // full layouts: row deltas and first rows
+    const uint16_t *psfirst;
     void *xs;                       // operator staging: n x 8 (fp32 or fp64)
     double *ys;                     // n x 8 fp64
     double *basis, *ab;             // n x lim each (column-major, ld = n)

@@ -1152,6 +1152,8 @@ __global__ void k_make_jobs(DvdProb *P, int nprob, SpmmJob *jobs, int want_phase
         j.m = op_cols(p, c0, cb);
         j.mode = op_mode(p);
         j.full = p.pfull;
+        j.drow = p.pdrow;
+        j.sfirst = p.psfirst;
         if (alg_bytes) atomicAdd(&alg_bytes[j.mode], op_alg_bytes(p, j.m, j.mode));
     }
     jobs[i] = j;
@@ -1861,6 +1863,8 @@ std::vector<int> DavidsonEngine::admit(const std::vector<DvdSpec> &specs, const 
         p.pnnz = sp.h->nnz;
         p.will_refine = sp.refine ? 1 : 0;
         p.pfull = sp.h->pc.full ? 1 : 0;
+        p.pdrow = sp.h->pc.drow.p;
+        p.psfirst = sp.h->pc.sfirst.p;
         p.slot = slot;
         p.phase = PH_INIT;
         p.niv = niv[q];

@@ -127,27 +127,38 @@ struct SpmmItem {
 struct SpmmJob;
 
 // Launch shape of the SpMM: panel height R (1024, 2048 or 4096; the layout is
-// built for it), workgroup size, entries per work item, slices per chunk
-// (the most whose tables a launch stages in LDS at once; fewer when the
-// launch width leaves less LDS), and prefetch depth (steps each wavefront
-// keeps in flight).
+// built for it), workgroup size, entries per work item, slices per work item
+// of a full layout (at most 128) and per chunk of a lower-triangle one (the
+// most whose tables a launch stages in LDS at once; fewer when the launch
+// width leaves less LDS), and prefetch depth (steps each wavefront keeps in
+// flight).
 struct SpmmShape {
     int R = 2048;
     int block = 512;
     int64_t target = 131072;
-    int nslice = 32;
+    int nslice = 16;
     int depth = 8;
 };
 
 struct Pcsc {
     int R = 0;                  // rows per panel (0 = not built)
-    bool full = false;          // both triangles laid out (split-precision matrices)
+    // Both triangles laid out (split-precision matrices).  A full layout
+    // keeps each segment's entries in row order and codes rows as one-byte
+    // deltas (drow) from the segment's first row (sfirst, per slice and
+    // lane); a gap above 255 rows takes zero-valued steps of 255.  Work
+    // items and wavefront walks start on slice boundaries, where every
+    // lane's running row is its sfirst.  The lower-triangle layout (FP64
+    // matrices) codes rows as 16-bit LDS offsets (lrow) in a bank schedule.
+    bool full = false;
     int64_t nseg = 0, nnz = 0;  // segments, laid-out (unpadded) entries
     int64_t nslice = 0, nstep = 0;
     int64_t target = 0;         // entries per work item the items were cut for
+    int item_slices = 0;        // and the most slices per item (full layouts)
     DBuf<int32_t> scol;         // nslice x 64 columns (-1: empty lane)
     DBuf<int64_t> sstep;        // nslice + 1 first steps
-    DBuf<uint16_t> lrow;        // nstep x 256: (row - panel start) x 8
+    DBuf<uint16_t> lrow;        // nstep x 256: (row - panel start) x 8 (lower layouts)
+    DBuf<uint8_t> drow;         // nstep x 256: row deltas (full layouts)
+    DBuf<uint16_t> sfirst;      // nslice x 64: first row - panel start of each lane's segment (full layouts)
     DBuf<double> val;           // nstep x 256 (FP64 values, pairs of lanes' halves)
     DBuf<float> hi, lo;         // nstep x 256 (split values)
     std::vector<int64_t> sstep_h;   // host copies for cutting work items
@@ -160,7 +171,8 @@ struct Pcsc {
     DBuf<SpmmJob> d_jobs;
     DBuf<SpmmItem> d_items;
     size_t bytes() const {
-        return scol.bytes() + sstep.bytes() + lrow.bytes() + val.bytes() + hi.bytes() + lo.bytes() +
+        return scol.bytes() + sstep.bytes() + lrow.bytes() + drow.bytes() + sfirst.bytes() + val.bytes() +
+               hi.bytes() + lo.bytes() +
                stage_x.bytes() + stage_y.bytes();
     }
 };
@@ -199,6 +211,8 @@ struct SpmmJob {
     int64_t n;
     int m, mode, active;
     int full;  // 1: both triangles laid out (column sums only), 0: the lower one
+    const uint8_t *drow;     // full layouts: row deltas
+    const uint16_t *sfirst;  // full layouts: first rows per slice and lane
 };
 
 // The launch shape in force (defaults, overridable once through the
@@ -228,7 +242,7 @@ int spmm_max_width(int mode);
 uint64_t spmm_shape_version();
 // Re-cuts h.pc's work items for about `target` entries per item, inside one
 // panel (host only).
-void build_items(HMat &h, int64_t target);
+void build_items(HMat &h, const SpmmShape &sh);
 // Launch width (1, 2, 4, 5 or 8) for m columns.
 int spmm_width(int m);
 // The panel-slice kernel over a list of work items whose jobs live in device

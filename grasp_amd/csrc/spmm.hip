@@ -7,37 +7,46 @@
 //
 // Layout (panel slices, built once per matrix on the device).  Rows are cut
 // into panels of R rows; inside a panel the entries of one column form a
-// segment.  A panel's segments are sorted by length, longest first, and dealt
-// in groups of 64 -- one per wavefront lane -- into slices.  A slice is stored
-// step-major: step k holds entries 4k..4k+3 of each of its 64 segments, so a
-// step is 256 entries that a wavefront reads with one 16-byte load per lane
-// (fp32 values) and one 8-byte load (row offsets).  A slice has as many steps
-// as its longest segment needs; the other lanes are padded with zero-valued
-// entries.  Per entry the stream carries the value (fp32 hi, hi+lo, or fp64)
-// and a 16-bit row code, the LDS byte offset (row - panel start) x 8; per
-// slice, its 64 columns and first step.
+// segment, and segments are dealt in groups of 64 -- one per wavefront lane
+// -- into slices.  A slice is stored step-major: step k holds entries
+// 4k..4k+3 of each of its 64 segments, so a step is 256 entries that a
+// wavefront reads with one 16-byte load per lane (fp32 values) and one small
+// load of row codes.  A slice has as many steps as its longest segment
+// needs; the other lanes are padded with zero-valued entries.  Per slice the
+// layout keeps its 64 columns and first step.
 //
-// Inside a segment the entries are ordered by (row mod 32, row) and lane l's
-// segment begins at its first entry of row class >= l mod 32.  Lanes of one
-// slice have segments of nearly equal length, so at every step their rows
-// fall in nearly distinct classes: the 64 LDS addresses of one row gather or
+// Full layouts (split-precision matrices, and their FP64 copy): both
+// triangles, so every entry H_ij appears as row i of column j and as row j
+// of column i.  Segments keep row order and code rows as one-byte deltas
+// from the lane's previous row; a gap above 255 rows takes zero-valued
+// escape entries of 255.  Per slice and lane the layout keeps the segment's
+// first row.  Columns are dealt into slices in column order.
+//
+// Lower-triangle layouts (FP64 matrices): the strictly-lower entries once,
+// with 16-bit row codes (the LDS byte offset (row - panel start) x 8).  A
+// panel's segments are sorted by length, longest first, and inside a segment
+// the entries are ordered by (row mod 32, row), lane l's segment beginning
+// at its first entry of row class >= l mod 32: lanes of one slice have
+// segments of nearly equal length, so at every step their rows fall in
+// nearly distinct classes and the 64 LDS addresses of one row gather or
 // scatter instruction spread over the banks.
 //
-// Kernel: one workgroup per work item (a run of one panel's steps).  The
-// panel's rows of x (fp32 for the bulk operator, fp64 otherwise) and its row
-// accumulators y_i (fp64) live in LDS, vector-major.  The item's steps are
-// split evenly over the wavefronts, and a wavefront walks its steps with the
-// next `depth` steps' loads in flight.  Lane l holds its segment's x_j and
-// column partials sum_i H_ij x_i in registers; per entry it reads x_i from
-// LDS and adds H_ij x_j into y_i with an LDS atomic.  At a slice end the
-// column partials leave as global atomics, at the item end the panel's rows.
-// A full layout (split-precision matrices) holds every entry also as row j
-// of column i, so the column partials alone are y: no LDS atomics, which
-// bound the lower-triangle walk, at twice the stream.
+// Kernel: one workgroup per work item.  A full layout's item is at most NI
+// whole slices of one panel: the panel's x rows (fp32 for the bulk operator,
+// fp64 otherwise, row-major) and the slice tables are staged in LDS once,
+// then each wavefront walks its share of the slices (cut at slice starts)
+// with the next `depth` steps' loads in flight.  Lane l keeps its column's
+// partial sums sum_i H_ij x_i in registers, reading x_i from LDS at its
+// running row; at a slice end they leave as global atomics -- the column
+// sums are y, the layout holding both triangles.  A lower-triangle item is a
+// run of one panel's steps, walked in chunks of NS slices: besides the
+// column partials every entry adds H_ij x_j into y_i through an LDS atomic,
+// and the panel's rows leave at the item's end.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 #include "gci_internal.h"
 
@@ -50,6 +59,7 @@ namespace {
 constexpr size_t LDS_MAX = 160 * 1024;  // LDS per workgroup on gfx950
 constexpr int SW = 64;                  // lanes (segments) per slice
 constexpr int SE = SW * 4;              // entries per step
+constexpr int ITEM_SLICES_MAX = 128;    // most slices in a full layout's work item
 
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -350,6 +360,106 @@ __global__ void k_sell_pads(int64_t nslice, const int64_t *__restrict__ sstep, c
     }
 }
 
+// ------------------------------------------ full layouts: row deltas
+
+// Sort keys of a full layout: every lower entry e = (i, j) as row i of
+// column j and as row j of column i (key nnz + e), ordered by (panel,
+// column, row).
+__global__ void k_keys_full(int64_t n, int64_t nnz, const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                            int R, uint64_t *__restrict__ keys, uint32_t *__restrict__ idx) {
+    const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const uint64_t panel = (uint64_t)(row / R), lr = (uint64_t)(row % R);
+    for (int64_t e = rp[row] + lane; e < rp[row + 1]; e += 64) {
+        const uint64_t c = (uint64_t)col[e];
+        keys[e] = (panel << 40) | (c << 16) | lr;
+        idx[e] = (uint32_t)e;
+        keys[nnz + e] = ((c / R) << 40) | ((uint64_t)row << 16) | (c % R);
+        idx[nnz + e] = (uint32_t)e;
+    }
+}
+
+constexpr int MAXDELTA = 255;  // largest one-byte row step
+
+// Escape steps before sorted entry e: a gap of g > 255 rows from the
+// previous entry of its segment takes (g - 1) / 255 zero-valued steps of 255.
+__host__ __device__ inline int32_t escapes(const uint64_t *keys, const int32_t *flag, int64_t e) {
+    if (flag[e]) return 0;
+    const int32_t gap = (int32_t)(keys[e] & 0xFFFFull) - (int32_t)(keys[e - 1] & 0xFFFFull);
+    return gap > MAXDELTA ? (gap - 1) / MAXDELTA : 0;
+}
+
+struct EscapesOf {
+    const uint64_t *keys;
+    const int32_t *flag;
+    __host__ __device__ int32_t operator()(int64_t e) const { return escapes(keys, flag, e); }
+};
+
+// per segment: its length in stream positions (entries plus escape steps)
+// and the key sorting a panel's segments longest first
+__global__ void k_seg_order_full(int64_t nseg, int64_t nnz, const int64_t *__restrict__ ustart,
+                                 const int32_t *__restrict__ seg_panel, const int32_t *__restrict__ escx,
+                                 int64_t esc_total, int32_t *__restrict__ ulen, uint64_t *__restrict__ okey,
+                                 int64_t *__restrict__ oval) {
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nseg; s += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t end = s + 1 < nseg ? ustart[s + 1] : nnz;
+        const int64_t pads = (end < nnz ? (int64_t)escx[end] : esc_total) - (int64_t)escx[ustart[s]];
+        const int64_t len = end - ustart[s] + pads;
+        ulen[s] = (int32_t)len;
+        okey[s] = ((uint64_t)seg_panel[s] << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)len);
+        oval[s] = s;
+    }
+}
+
+// per slice: its steps (the longest segment, in lane 0, in quads) and each
+// lane's first row
+__global__ void k_slice_full(int64_t nseg, const int64_t *__restrict__ ustart, const int32_t *__restrict__ ulen,
+                             const int64_t *__restrict__ seg_slice, const int32_t *__restrict__ seg_lane,
+                             const uint64_t *__restrict__ keys, int64_t *__restrict__ slice_steps,
+                             uint16_t *__restrict__ sfirst) {
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nseg; s += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t sl = seg_slice[s];
+        const int lane = seg_lane[s];
+        sfirst[sl * SW + lane] = (uint16_t)(keys[ustart[s]] & 0xFFFFull);
+        if (lane == 0) slice_steps[sl] = (ulen[s] + 3) / 4;
+    }
+}
+
+// The stream of a full layout: each entry at its position in its lane (the
+// entries before it in its segment plus their escape steps), with its row
+// delta and value; the escape steps before it get delta 255 (value 0).
+__global__ void k_fill_full(int64_t nnz, const uint64_t *__restrict__ keys, const int32_t *__restrict__ flag,
+                            const int64_t *__restrict__ segid, const int64_t *__restrict__ ustart,
+                            const int32_t *__restrict__ escx,
+                            const int64_t *__restrict__ seg_slice, const int32_t *__restrict__ seg_lane,
+                            const int64_t *__restrict__ sstep, const uint32_t *__restrict__ perm,
+                            const double *__restrict__ val, const float *__restrict__ hi, const float *__restrict__ lo,
+                            uint8_t *__restrict__ drow, double *__restrict__ pval, float *__restrict__ phi,
+                            float *__restrict__ plo) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t sg = seg_of(flag, segid, e);
+        const int64_t us = ustart[sg];
+        const int64_t p = escapes(keys, flag, e);
+        const int64_t pos = (e - us) + ((int64_t)escx[e] + p - (int64_t)escx[us]);
+        const int lane = seg_lane[sg];
+        const int64_t s0 = sstep[seg_slice[sg]];
+        const int64_t delta =
+            flag[e] ? 0 : (int64_t)(keys[e] & 0xFFFFull) - (int64_t)(keys[e - 1] & 0xFFFFull) - MAXDELTA * p;
+        for (int64_t q = 1; q <= p; ++q)
+            drow[slot_pos(s0 + (pos - q) / 4, lane, (int)((pos - q) & 3))] = (uint8_t)MAXDELTA;
+        const int64_t step = s0 + pos / 4;
+        const int t = (int)(pos & 3);
+        drow[slot_pos(step, lane, t)] = (uint8_t)delta;
+        const int64_t src = (int64_t)perm[e];
+        if (pval) pval[val_pos(step, lane, t)] = val[src];
+        if (phi) {
+            phi[slot_pos(step, lane, t)] = hi[src];
+            plo[slot_pos(step, lane, t)] = lo[src];
+        }
+    }
+}
+
 // ------------------------------------------------------------ kernels
 
 template <int MODE>
@@ -396,48 +506,80 @@ struct StreamRsrc {
 // entry, a multiple of 4).  settle() marks the registers as read on every
 // path (an empty asm statement with them as inputs), so the wait for their
 // load is placed before any branch and is the same on all paths.
-template <int MODE>
-struct Quad;
-
+//
+// Row part: DL = false (lower layouts), four 16-bit LDS byte offsets (one
+// 8-byte load); DL = true (full layouts), four one-byte row deltas (one
+// 4-byte load).
+template <bool DL>
+struct QRows;
 template <>
-struct Quad<GCI_OP_SPLIT_BULK> {
+struct QRows<false> {
     u2v r;
-    f4v h;
     __device__ inline void load(const StreamRsrc &p, uint32_t e) {
         r = __builtin_bit_cast(u2v, __builtin_amdgcn_raw_buffer_load_b64(p.rows, e * 2u, 0, 0));
+    }
+    __device__ inline void settle() const { asm volatile("" ::"v"(r)); }
+};
+template <>
+struct QRows<true> {
+    uint32_t r;
+    __device__ inline void load(const StreamRsrc &p, uint32_t e) {
+        r = __builtin_amdgcn_raw_buffer_load_b32(p.rows, e, 0, 0);
+    }
+    __device__ inline void settle() const { asm volatile("" ::"v"(r)); }
+};
+
+template <int MODE>
+struct QVals;
+
+template <>
+struct QVals<GCI_OP_SPLIT_BULK> {
+    f4v h;
+    __device__ inline void load(const StreamRsrc &p, uint32_t e) {
         h = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(p.hi, e * 4u, 0, 0));
     }
     __device__ inline double value(int j) const { return (double)h[j]; }
-    __device__ inline void settle() const { asm volatile("" ::"v"(r), "v"(h)); }
+    __device__ inline void settle() const { asm volatile("" ::"v"(h)); }
 };
 
 template <>
-struct Quad<GCI_OP_SPLIT_FULL> {
-    u2v r;
+struct QVals<GCI_OP_SPLIT_FULL> {
     f4v h, l;
     __device__ inline void load(const StreamRsrc &p, uint32_t e) {
-        r = __builtin_bit_cast(u2v, __builtin_amdgcn_raw_buffer_load_b64(p.rows, e * 2u, 0, 0));
         h = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(p.hi, e * 4u, 0, 0));
         l = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(p.lo, e * 4u, 0, 0));
     }
     __device__ inline double value(int j) const { return (double)h[j] + (double)l[j]; }
-    __device__ inline void settle() const { asm volatile("" ::"v"(r), "v"(h), "v"(l)); }
+    __device__ inline void settle() const { asm volatile("" ::"v"(h), "v"(l)); }
 };
 
 template <>
-struct Quad<GCI_OP_FP64> {
-    u2v r;
+struct QVals<GCI_OP_FP64> {
     d2v a, b;
     // e = step * 256 + lane * 4: the lane's halves sit at step * 256 + lane * 2
     // and step * 256 + 128 + lane * 2 (val_pos)
     __device__ inline void load(const StreamRsrc &p, uint32_t e) {
         const uint32_t h0 = (e & ~(uint32_t)(SE - 1)) + ((e & (SE - 1)) >> 1);
-        r = __builtin_bit_cast(u2v, __builtin_amdgcn_raw_buffer_load_b64(p.rows, e * 2u, 0, 0));
         a = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(p.val, h0 * 8u, 0, 0));
         b = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(p.val, (h0 + SE / 2) * 8u, 0, 0));
     }
     __device__ inline double value(int j) const { return j < 2 ? a[j] : b[j - 2]; }
-    __device__ inline void settle() const { asm volatile("" ::"v"(r), "v"(a), "v"(b)); }
+    __device__ inline void settle() const { asm volatile("" ::"v"(a), "v"(b)); }
+};
+
+template <int MODE, bool DL>
+struct Quad {
+    QRows<DL> rw;
+    QVals<MODE> v;
+    __device__ inline void load(const StreamRsrc &p, uint32_t e) {
+        rw.load(p, e);
+        v.load(p, e);
+    }
+    __device__ inline double value(int j) const { return v.value(j); }
+    __device__ inline void settle() const {
+        rw.settle();
+        v.settle();
+    }
 };
 
 // M consecutive values of type T from global memory, unconverted.
@@ -460,42 +602,94 @@ __device__ inline void load_raw(const GAS T *p, T (&o)[M]) {
     }
 }
 
-// LDS carve-up of one workgroup (bytes): ys (M x R fp64, vector-major) at 0
-// when the launch may meet lower-triangle layouts (FP64 operator),
-// then xs (M x R of T), and the tables of the item's (at most NS) slices:
-// cs (column partial sums, NS x 64 rows of M fp64), xj (the x_j rows,
-// NS x 64 rows of M T, again only for lower-triangle layouts, whose
-// entries also update y_i), cl (columns, NS x 64 ints) and ss (first
-// steps, NS + 1 ints).
+// LDS carve-ups of one workgroup (bytes).
+//
+// Lower-triangle walk (FP64): ys (M x R fp64, vector-major) at 0, then xs
+// (M x R of T), and the tables of a chunk's (at most NS) slices: cs (column
+// partial sums, NS x 64 rows of M fp64), xj (the x_j rows, NS x 64 rows of M
+// T), cl (columns, NS x 64 ints) and ss (first steps, NS + 1 ints).
+//
+// Stream walk (full layouts): xs (R rows of M T, row-major) at 0, then the
+// tables of the item's (at most NI) slices: cl (columns), fr (first rows -
+// panel start, u16) and ss (first steps), and per wavefront a staging row of
+// 64 x M fp64 (st) that turns a slice's column sums into atomics over
+// consecutive addresses.
 struct SellLds {
-    size_t xs, cs, xj, cl, ss, total;
+    size_t xs, cs, xj, cl, fr, ss, st, total;
 };
+constexpr size_t round16(size_t b) { return (b + 15) & ~(size_t)15; }
 template <int M, class T>
-__host__ __device__ constexpr SellLds sell_lds(int R, int NS, bool rows) {
+__host__ __device__ constexpr SellLds sell_lds(int R, int NS) {
     SellLds m{};
-    m.xs = rows ? (size_t)R * M * sizeof(double) : 0;  // ys only for lower-triangle layouts
-    m.cs = m.xs + (size_t)R * M * sizeof(T);
+    m.xs = (size_t)R * M * sizeof(double);
+    m.cs = m.xs + round16((size_t)R * M * sizeof(T));
     m.xj = m.cs + (size_t)NS * SW * M * sizeof(double);
-    m.cl = m.xj + (rows ? (size_t)NS * SW * M * sizeof(T) : 0);
+    m.cl = m.xj + round16((size_t)NS * SW * M * sizeof(T));
     m.ss = m.cl + (size_t)NS * SW * sizeof(int);
-    m.total = m.ss + (((size_t)(NS + 1) * sizeof(int) + 15) & ~(size_t)15);
+    m.total = m.ss + round16((size_t)(NS + 1) * sizeof(int));
     return m;
+}
+template <int M, class T>
+__host__ __device__ constexpr SellLds stream_lds(int R, int NI, int W) {
+    SellLds m{};
+    m.xs = 0;
+    m.cl = round16((size_t)R * M * sizeof(T));
+    m.fr = m.cl + (size_t)NI * SW * sizeof(int);
+    m.ss = m.fr + (size_t)NI * SW * sizeof(uint16_t);
+    m.st = m.ss + round16((size_t)(NI + 1) * sizeof(int));
+    m.total = m.st + (size_t)W * SW * M * sizeof(double);
+    return m;
+}
+
+// M consecutive values of type T from LDS (a row of row-major x), in as
+// few reads as the row's size and 16-byte alignment allow.
+template <int M, class T>
+__device__ inline void lds_row(const T *p, T (&o)[M]) {
+    if constexpr ((M & (M - 1)) == 0 && M * sizeof(T) >= 16) {
+#pragma unroll
+        for (int q = 0; q < (int)(M * sizeof(T) / 16); ++q) {
+            const f4v w = reinterpret_cast<const f4v *>(p)[q];
+            __builtin_memcpy(reinterpret_cast<char *>(o) + 16 * q, &w, 16);
+        }
+    } else if constexpr ((M & (M - 1)) == 0 && M * sizeof(T) == 8) {
+        const f2v w = reinterpret_cast<const f2v *>(p)[0];
+        __builtin_memcpy(o, &w, 8);
+    } else {
+#pragma unroll
+        for (int v = 0; v < M; ++v) o[v] = p[v];
+    }
 }
 
 // One lane's quad: its four entries (row codes off = lr * 8, values a) applied
 // to the M vectors: the column partials acc[v] += a x_i[v] (x_i from LDS) and
 // y_i[v] += a x_j[v] through LDS atomics.  All row reads are issued first.
-template <int M, int MODE, int R, bool SCATTER>
-__device__ inline void consume(const Quad<MODE> &q, const char *xs, char *ys, const double (&xj)[M],
-                               double (&acc)[M]) {
+// For a full layout (DL) the rows come from the lane's running row lr
+// (row - panel start) plus the quad's four deltas.
+template <int M, int MODE, int R, bool SCATTER, bool DL>
+__device__ inline void consume(const Quad<MODE, DL> &q, const char *xs, char *ys, const double (&xj)[M],
+                               double (&acc)[M], uint32_t &lr) {
     using T = typename XT<MODE>::T;
-    const uint32_t off[4] = {q.r.x & 0xFFFFu, q.r.x >> 16, q.r.y & 0xFFFFu, q.r.y >> 16};
+    uint32_t off[4];
     T xv[4][M];
+    if constexpr (DL) {
+        // full layouts keep x row-major in LDS: a row's M values are one
+        // wide read
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const char *xb = xs + (sizeof(T) == 4 ? off[j] >> 1 : off[j]);
+        for (int j = 0; j < 4; ++j) {
+            lr += (q.rw.r >> (8 * j)) & 0xFFu;
+            lds_row<M, T>(reinterpret_cast<const T *>(xs) + (size_t)lr * M, xv[j]);
+        }
+    } else {
+        off[0] = q.rw.r.x & 0xFFFFu;
+        off[1] = q.rw.r.x >> 16;
+        off[2] = q.rw.r.y & 0xFFFFu;
+        off[3] = q.rw.r.y >> 16;
 #pragma unroll
-        for (int v = 0; v < M; ++v) xv[j][v] = *reinterpret_cast<const T *>(xb + (size_t)v * R * sizeof(T));
+        for (int j = 0; j < 4; ++j) {
+            const char *xb = xs + (sizeof(T) == 4 ? off[j] >> 1 : off[j]);
+#pragma unroll
+            for (int v = 0; v < M; ++v) xv[j][v] = *reinterpret_cast<const T *>(xb + (size_t)v * R * sizeof(T));
+        }
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -520,22 +714,176 @@ __device__ inline void ring_steps(Step &step, Q (&q)[D + 1]) {
     if constexpr (J < D) ring_steps<J + 1, D>(step, q);
 }
 
-// The SpMM work-item kernel (see the file header).
+// The walk of a full layout (split-precision matrices, and FP64 ones laid
+// out full): one workgroup per work item (whole slices of one panel).  The
+// panel's x rows and the item's slice tables are staged in LDS once; then
+// every wavefront walks its share of the item's slices (cut at slice starts,
+// where each lane's running row is known) with one prefetch ring over the
+// whole share, and at each slice end adds its column sums -- the
+// matrix-vector product's rows, as the layout holds every entry in both
+// orientations -- into y with global atomics.  No barrier after the
+// staging, so a wavefront's ring never drains until its share ends.
 template <int M, int MODE, int R, int BS, int D>
-__global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items, const SpmmJob *__restrict__ jobs,
-                                             int NS) {
+__device__ inline void sell_stream(const SpmmItem &it, const SpmmJob &jb, char *smem, int NI) {
     using T = typename XT<MODE>::T;
     constexpr int W = BS / 64;  // wavefronts per workgroup
-    extern __shared__ __align__(64) char smem[];
-    const SpmmItem it = items[blockIdx.x];
-    const SpmmJob jb = jobs[it.job];
-    if (!jb.active || jb.mode != MODE) return;
+    const SellLds lm = stream_lds<M, T>(R, NI, W);
+    T *xs = reinterpret_cast<T *>(smem + lm.xs);
+    int *cl = reinterpret_cast<int *>(smem + lm.cl);
+    uint16_t *fr = reinterpret_cast<uint16_t *>(smem + lm.fr);
+    int *ss = reinterpret_cast<int *>(smem + lm.ss);
+    const GAS T *gx = (const GAS T *)jb.xs;
+    GAS double *gy = (GAS double *)jb.ys;
+    const int t = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
     const int rows = it.rows;
-    if (rows > R) return;  // never reached: items are cut for the shape
-    // split-precision matrices are always laid out full; FP64 ones may be
-    // either (the launch's LDS then holds the row accumulators)
-    constexpr bool ROWS = MODE == GCI_OP_FP64;
-    const SellLds lm = sell_lds<M, T>(R, NS, ROWS);
+    const int nsl = (int)(it.slice1 - it.slice0);
+    const int nst = (int)(it.step1 - it.step0);
+    if (nsl > NI) return;  // never reached: items are cut for the shape
+    // x rows, row-major: every load of a batch is issued before its stores
+    // (indexes clamped to the panel, stores masked), so one wait covers it
+    constexpr int XP = (R + BS - 1) / BS;
+    constexpr int XB = XP < 4 ? XP : 4;
+    for (int j0 = 0; j0 < XP; j0 += XB) {
+        T xr[XB][M];
+#pragma unroll
+        for (int j = 0; j < XB; ++j) load_raw<M, T>(gx + (it.row0 + min(t + (j0 + j) * BS, rows - 1)) * M, xr[j]);
+#pragma unroll
+        for (int j = 0; j < XB; ++j) {
+            const int rr = t + (j0 + j) * BS;
+            if (rr < rows) {
+#pragma unroll
+                for (int v = 0; v < M; ++v) xs[rr * M + v] = xr[j][v];
+            }
+        }
+    }
+    // slice tables: columns, first rows, first steps (relative to the item),
+    // again in batches whose loads are all in flight together
+    {
+        const int nt = nsl * SW;
+        const GAS int *gc = (const GAS int *)jb.scol + it.slice0 * SW;
+        const GAS uint16_t *gf = (const GAS uint16_t *)jb.sfirst + it.slice0 * SW;
+        constexpr int TB = 8;
+        for (int i0 = 0; i0 < nt; i0 += TB * BS) {
+            int c[TB];
+            uint16_t f[TB];
+#pragma unroll
+            for (int j = 0; j < TB; ++j) {
+                const int src = min(i0 + t + j * BS, nt - 1);
+                c[j] = gc[src];
+                f[j] = gf[src];
+            }
+#pragma unroll
+            for (int j = 0; j < TB; ++j) {
+                const int i = i0 + t + j * BS;
+                if (i < nt) {
+                    cl[i] = c[j];
+                    fr[i] = f[j];
+                }
+            }
+        }
+        static_assert(ITEM_SLICES_MAX < BS, "one first step per thread");
+        if (t <= nsl) ss[t] = (int)(((const GAS int64_t *)jb.sstep)[it.slice0 + t] - it.step0);
+    }
+    __syncthreads();
+
+    // the first slice start at or after step x (ss ascends strictly: every
+    // slice has a step; ss[nsl] = nst)
+    auto slice_start_from = [&](int x) {
+        int cnt = 0;
+        for (int base = 0;; base += 64) {
+            const int i = base + lane;
+            const uint64_t mk = __ballot(i <= nsl && ss[min(i, nsl)] < x);
+            cnt += __popcll(mk);
+            if (mk != ~0ull) break;
+        }
+        return __builtin_amdgcn_readfirstlane(ss[min(cnt, nsl)]);
+    };
+    // an even split of the steps, moved to slice starts
+    const int a = slice_start_from((int)((int64_t)nst * w / W));
+    const int b = (w == W - 1) ? nst : slice_start_from((int)((int64_t)nst * (w + 1) / W));
+    if (a >= b) return;
+    int si = 0;  // the slice starting at step a
+    for (int base = 1;; base += 64) {
+        const int i = base + lane;
+        const uint64_t mk = __ballot(i <= nsl && ss[min(i, nsl)] <= a);
+        si += __popcll(mk);
+        if (mk != ~0ull) break;
+    }
+    const int64_t e0 = (it.step0 + a) * (int64_t)SE;
+    const uint32_t ne = (uint32_t)(b - a) * SE;
+    StreamRsrc P;
+    P.rows = rsrc(jb.drow + e0, ne);
+    P.hi = rsrc(jb.hi + e0, ne * 4u);
+    P.lo = rsrc(jb.lo + e0, ne * 4u);
+    P.val = rsrc(jb.val + e0, ne * 8u);
+    const double xj[M] = {};
+    double acc[M];
+    uint32_t lr = 0;  // the lane's running row - panel start
+    auto enter = [&]() {
+#pragma unroll
+        for (int v = 0; v < M; ++v) acc[v] = 0.0;
+        lr = fr[si * SW + lane];
+    };
+    // a slice's column sums leave through the wavefront's staging row: M
+    // atomics, the j-th covering (lane, vector) pairs 64j .. 64j + 63, which
+    // are consecutive doubles of y where the slice's columns are
+    double *st = reinterpret_cast<double *>(smem + lm.st) + w * (SW * M);
+    auto leave = [&]() {
+#pragma unroll
+        for (int v = 0; v < M; ++v) st[lane * M + v] = acc[v];
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            const int e = j * SW + lane, v = e % M;
+            const double y = st[e];
+            const int c = cl[si * SW + e / M];
+            if (c >= 0 && v < jb.m && y != 0.0) __builtin_amdgcn_global_atomic_fadd_f64(gy + (int64_t)c * M + v, y);
+        }
+    };
+    enter();
+    int snext = __builtin_amdgcn_readfirstlane(ss[si + 1]);
+    // The loads run D steps ahead of the consumer through a ring of D + 1
+    // register sets; a load past the wavefront's steps reads zeros, so every
+    // step issues the same loads.
+    Quad<MODE, true> q[D + 1];
+    int k = a;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        q[d].load(P, (uint32_t)(d * SW + lane) * 4u);
+        __builtin_amdgcn_sched_barrier(0);  // issued in ring order, as the loop issues them
+    }
+    auto step = [&](Quad<MODE, true> &cur, Quad<MODE, true> &nxt) {
+        nxt.load(P, (uint32_t)((k - a + D) * SW + lane) * 4u);
+        cur.settle();
+        if (k < b) {
+            consume<M, MODE, R, false, true>(cur, reinterpret_cast<const char *>(xs), nullptr, xj, acc, lr);
+            if (k + 1 == snext && k + 1 < b) {
+                leave();
+                ++si;
+                enter();
+                snext = __builtin_amdgcn_readfirstlane(ss[si + 1]);
+            }
+        }
+        ++k;
+    };
+    while (k < b) ring_steps<0, D>(step, q);
+#pragma unroll
+    for (int d = 0; d <= D; ++d) q[d].settle();
+    leave();
+}
+
+// The walk of a lower-triangle layout (FP64 values): the item in chunks of
+// at most NS slices, whose tables (columns, x_j rows, column sums) fill the
+// LDS beside the panel's x rows and row accumulators.  Every entry acts
+// twice: its column partial in registers, and y_i += H_ij x_j through an LDS
+// atomic; chunk ends flush the column sums, the item's end the rows.
+template <int M, int R, int BS, int D>
+__device__ inline void sell_lower(const SpmmItem &it, const SpmmJob &jb, char *smem, int NS) {
+    constexpr int MODE = GCI_OP_FP64;
+    using T = double;
+    constexpr int W = BS / 64;  // wavefronts per workgroup
+    const int rows = it.rows;
+    const SellLds lm = sell_lds<M, T>(R, NS);
     double *ys = reinterpret_cast<double *>(smem);
     T *xs = reinterpret_cast<T *>(smem + lm.xs);
     double *cs = reinterpret_cast<double *>(smem + lm.cs);
@@ -545,7 +893,6 @@ __global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items,
     const GAS T *gx = (const GAS T *)jb.xs;
     GAS double *gy = (GAS double *)jb.ys;
     const int t = threadIdx.x;
-    const bool full = !ROWS || jb.full != 0;
     // wavefront-uniform values are read through readfirstlane, so the walk's
     // control flow is scalar
     const int w = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
@@ -556,7 +903,7 @@ __global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items,
 #pragma unroll
         for (int v = 0; v < M; ++v) {
             xs[v * R + rr] = xr[v];
-            if (!full) ys[v * R + rr] = 0.0;
+            ys[v * R + rr] = 0.0;
         }
     }
     // the item's slices in chunks of at most NS, whose tables fill the LDS
@@ -569,27 +916,27 @@ __global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items,
         for (int i = t; i < nsl * SW; i += BS) {
             const int c = jb.scol[c0 * SW + i];
             cl[i] = c;
-            if constexpr (ROWS) {  // x_j rows only where entries update y_i too
-                T xr[M];
-                if (c >= 0) {
-                    load_raw<M, T>(gx + (int64_t)c * M, xr);
-                } else {
+            T xr[M];
+            if (c >= 0) {
+                load_raw<M, T>(gx + (int64_t)c * M, xr);
+            } else {
 #pragma unroll
-                    for (int v = 0; v < M; ++v) xr[v] = (T)0;
-                }
-#pragma unroll
-                for (int v = 0; v < M; ++v) xjt[i * M + v] = xr[v];
+                for (int v = 0; v < M; ++v) xr[v] = (T)0;
             }
 #pragma unroll
-            for (int v = 0; v < M; ++v) cs[i * M + v] = 0.0;
+            for (int v = 0; v < M; ++v) {
+                xjt[i * M + v] = xr[v];
+                cs[i * M + v] = 0.0;
+            }
         }
         __syncthreads();
 
-        // The chunk's steps split evenly over the wavefronts.  The only
-        // global accesses inside a walk are the stream loads, so the waits
-        // for them are exact.
-        const int a = (int)((int64_t)nst * w / W), b = (int)((int64_t)nst * (w + 1) / W);
-        if (a < b) {
+        // One wavefront's walk over an even share [a, b) of the chunk's
+        // steps.  The only global accesses inside a walk are the stream
+        // loads, so the waits for them are exact.
+        [&]() {
+            const int a = (int)((int64_t)nst * w / W), b = (int)((int64_t)nst * (w + 1) / W);
+            if (a >= b) return;
             // the slice holding step a: the last i with ss[i] <= a (ss[nsl] >= nst)
             int si = 0;
             for (int base = 1;; base += 64) {
@@ -602,14 +949,13 @@ __global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items,
             const uint32_t ne = (uint32_t)(b - a) * SE;
             StreamRsrc P;
             P.rows = rsrc(jb.lrow + e0, ne * 2u);
-            P.hi = rsrc(jb.hi + e0, ne * 4u);
-            P.lo = rsrc(jb.lo + e0, ne * 4u);
             P.val = rsrc(jb.val + e0, ne * 8u);
             double xj[M], acc[M];
-            auto enter = [&]() {  // x_j of slice si from its table
+            uint32_t lr = 0;  // unused by lower layouts
+            auto enter = [&]() {  // slice si: its lane's x_j
 #pragma unroll
                 for (int v = 0; v < M; ++v) {
-                    xj[v] = ROWS ? (double)xjt[(si * SW + lane) * M + v] : 0.0;
+                    xj[v] = (double)xjt[(si * SW + lane) * M + v];
                     acc[v] = 0.0;
                 }
             };
@@ -621,28 +967,20 @@ __global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items,
             };
             enter();
             int snext = __builtin_amdgcn_readfirstlane(ss[si + 1]);  // first step of the next slice
-            // The loads run D steps ahead of the consumer through a ring of
-            // D + 1 register sets; a load past the wavefront's steps reads
-            // zeros, so every step issues the same loads.
-            Quad<MODE> q[D + 1];
+            Quad<MODE, false> q[D + 1];
             int k = a;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 q[d].load(P, (uint32_t)(d * SW + lane) * 4u);
                 __builtin_amdgcn_sched_barrier(0);  // issued in ring order, as the loop issues them
             }
-            auto step = [&](Quad<MODE> &cur, Quad<MODE> &nxt) {
+            auto step = [&](Quad<MODE, false> &cur, Quad<MODE, false> &nxt) {
                 nxt.load(P, (uint32_t)((k - a + D) * SW + lane) * 4u);
                 cur.settle();
                 if (k < b) {
-                    // a full layout's entries appear in both orientations:
-                    // column sums alone give y
-                    if (full)
-                        consume<M, MODE, R, false>(cur, reinterpret_cast<const char *>(xs),
-                                                   reinterpret_cast<char *>(ys), xj, acc);
-                    else
-                        consume<M, MODE, R, true>(cur, reinterpret_cast<const char *>(xs),
-                                                  reinterpret_cast<char *>(ys), xj, acc);
+                    // a lower layout's entries update y_i too
+                    consume<M, MODE, R, true, false>(cur, reinterpret_cast<const char *>(xs),
+                                                     reinterpret_cast<char *>(ys), xj, acc, lr);
                     if (k + 1 == snext && k + 1 < b) {
                         leave();
                         ++si;
@@ -656,7 +994,7 @@ __global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items,
 #pragma unroll
             for (int d = 0; d <= D; ++d) q[d].settle();
             leave();
-        }
+        }();
         __syncthreads();
         // the chunk's column sums: M consecutive lanes per column row
         for (int idx = t; idx < nsl * SW * M; idx += BS) {
@@ -668,11 +1006,28 @@ __global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items,
         __syncthreads();  // the next chunk overwrites the tables
     }
     // the panel's rows, as contiguous runs of the row-major staging buffer
-    if (full) return;
     for (int idx = t; idx < rows * M; idx += BS) {
         const int rr = idx / M, v = idx % M;
         const double y = ys[v * R + rr];
         if (v < jb.m && y != 0.0) __builtin_amdgcn_global_atomic_fadd_f64(gy + (it.row0 + rr) * M + v, y);
+    }
+}
+
+// The SpMM work-item kernel (see the file header): the stream walk for full
+// layouts, the chunked walk for lower-triangle ones.
+template <int M, int MODE, int R, int BS, int D>
+__global__ __launch_bounds__(BS) void k_sell(const SpmmItem *__restrict__ items, const SpmmJob *__restrict__ jobs,
+                                             int NS, int NI) {
+    extern __shared__ __align__(64) char smem[];
+    const SpmmItem it = items[blockIdx.x];
+    const SpmmJob jb = jobs[it.job];
+    if (!jb.active || jb.mode != MODE) return;
+    if (it.rows > R) return;  // never reached: items are cut for the shape
+    if constexpr (MODE != GCI_OP_FP64) {  // split-precision matrices are always laid out full
+        sell_stream<M, MODE, R, BS, D>(it, jb, smem, NI);
+    } else {
+        if (jb.full) sell_stream<M, MODE, R, BS, D>(it, jb, smem, NI);
+        else sell_lower<M, R, BS, D>(it, jb, smem, NS);
     }
 }
 
@@ -704,22 +1059,27 @@ SpmmShape g_shape;         // shape all launches use
 bool g_shape_set = false;
 uint64_t g_shape_version = 1;
 
-int chunk_slices(const SpmmShape &sh, int M, int mode);
+int chunk_slices(const SpmmShape &sh, int M);
+int item_slices(const SpmmShape &sh) { return std::min(sh.nslice, ITEM_SLICES_MAX); }
 
 template <int M, int MODE, int R, int BS, int D>
 void launch_depth(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh,
                   hipStream_t s) {
     using T = typename XT<MODE>::T;
-    const int ns = chunk_slices(sh, M, MODE);
-    GCI_REQUIRE(ns >= 1, "spmm: panel height and launch width exceed the LDS");
-    const size_t shm = sell_lds<M, T>(R, ns, MODE == GCI_OP_FP64).total;
+    // FP64 launches may meet lower-triangle layouts too: room for both walks
+    const int ns = MODE == GCI_OP_FP64 ? chunk_slices(sh, M) : 0;
+    GCI_REQUIRE(MODE != GCI_OP_FP64 || ns >= 1, "spmm: panel height and launch width exceed the LDS");
+    const int ni = item_slices(sh);
+    size_t shm = stream_lds<M, T>(R, ni, BS / 64).total;
+    if (MODE == GCI_OP_FP64) shm = std::max(shm, sell_lds<M, double>(R, ns).total);
+    GCI_REQUIRE(shm <= LDS_MAX, "spmm: panel height and launch width exceed the LDS");
     static size_t attr = 0;  // one instantiation per template tuple
     if (shm > attr) {
         GCI_HIP(hipFuncSetAttribute((const void *)k_sell<M, MODE, R, BS, D>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         attr = shm;
     }
-    hipLaunchKernelGGL((k_sell<M, MODE, R, BS, D>), dim3((unsigned)nitems), dim3(BS), shm, s, d_items, d_jobs, ns);
+    hipLaunchKernelGGL((k_sell<M, MODE, R, BS, D>), dim3((unsigned)nitems), dim3(BS), shm, s, d_items, d_jobs, ns, ni);
 }
 
 template <int M, int MODE, int R, int BS>
@@ -732,8 +1092,27 @@ void launch_shape(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs
 
 // 1024-thread workgroups only up to 4 columns (8 would not fit the 128
 // registers per lane that 16 wavefronts leave); wider launches use 512
+// More than four columns of a full layout can fill so much LDS that a
+// 512-thread workgroup is alone on its CU (FP64 x rows of the split operator's
+// refinement): such launches take 1024-thread workgroups at depth 2 (within
+// the 128 registers per lane that 16 wavefronts leave), twice the wavefronts
+// in flight.
+template <int M, int MODE, int R>
+bool wide_pays(const SpmmShape &sh) {
+    using T = typename XT<MODE>::T;
+    const int ni = item_slices(sh);
+    return MODE != GCI_OP_FP64 && M > 4 && sh.block == 512 && 2 * stream_lds<M, T>(R, ni, 8).total > LDS_MAX &&
+           stream_lds<M, T>(R, ni, 16).total <= LDS_MAX;
+}
+
 template <int M, int MODE, int R>
 void launch_bs(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, const SpmmShape &sh, hipStream_t s) {
+    if constexpr (MODE != GCI_OP_FP64 && M > 4) {
+        if (wide_pays<M, MODE, R>(sh)) {
+            launch_depth<M, MODE, R, 1024, 2>(d_items, nitems, d_jobs, sh, s);
+            return;
+        }
+    }
     if (sh.block == 256) {
         launch_shape<M, MODE, R, 256>(d_items, nitems, d_jobs, sh, s);
     } else if constexpr (M <= 4) {
@@ -768,23 +1147,43 @@ int env_int(const char *name, int dflt) {
     return v && *v ? std::atoi(v) : dflt;
 }
 
-size_t lds_bytes(int R, int NS, int M, int mode) {
+// LDS of the lower-triangle walk (FP64) with NS slices per chunk
+size_t lower_bytes(int R, int NS, int M) {
+    switch (M) {
+        case 1: return sell_lds<1, double>(R, NS).total;
+        case 2: return sell_lds<2, double>(R, NS).total;
+        case 4: return sell_lds<4, double>(R, NS).total;
+        case 5: return sell_lds<5, double>(R, NS).total;
+        default: return sell_lds<8, double>(R, NS).total;
+    }
+}
+
+// LDS of the stream walk (full layouts)
+size_t stream_bytes(int R, int NI, int W, int M, int mode) {
     if (mode == GCI_OP_SPLIT_BULK) {
         switch (M) {
-            case 1: return sell_lds<1, float>(R, NS, false).total;
-            case 2: return sell_lds<2, float>(R, NS, false).total;
-            case 4: return sell_lds<4, float>(R, NS, false).total;
-            case 5: return sell_lds<5, float>(R, NS, false).total;
-            default: return sell_lds<8, float>(R, NS, false).total;
+            case 1: return stream_lds<1, float>(R, NI, W).total;
+            case 2: return stream_lds<2, float>(R, NI, W).total;
+            case 4: return stream_lds<4, float>(R, NI, W).total;
+            case 5: return stream_lds<5, float>(R, NI, W).total;
+            default: return stream_lds<8, float>(R, NI, W).total;
         }
     }
     switch (M) {
-        case 1: return sell_lds<1, double>(R, NS, mode == GCI_OP_FP64).total;
-        case 2: return sell_lds<2, double>(R, NS, mode == GCI_OP_FP64).total;
-        case 4: return sell_lds<4, double>(R, NS, mode == GCI_OP_FP64).total;
-        case 5: return sell_lds<5, double>(R, NS, mode == GCI_OP_FP64).total;
-        default: return sell_lds<8, double>(R, NS, mode == GCI_OP_FP64).total;
+        case 1: return stream_lds<1, double>(R, NI, W).total;
+        case 2: return stream_lds<2, double>(R, NI, W).total;
+        case 4: return stream_lds<4, double>(R, NI, W).total;
+        case 5: return stream_lds<5, double>(R, NI, W).total;
+        default: return stream_lds<8, double>(R, NI, W).total;
     }
+}
+
+template <class It, class O>
+void scan_exclusive_it(It in, O *out, int64_t n, hipStream_t s) {
+    size_t tb = 0;
+    GCI_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, s));
+    SBuf<char> tmp(tb, s);
+    GCI_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, in, out, n, s));
 }
 
 template <class I, class O>
@@ -818,17 +1217,24 @@ void sort_pairs(const K *k0, K *k1, const V *v0, V *v1, int64_t n, hipStream_t s
 
 unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); }
 
-// Slices per chunk of a launch: the shape's count, or fewer if the LDS of
-// this width and mode holds fewer (0: not even one).
-int chunk_slices(const SpmmShape &sh, int M, int mode) {
+// Slices per chunk of an FP64 launch over a lower-triangle layout: the
+// shape's count, or fewer if the LDS of this width holds fewer (0: not even
+// one).
+int chunk_slices(const SpmmShape &sh, int M) {
     int ns = sh.nslice;
-    while (ns > 0 && lds_bytes(sh.R, ns, M, mode) > LDS_MAX) ns = ns > 64 ? ns - ns / 8 : ns - 1;
+    while (ns > 0 && lower_bytes(sh.R, ns, M) > LDS_MAX) ns = ns > 64 ? ns - ns / 8 : ns - 1;
     return ns;
 }
 
 // A launch width runs only if its chunks hold at least this many slices
 // (narrower column blocks are cheaper than tiny chunks).
 constexpr int NS_MIN = 8;
+
+// whether a launch of M columns in this mode fits the LDS
+bool width_fits(const SpmmShape &sh, int M, int mode) {
+    return stream_bytes(sh.R, item_slices(sh), sh.block / 64, M, mode) <= LDS_MAX &&
+           (mode != GCI_OP_FP64 || chunk_slices(sh, M) >= std::min(NS_MIN, sh.nslice));
+}
 
 }  // namespace
 
@@ -837,7 +1243,7 @@ bool valid_shape(const SpmmShape &sh) {
     return (sh.R == 1024 || sh.R == 2048 || sh.R == 4096) && (sh.block == 256 || sh.block == 512 || sh.block == 1024) &&
            sh.target >= 4096 && sh.nslice >= 1 && sh.nslice <= 1024 &&
            (sh.depth == 2 || sh.depth == 4 || sh.depth == 8) &&
-           chunk_slices(sh, 1, GCI_OP_FP64) >= std::min(NS_MIN, sh.nslice);
+           width_fits(sh, 1, GCI_OP_FP64);
 }
 
 SpmmShape spmm_shape() {
@@ -867,7 +1273,7 @@ uint64_t spmm_shape_version() { return g_shape_version; }
 int spmm_max_width(int mode) {
     const SpmmShape sh = spmm_shape();
     for (int w : {8, 5, 4, 2})
-        if (chunk_slices(sh, w, mode) >= std::min(NS_MIN, sh.nslice)) return w;
+        if (width_fits(sh, w, mode)) return w;
     return 1;
 }
 
@@ -886,10 +1292,12 @@ void spmm_items(const SpmmItem *d_items, int64_t nitems, const SpmmJob *d_jobs, 
     }
 }
 
-void build_items(HMat &h, int64_t target) {
+void build_items(HMat &h, const SpmmShape &sh) {
     Pcsc &pc = h.pc;
     pc.items.clear();
     const int R = pc.R;
+    const int64_t target = sh.target;
+    const int ni = item_slices(sh);
     const int64_t ts = std::max<int64_t>(1, target / SE);  // steps per item
     const std::vector<int64_t> &T = pc.sstep_h;
     for (int64_t sa = 0; sa < pc.nslice;) {
@@ -898,6 +1306,18 @@ void build_items(HMat &h, int64_t target) {
         while (sb < pc.nslice && pc.spanel_h[sb] == p) ++sb;
         const int64_t row0 = (int64_t)p * R;
         const int32_t rows = (int32_t)std::min<int64_t>(R, h.n - row0);
+        if (pc.full) {
+            // full layouts: items of whole slices (a walk starts on a slice
+            // boundary, where every lane's running row is known)
+            for (int64_t c = sa; c < sb;) {
+                int64_t c1 = c + 1;
+                while (c1 < sb && T[c1] - T[c] < ts && c1 - c < ni) ++c1;
+                pc.items.push_back(SpmmItem{T[c], T[c1], c, c1, row0, rows, 0});
+                c = c1;
+            }
+            sa = sb;
+            continue;
+        }
         int64_t cur = sa;  // the slice holding step st
         for (int64_t st = T[sa]; st < T[sb];) {
             const int64_t en = std::min(st + ts, T[sb]);
@@ -910,13 +1330,14 @@ void build_items(HMat &h, int64_t target) {
         sa = sb;
     }
     pc.target = target;
+    pc.item_slices = ni;
 }
 
 void recut_items(HMat &h) {
     const SpmmShape sh = spmm_shape();
     GCI_REQUIRE(h.pc.R == sh.R, "spmm: panel height changed while the matrix is in use");
-    if (h.pc.target != sh.target) {
-        build_items(h, sh.target);
+    if (h.pc.target != sh.target || h.pc.item_slices != item_slices(sh)) {
+        build_items(h, sh);
         h.pc.d_items.release();
     }
 }
@@ -941,7 +1362,8 @@ void ensure_items(HMat &h) {
 }
 
 size_t spmm_lds_bytes(const SpmmShape &sh, int M, int mode) {
-    return lds_bytes(sh.R, std::max(1, chunk_slices(sh, M, mode)), M, mode);
+    const size_t b = stream_bytes(sh.R, item_slices(sh), sh.block / 64, M, mode);
+    return mode == GCI_OP_FP64 ? std::max(b, lower_bytes(sh.R, std::max(1, chunk_slices(sh, M)), M)) : b;
 }
 
 size_t spmm_lds_max() { return LDS_MAX; }
@@ -958,6 +1380,124 @@ void retain_pool() {
     done[dev] = true;
 }
 
+// The full layout of a split-precision matrix: both triangles, each
+// segment's entries in row order with one-byte row deltas, no bank schedule
+// (a full layout's walk makes no LDS atomics).
+void build_full(HMat &h, int R) {
+    Pcsc &pc = h.pc;
+    const int64_t nnz = 2 * h.nnz, n = h.n;
+    hipStream_t s = h.stream;
+    pc.nnz = nnz;
+    GCI_REQUIRE(h.nnz < ((int64_t)1 << 32), "spmm: more than 2^32 - 1 off-diagonal entries");
+    const unsigned eb = grid_for(nnz);
+    SBuf<uint64_t> k1(nnz, s);
+    SBuf<uint32_t> i1(nnz, s);
+    {
+        SBuf<uint64_t> k0(nnz, s);
+        SBuf<uint32_t> i0(nnz, s);
+        const int64_t blocks = (n * 64 + 255) / 256;
+        hipLaunchKernelGGL(k_keys_full, dim3((unsigned)blocks), dim3(256), 0, s, n, h.nnz, h.rp.p, h.col.p, R, k0.p,
+                           i0.p);
+        GCI_HIP(hipGetLastError());
+        if (!sort_pairs_db(k0.p, k1.p, i0.p, i1.p, nnz, s)) {
+            std::swap(k0.p, k1.p);
+            std::swap(i0.p, i1.p);
+        }
+    }
+    // segments and the escape steps before each entry (an exclusive prefix
+    // of escapes(), evaluated on the fly)
+    SBuf<int32_t> flag(nnz, s), escx(nnz, s);
+    SBuf<int64_t> segid(nnz, s);
+    hipLaunchKernelGGL(k_seg_flags, dim3(eb), dim3(256), 0, s, nnz, k1.p, flag.p);
+    GCI_HIP(hipGetLastError());
+    scan_exclusive(flag.p, segid.p, nnz, s);
+    {
+        hipcub::CountingInputIterator<int64_t> ids(0);
+        hipcub::TransformInputIterator<int32_t, EscapesOf, hipcub::CountingInputIterator<int64_t>> escs(
+            ids, EscapesOf{k1.p, flag.p});
+        scan_exclusive_it(escs, escx.p, nnz, s);
+    }
+    int64_t last = 0;
+    int32_t lastf = 0, lastescx = 0;
+    uint64_t lastkeys[2] = {0, 0};
+    GCI_HIP(hipMemcpyAsync(&last, segid.p + nnz - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    GCI_HIP(hipMemcpyAsync(&lastf, flag.p + nnz - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GCI_HIP(hipMemcpyAsync(&lastescx, escx.p + nnz - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GCI_HIP(hipMemcpyAsync(lastkeys, k1.p + nnz - 2, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    GCI_HIP(hipStreamSynchronize(s));
+    const int32_t lastflag[2] = {0, lastf};
+    const int64_t nseg = last + lastf, esc_total = (int64_t)lastescx + escapes(lastkeys, lastflag, 1);
+    GCI_REQUIRE(esc_total < ((int64_t)1 << 31), "spmm: too many row escapes in a full layout");
+    pc.nseg = nseg;
+    SBuf<int32_t> seg_col(nseg, s), spanel(nseg, s), ulen(nseg, s);
+    SBuf<int64_t> ustart(nseg, s);
+    hipLaunchKernelGGL(k_seg_meta, dim3(eb), dim3(256), 0, s, nnz, k1.p, flag.p, segid.p, seg_col.p, ustart.p,
+                       spanel.p);
+    GCI_HIP(hipGetLastError());
+    // segments of a panel, longest first (in stream positions)
+    const int64_t npanel = (n + R - 1) / R;
+    const unsigned sg = grid_for(nseg);
+    SBuf<int64_t> order(nseg, s);
+    {
+        SBuf<uint64_t> okey(nseg, s), okey1(nseg, s);
+        SBuf<int64_t> oval(nseg, s);
+        hipLaunchKernelGGL(k_seg_order_full, dim3(sg), dim3(256), 0, s, nseg, nnz, ustart.p, spanel.p, escx.p,
+                           esc_total, ulen.p, okey.p, oval.p);
+        GCI_HIP(hipGetLastError());
+        sort_pairs(okey.p, okey1.p, oval.p, order.p, nseg, s);
+    }
+    // slices
+    SBuf<int64_t> pslices(npanel + 1, s), pslice0(npanel + 1, s), pfirst(npanel + 1, s);
+    hipLaunchKernelGGL(k_panel_slices, dim3(grid_for(npanel + 1)), dim3(256), 0, s, npanel, nseg, spanel.p, pfirst.p,
+                       pslices.p);
+    GCI_HIP(hipGetLastError());
+    scan_exclusive(pslices.p, pslice0.p, npanel + 1, s);
+    int64_t nslice = 0;
+    GCI_HIP(hipMemcpyAsync(&nslice, pslice0.p + npanel, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    GCI_HIP(hipStreamSynchronize(s));
+    GCI_REQUIRE(nslice < 0x7FFFFFFF, "spmm: too many slices");
+    pc.nslice = nslice;
+    pc.scol.alloc((size_t)nslice * SW);
+    pc.sfirst.alloc((size_t)nslice * SW);
+    SBuf<int32_t> slice_panel(nslice, s), seg_lane(nseg, s);
+    SBuf<int64_t> slane_seg((size_t)nslice * SW, s), slice_steps(nslice + 1, s), seg_slice(nseg, s),
+        pcap(nslice + 1, s);
+    GCI_HIP(hipMemsetAsync(pc.scol.p, 0xFF, (size_t)nslice * SW * sizeof(int32_t), s));  // -1: empty lane
+    GCI_HIP(hipMemsetAsync(pc.sfirst.p, 0, (size_t)nslice * SW * sizeof(uint16_t), s));
+    GCI_HIP(hipMemsetAsync(slice_steps.p, 0, (nslice + 1) * sizeof(int64_t), s));
+    hipLaunchKernelGGL(k_seg_place, dim3(sg), dim3(256), 0, s, nseg, order.p, spanel.p, seg_col.p, ulen.p, pfirst.p,
+                       pslice0.p, pc.scol.p, slane_seg.p, slice_panel.p, seg_slice.p, seg_lane.p, pcap.p);
+    GCI_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_slice_full, dim3(sg), dim3(256), 0, s, nseg, ustart.p, ulen.p, seg_slice.p, seg_lane.p,
+                       k1.p, slice_steps.p, pc.sfirst.p);
+    GCI_HIP(hipGetLastError());
+    pc.sstep.alloc(nslice + 1);
+    scan_exclusive(slice_steps.p, pc.sstep.p, nslice + 1, s);
+    pc.sstep_h.resize(nslice + 1);
+    pc.spanel_h.resize(nslice);
+    GCI_HIP(hipMemcpyAsync(pc.sstep_h.data(), pc.sstep.p, (nslice + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    GCI_HIP(hipMemcpyAsync(pc.spanel_h.data(), slice_panel.p, nslice * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GCI_HIP(hipStreamSynchronize(s));
+    const int64_t nstep = pc.sstep_h[nslice];
+    pc.nstep = nstep;
+    const size_t nst = (size_t)nstep * SE;
+    pc.drow.alloc(nst);
+    GCI_HIP(hipMemsetAsync(pc.drow.p, 0, nst, s));  // tail pads: delta 0, value 0
+    if (h.val.p && h.layout_fp64) {
+        pc.val.alloc(nst);
+        GCI_HIP(hipMemsetAsync(pc.val.p, 0, nst * sizeof(double), s));
+    }
+    pc.hi.alloc(nst);
+    pc.lo.alloc(nst);
+    GCI_HIP(hipMemsetAsync(pc.hi.p, 0, nst * sizeof(float), s));
+    GCI_HIP(hipMemsetAsync(pc.lo.p, 0, nst * sizeof(float), s));
+    hipLaunchKernelGGL(k_fill_full, dim3(eb), dim3(256), 0, s, nnz, k1.p, flag.p, segid.p, ustart.p, escx.p,
+                       seg_slice.p, seg_lane.p, pc.sstep.p, i1.p, h.val.p, h.hi.p, h.lo.p, pc.drow.p, pc.val.p,
+                       pc.hi.p, pc.lo.p);
+    GCI_HIP(hipGetLastError());
+    GCI_HIP(hipStreamSynchronize(s));
+}
+
 void ensure_pcsc(HMat &h) {
     if (h.pc.R) return;
     const SpmmShape sh = spmm_shape();
@@ -965,9 +1505,19 @@ void ensure_pcsc(HMat &h) {
     Pcsc &pc = h.pc;
     // Split-precision matrices (the batched operators, several vectors per
     // product) lay out both triangles: every output is then a column sum
-    // of gathered x, with no LDS atomics, at twice the stream bytes.  FP64
+    // of gathered x, with no LDS atomics, at twice the entries (one-byte
+    // row deltas keep a stored entry at 5 bytes for fp32 values).  FP64
     // matrices (the plain single-vector form) keep the lower triangle.
     pc.full = h.precision == GCI_SPLIT32;
+    if (pc.full) {
+        pc.nseg = pc.nslice = pc.nstep = 0;
+        pc.sstep_h.assign(1, 0);
+        pc.spanel_h.clear();
+        if (h.nnz > 0) build_full(h, R);
+        pc.R = R;
+        build_items(h, sh);
+        return;
+    }
     const int64_t nnz = pc.full ? 2 * h.nnz : h.nnz, n = h.n;
     hipStream_t s = h.stream;
     pc.nnz = nnz;
@@ -1096,7 +1646,7 @@ void ensure_pcsc(HMat &h) {
         GCI_HIP(hipStreamSynchronize(s));
     }
     pc.R = R;
-    build_items(h, sh.target);
+    build_items(h, sh);
 }
 
 // The matrix's work items in device memory (uploaded if missing or stale).
@@ -1138,7 +1688,8 @@ void spmm_noalloc(HMat &h, int m, const double *x, int ldx, double *y, int ldy, 
     if (h.n == 0) return;
     const SpmmShape sh = spmm_shape();
     Pcsc &pc = h.pc;
-    GCI_REQUIRE(pc.R == sh.R && pc.target == sh.target && (pc.items.empty() || pc.d_items.n >= pc.items.size()),
+    GCI_REQUIRE(pc.R == sh.R && pc.target == sh.target && pc.item_slices == item_slices(sh) &&
+                    (pc.items.empty() || pc.d_items.n >= pc.items.size()),
                 "spmm_noalloc: the matrix's SpMM layout is not prepared for the shape in force; call "
                 "gci_spmm_layout first");
     const size_t need = spmm_scratch_need(h, m, mode);
@@ -1160,7 +1711,7 @@ void spmm_noalloc(HMat &h, int m, const double *x, int ldx, double *y, int ldy, 
         // this descriptor write
         hipLaunchKernelGGL(k_put_job, dim3(1), dim3(1), 0, s, job,
                            SpmmJob{pc.scol.p, pc.sstep.p, pc.lrow.p, pc.val.p, pc.hi.p, pc.lo.p, xs, ys, h.n, mc,
-                                   mode, 1, pc.full ? 1 : 0});
+                                   mode, 1, pc.full ? 1 : 0, pc.drow.p, pc.sfirst.p});
         const unsigned pb = grid_for(h.n * M);
         if (mode == GCI_OP_SPLIT_BULK)
             hipLaunchKernelGGL(k_pack<float>, dim3(pb), dim3(256), 0, s, h.n, mc, M, xc, (int64_t)ldx,
@@ -1203,7 +1754,8 @@ void spmm(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode
         for (int md = 0; md < 3; ++md)
             for (int mc = 1; mc <= 8; ++mc)
                 jt[md * 8 + mc - 1] = SpmmJob{pc.scol.p, pc.sstep.p, pc.lrow.p, pc.val.p, pc.hi.p, pc.lo.p,
-                                              pc.stage_x.p, pc.stage_y.p, h.n, mc, md, 1, pc.full ? 1 : 0};
+                                              pc.stage_x.p, pc.stage_y.p, h.n, mc, md, 1, pc.full ? 1 : 0,
+                                              pc.drow.p, pc.sfirst.p};
         pc.d_jobs.alloc(jt.size());
         GCI_HIP(hipMemcpy(pc.d_jobs.p, jt.data(), jt.size() * sizeof(SpmmJob), hipMemcpyHostToDevice));
     }

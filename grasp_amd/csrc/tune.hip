@@ -82,7 +82,8 @@ struct TuneSet {
             ensure_items(h);
             const Pcsc &pc = h.pc;
             jobs.push_back(SpmmJob{pc.scol.p, pc.sstep.p, pc.lrow.p, pc.val.p, pc.hi.p, pc.lo.p, xs[i].p,
-                                   ys[i].p, h.n, std::min(W, m), mode, 1, pc.full ? 1 : 0});
+                                   ys[i].p, h.n, std::min(W, m), mode, 1, pc.full ? 1 : 0, pc.drow.p,
+                                   pc.sfirst.p});
             for (SpmmItem it : pc.items) {
                 it.job = (int32_t)i;
                 items.push_back(it);
