@@ -56,12 +56,57 @@ def test_spmm_shapes_all_widths(shape_guard, shape):
 def test_spmm_segment_extremes(shape_guard, fill, n):
     # a nearly dense block (segments of whole panel height, one slice per
     # panel column group) and a very sparse one (segments of one entry,
-    # slices that are one step long)
+    # slices that are one step long; in the full layouts of the split
+    # matrix, row gaps far above 255 that take escape entries)
     rci.set_spmm_shape(R=1024, block=512, depth=4, nslice=8, target=16384)
     csr = W.clustered_block(n, 41, fill=fill)
     h = HamiltonianBlock.from_csr(csr)
+    h32 = HamiltonianBlock.from_csr(csr, precision=L.GCI_SPLIT32)
     for m in (1, 4, 8):
         _check(h, csr, m, L.GCI_OP_FP64, 400 + m)
+        _check(h32, csr, m, L.GCI_OP_SPLIT_FULL, 500 + m)
+        _check(h32, csr, m, L.GCI_OP_SPLIT_BULK, 600 + m)
+        _check(h32, csr, m, L.GCI_OP_FP64, 700 + m)
+
+
+def _lower_csr(n, entries, seed):
+    """LowerCSR with the given strictly-lower (row, column) pairs (0-based)."""
+    rng = np.random.default_rng(seed)
+    by_row = {}
+    for i, j in entries:
+        assert i > j
+        by_row.setdefault(i, []).append(j)
+    row_end, col, val = [0], [], []
+    for i in range(n):
+        for j in sorted(by_row.get(i, [])):
+            col.append(j + 1)
+            val.append(rng.normal(0.0, 1e-2))
+        col.append(i + 1)
+        val.append(rng.normal(-338.0, 5.0))
+        row_end.append(len(col))
+    return W.LowerCSR(n, np.array(row_end, np.int64), np.array(col, np.int32), np.array(val))
+
+
+def test_spmm_full_layout_escapes(shape_guard):
+    """Row gaps at and around the one-byte delta limit in a full layout: 255
+    (no escape), 256 and 510 (one escape entry of 255 rows), 511 (two), 899
+    (three), a segment whose first row is far into its panel, and one in a
+    later panel.  The running row must land on every entry exactly, in all
+    operator modes of the split matrix and in the FP64 matrix's lower
+    layout."""
+    rci.set_spmm_shape(R=2048, block=512, depth=8, nslice=16, target=131072)
+    n = 5000
+    entries = [(r, 5) for r in (6, 261, 517, 1027, 1538, 2047)]
+    entries += [(r, 0) for r in (300, 600, 1000, 2000, 2047)]
+    entries += [(2040, 1000), (4100, 3000), (4999, 3000), (4999, 4998), (2048, 2047)]
+    csr = _lower_csr(n, entries, 77)
+    h32 = HamiltonianBlock.from_csr(csr, precision=L.GCI_SPLIT32)
+    h64 = HamiltonianBlock.from_csr(csr)
+    for m in (1, 4, 5):
+        _check(h32, csr, m, L.GCI_OP_SPLIT_BULK, 800 + m)
+        _check(h32, csr, m, L.GCI_OP_SPLIT_FULL, 810 + m)
+        _check(h32, csr, m, L.GCI_OP_FP64, 820 + m)
+        _check(h64, csr, m, L.GCI_OP_FP64, 830 + m)
 
 
 def test_spmm_autotune_keeps_parity(shape_guard):
@@ -75,8 +120,9 @@ def test_spmm_autotune_keeps_parity(shape_guard):
 
 
 def test_spmm_layout_overhead(shape_guard):
-    # the bank schedule pads a slice by at most a few positions beyond its
-    # longest segment, so the stream stays close to the stored entries
+    # a full layout pads each slice to its longest segment and adds escape
+    # entries for row gaps above 255; on a clustered block the stream stays
+    # close to the laid-out entries
     rci.set_spmm_shape(R=1024)
     csr = W.clustered_block(6000, 61, fill=0.05)
     h = HamiltonianBlock.from_csr(csr, precision=L.GCI_SPLIT32)
