@@ -16,7 +16,7 @@ Also reported on the same line:
                on the same blocks: eigenpairs/s and the speedup
   accuracy     max relative eigenvalue error of the batched form vs the
                plain FP64 form (total energies EVAL + EAV)
-  roofline     the symmetric SpMM kernel (panel-CSC) inside the timed
+  roofline     the symmetric SpMM kernel (panel-slice) inside the timed
                region: algorithmic bytes / kernel time from HIP events
   serving      continuous batching: Poisson arrivals, p50/p99 latency
   cpu_baseline the CPU restatement (oracle) of MANEIG on a bounded sample
@@ -78,7 +78,7 @@ def precision_form(args, L, rci, synth, sync):
     t_gen = time.time() - t0
     nnz_l = b.nnz_offdiag + b.n
     log(f"precision form: n = {b.n}, nnz_L = {nnz_l:.4e}, generated in {t_gen:.1f}s")
-    # first solve builds the panel-CSC layout (one-time); time the second
+    # first solve builds the panel-slice layout (one-time); time the second
     rci.batch_solve([b], [nev], [args.tol], [eav])
     sync()
     t0 = time.perf_counter()
@@ -339,7 +339,7 @@ def main():
     # eigenvectors land in one page-locked result buffer, reused each step
     # (as a serving loop keeps its result memory)
     vec_out = rci.pinned_empty(sum(b.n * k for b, k in zip(blocks, nev)))
-    # warmup (builds the panel-CSC layouts once)
+    # warmup (builds the panel-slice layouts once)
     for w in range(args.warmup):
         t = time.time()
         rci.batch_solve(blocks, nev, args.tol, eavs, out=vec_out)

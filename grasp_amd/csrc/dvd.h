@@ -247,7 +247,7 @@ class DavidsonEngine {
 };
 
 // Operator profiling (gci_profile_*): per operator mode, launches, summed
-// kernel milliseconds (HIP events around the panel-CSC kernel) and summed
+// kernel milliseconds (HIP events around the panel-slice kernel) and summed
 // algorithmic bytes of the jobs each launch served.
 struct OpProfile {
     bool on = false;

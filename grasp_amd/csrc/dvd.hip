@@ -1114,7 +1114,7 @@ __global__ void k_final_post(DvdProb *P, const double *partial) {
 
 // ---------------------------------------------------- batched operator
 // AB[:, kpass..kpass+nnew) = H B[:, kpass..) for every problem in the stage,
-// through the panel-CSC SpMM (spmm.hip): jobs are made on the device from
+// through the panel-slice SpMM (spmm.hip): jobs are made on the device from
 // the problem states, x is staged row-major (fp32 for the bulk operator),
 // and the FP64 diagonal is applied when the result is unpacked.
 __device__ inline int op_mode(const DvdProb &p) {
