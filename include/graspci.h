@@ -252,9 +252,10 @@ int gci_profile_read(double *stats);
  * SpMM's launch shape: panel height R (1024, 2048 or 4096: rows whose x
  * (and, for lower-triangle layouts, y) stay in LDS; the panel-slice layout
  * is built for it), workgroup size (256, 512, 1024), at most `nslice` slices
- * (64-column groups, whose column tables are staged in LDS) and about
- * `target` entries per work item (a run of one panel's steps), and steps
- * each wavefront keeps in flight `depth` (2, 4, 8).
+ * (64-column groups, whose column tables are staged in LDS) per work item of
+ * a full layout (at most 128) and per LDS chunk of a lower-triangle one,
+ * about `target` entries per work item (whole slices, or a run of steps, of
+ * one panel), and steps each wavefront keeps in flight `depth` (2, 4, 8).
  * One shape is in force per process; a matrix laid out for another R is laid
  * out again at its next use outside a Davidson engine.  Set or tune the shape
  * before solves, not while requests are resident (an engine refuses a change
@@ -266,8 +267,9 @@ typedef struct gci_spmm_shape {
 int gci_spmm_get_shape(gci_spmm_shape *out);
 int gci_spmm_set_shape(const gci_spmm_shape *shape);
 /* The panel-slice layout of a matrix for the shape in force (laid out now if
- * it was not): its column segments, slices and stored entries (padding
- * included: stored / off-diagonal entries is the stream's overhead). */
+ * it was not): its column segments, slices and stored entries (pads and
+ * escape entries included: stored / laid-out entries is the stream's
+ * overhead). */
 int gci_spmm_layout(gci_hmat *h, int64_t *nseg, int64_t *nslice, int64_t *stored);
 /* Mean milliseconds of the SpMM kernel launches of one m-column operator
  * application (mode GCI_OP_*) to every matrix of the set at once -- all
