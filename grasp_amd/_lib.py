@@ -75,6 +75,7 @@ EXPORTS = [
     "gci_server_submit_shared",
     "gci_host_alloc", "gci_host_free",
     "gci_spmm_device_noalloc",
+    "gci_yzk", "gci_maneig_mcdhf",
 ]
 GCI_NOALLOC_SCRATCH_BYTES = 48 << 20
 

@@ -198,6 +198,113 @@ int gci_maneig(gci_hmat *h, int nvec, double *eval, double *evec, gci_dvd_stats 
     });
 }
 
+int gci_maneig_mcdhf(gci_hmat *h, int nvect, int nlev, const int32_t *iccmin, int warm, double *eval,
+                     double *evec, gci_dvd_stats *st) {
+    return guarded([&] {
+        GCI_REQUIRE(h && iccmin && eval && evec, "gci_maneig_mcdhf: null argument");
+        GCI_REQUIRE(nlev >= 1 && nvect >= 1, "gci_maneig_mcdhf: nlev and nvect must be positive");
+        const int64_t n = h->n;
+        const int nvex = (int)std::min<int64_t>(nvect, n);
+        for (int j = 0; j < nlev; ++j)
+            GCI_REQUIRE(iccmin[j] >= 1 && iccmin[j] <= nvex, "gci_maneig_mcdhf: level outside 1..min(nvect, NCF)");
+        const auto t0 = std::chrono::steady_clock::now();
+        gci_dvd_stats local{};
+        if (n == 1) {
+            // rmcdhf90/maneig.f90:78-82: EVAL = 0, EVEC = 1
+            for (int j = 0; j < nlev; ++j) {
+                eval[j] = 0.0;
+                evec[j] = 1.0;
+            }
+        } else {
+            // NVEX = NIV = min(NVECT, NCF), LIM = min(NCF, 2 NVECT + 40),
+            // MAXITR = min(200 NVECT, NCF), N1000 = 4000
+            // (rmcdhf90/maneig.f90:103-113)
+            GCI_REQUIRE(nvex <= MAXNUME, "gci_maneig_mcdhf: nvect must be at most " + std::to_string(MAXNUME));
+            std::vector<double> all_val(nvex), all_vec;
+            std::vector<double> iv;
+            const int64_t n1000 = 4000;
+            if (!warm || n <= n1000) {
+                // INIEST2(N1000) (rmcdhf90/maneig.f90:115-116): for NCF <= 4000
+                // the whole block, which is then the answer
+                const int64_t ns = std::min<int64_t>(n1000, n);
+                std::vector<double> w, z;
+                lowest_of_leading_block(*h, ns, nvex, w, z);
+                iv.assign((size_t)n * nvex, 0.0);
+                for (int j = 0; j < nvex; ++j) {
+                    std::copy(z.begin() + (size_t)j * ns, z.begin() + (size_t)(j + 1) * ns, iv.begin() + (size_t)j * n);
+                    all_val[j] = w[j];
+                }
+            } else {
+                // warm start: the previous iteration's vectors (orthonormal)
+                // at columns ICCMIN (rmcdhf90/maneig.f90:118-124).  The
+                // reference leaves the other columns as allocated; here column
+                // c holds the unit vector at the c-th smallest diagonal
+                // element.
+                iv.assign((size_t)n * nvex, 0.0);
+                std::vector<char> given(nvex, 0);
+                for (int j = 0; j < nlev; ++j) {
+                    const int c = iccmin[j] - 1;
+                    std::copy(evec + (size_t)j * n, evec + (size_t)(j + 1) * n, iv.begin() + (size_t)c * n);
+                    given[c] = 1;
+                }
+                std::vector<double> d(n);
+                GCI_HIP(hipStreamSynchronize(h->stream));
+                GCI_HIP(hipMemcpy(d.data(), h->diag.p, n * sizeof(double), hipMemcpyDeviceToHost));
+                std::vector<int64_t> idx(n);
+                for (int64_t i = 0; i < n; ++i) idx[i] = i;
+                std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return d[a] < d[b]; });
+                for (int c = 0; c < nvex; ++c)
+                    if (!given[c]) iv[(size_t)c * n + idx[c]] = 1.0;
+                // INITDVD takes its start block as orthonormal
+                // (libdvd90/dvdson.f90:886-905): a filled column is made
+                // orthogonal to the given columns and the filled ones before
+                // it, two classical Gram-Schmidt passes
+                for (int c = 0; c < nvex; ++c) {
+                    if (given[c]) continue;
+                    double *v = iv.data() + (size_t)c * n;
+                    for (int pass = 0; pass < 2; ++pass)
+                        for (int q = 0; q < nvex; ++q) {
+                            if (q == c || (!given[q] && q > c)) continue;
+                            const double *u = iv.data() + (size_t)q * n;
+                            double dot = 0.0;
+                            for (int64_t i = 0; i < n; ++i) dot += u[i] * v[i];
+                            for (int64_t i = 0; i < n; ++i) v[i] -= dot * u[i];
+                        }
+                    double nrm = 0.0;
+                    for (int64_t i = 0; i < n; ++i) nrm += v[i] * v[i];
+                    nrm = std::sqrt(nrm);
+                    for (int64_t i = 0; i < n; ++i) v[i] /= nrm;
+                }
+            }
+            if (n > n1000) {
+                // GDVD with MBLOCK = 1, CRITE 1e-17, CRITC = CRITR = 1e-9
+                // (rmcdhf90/maneig.f90:139-162)
+                gci_dvd_params p{};
+                p.nvec = nvex;
+                p.lim = (int)std::min<int64_t>(n, 2 * (int64_t)nvect + 40);
+                p.maxiter = (int)std::min<int64_t>(200 * (int64_t)nvect, n);
+                p.mblock = 1;
+                p.mode = GCI_OP_FP64;
+                DvdSpec sp = resolve(*h, &p);
+                all_vec.assign((size_t)n * nvex, 0.0);
+                solve_one(*h, sp, iv.data(), nvex, all_val.data(), all_vec.data(), nullptr, &local);
+                if (local.ierr & 2048) throw Error{GCI_ERR_NOCONV, "maneig (rmcdhf90): DVDSON wrong (IERR 2048)"};
+            } else {
+                all_vec.swap(iv);
+            }
+            // the levels ICCMIN (rmcdhf90/maneig.f90:171-176)
+            for (int j = 0; j < nlev; ++j) {
+                const int c = iccmin[j] - 1;
+                eval[j] = all_val[c];
+                std::copy(all_vec.begin() + (size_t)c * n, all_vec.begin() + (size_t)(c + 1) * n, evec + (size_t)j * n);
+            }
+        }
+        sign_fix(n, nlev, evec);  // rmcdhf90/maneig.f90:179-196
+        local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (st) *st = local;
+    });
+}
+
 // Batched form: every block's Davidson runs interleaved in one engine
 // (split-precision bulk operator when the block holds hi/lo, block
 // Davidson with MBLOCK = NEV, convergence on the relative residual

@@ -117,7 +117,8 @@ __global__ __launch_bounds__(256) void k_rk_table(RkConst kc, const YDesc *__res
                                                   const double *__restrict__ P, const double *__restrict__ Q, int ld,
                                                   const int *__restrict__ mf, const double *__restrict__ gr,
                                                   const double *__restrict__ grp, const double *__restrict__ grpor,
-                                                  double *__restrict__ vals) {
+                                                  double *__restrict__ vals, double *__restrict__ yk_out,
+                                                  int *__restrict__ mtp_out) {
     __shared__ double r[NMAXG], rp[NMAXG], rpor[NMAXG];
     __shared__ double yk[NMAXG];
     __shared__ double temp[NMAXG + 8];   // r^k RHOP, the outward march's integrand
@@ -235,6 +236,11 @@ __global__ __launch_bounds__(256) void k_rk_table(RkConst kc, const YDesc *__res
     }
     if (t == 0) ymtp = mtp;
     __syncthreads();
+    // the Hartree function itself, TB(1:N) and MTP as YZK leaves them
+    if (yk_out) {
+        for (int ii = t; ii < n; ii += blockDim.x) yk_out[(size_t)blockIdx.x * n + ii] = yk[ii];
+        if (t == 0) mtp_out[blockIdx.x] = mtp;
+    }
     // ---- SLATER (slater.f90:278-294) for every (a, c) sharing this Y^k
     double *ta = tas[wave];
     for (int e = y.e0 + wave; e < y.e1; e += blockDim.x / 64) {
@@ -445,9 +451,40 @@ int gci_rk_table(const gci_grid *grid, const gci_orbitals *orb, int32_t *j2max_o
         auto dout = to_dev(outi.data(), outi.size());
         DBuf<double> dv(n);
         hipLaunchKernelGGL(k_rk_table, dim3((unsigned)ys.size()), dim3(256), 0, 0, kc, dys.p, dac.p, dout.p, dP.p,
-                           dQ.p, orb->ld, dmf.p, dr.p, drp.p, drpor.p, dv.p);
+                           dQ.p, orb->ld, dmf.p, dr.p, drp.p, drpor.p, dv.p, nullptr, nullptr);
         GCI_HIP(hipGetLastError());
         GCI_HIP(hipMemcpy(vals, dv.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+int gci_yzk(const gci_grid *grid, const gci_orbitals *orb, int npairs, const int32_t *k, const int32_t *b,
+            const int32_t *d, double *yk, int32_t *mtp) {
+    return guarded([&] {
+        check_inputs(grid, orb);
+        GCI_REQUIRE(npairs >= 0, "gci_yzk: negative count");
+        if (npairs == 0) return;
+        GCI_REQUIRE(k && b && d && yk && mtp, "gci_yzk: null argument");
+        const int nw = orb->nw;
+        std::vector<YDesc> ys(npairs);
+        for (int q = 0; q < npairs; ++q) {
+            GCI_REQUIRE(k[q] >= 0 && k[q] <= 20, "gci_yzk: k out of range (0..20)");
+            GCI_REQUIRE(b[q] >= 1 && b[q] <= nw && d[q] >= 1 && d[q] <= nw, "gci_yzk: orbital index out of range");
+            ys[q] = YDesc{k[q], b[q] - 1, d[q] - 1, 0, 0};  // no (a, c) entries: the function only
+        }
+        RkConst kc = make_const(grid);
+        const int n = grid->n, n10 = n + 10;
+        auto dP = to_dev(orb->P, (size_t)nw * orb->ld);
+        auto dQ = to_dev(orb->Q, (size_t)nw * orb->ld);
+        auto dmf = to_dev(orb->mf, nw);
+        auto dr = to_dev(grid->r, n10), drp = to_dev(grid->rp, n10), drpor = to_dev(grid->rpor, n10);
+        auto dys = to_dev(ys.data(), ys.size());
+        DBuf<double> dyk((size_t)npairs * n);
+        DBuf<int> dm(npairs);
+        hipLaunchKernelGGL(k_rk_table, dim3((unsigned)npairs), dim3(256), 0, 0, kc, dys.p, nullptr, nullptr, dP.p,
+                           dQ.p, orb->ld, dmf.p, dr.p, drp.p, drpor.p, nullptr, dyk.p, dm.p);
+        GCI_HIP(hipGetLastError());
+        GCI_HIP(hipMemcpy(yk, dyk.p, dyk.bytes(), hipMemcpyDeviceToHost));
+        GCI_HIP(hipMemcpy(mtp, dm.p, dm.bytes(), hipMemcpyDeviceToHost));
     });
 }
 

@@ -99,6 +99,17 @@ module gci_bind
          integer(c_int) :: st
       end function gci_rk_table
 
+      function gci_yzk(grid, orb, npairs, k, b, d, yk, mtp) bind(C, name="gci_yzk") result(st)
+         import :: c_int, c_int32_t, c_double, gci_grid, gci_orbitals
+         type(gci_grid), intent(in) :: grid
+         type(gci_orbitals), intent(in) :: orb
+         integer(c_int), value :: npairs
+         integer(c_int32_t), intent(in) :: k(*), b(*), d(*)
+         real(c_double) :: yk(*)
+         integer(c_int32_t) :: mtp(*)
+         integer(c_int) :: st
+      end function gci_yzk
+
       function gci_one_body(grid, orb, zz, c, iab) bind(C, name="gci_one_body") result(st)
          import :: c_int, c_double, gci_grid, gci_orbitals
          type(gci_grid), intent(in) :: grid
@@ -213,6 +224,17 @@ module gci_bind
          type(gci_dvd_stats) :: stats
          integer(c_int) :: st
       end function gci_maneig
+
+      function gci_maneig_mcdhf(h, nvect, nlev, iccmin, warm, eval, evec, stats) &
+         bind(C, name="gci_maneig_mcdhf") result(st)
+         import :: c_int, c_int32_t, c_double, c_ptr, gci_dvd_stats
+         type(c_ptr), value :: h
+         integer(c_int), value :: nvect, nlev, warm
+         integer(c_int32_t), intent(in) :: iccmin(*)
+         real(c_double) :: eval(*), evec(*)
+         type(gci_dvd_stats) :: stats
+         integer(c_int) :: st
+      end function gci_maneig_mcdhf
 
       function gci_batch_solve(nblocks, blocks, nev, tol, eav, eval, eval_off, evec, evec_off, stats) &
          bind(C, name="gci_batch_solve") result(st)

@@ -19,6 +19,16 @@
 !                  the arrays of cteilsrk_C / kkstart_C (INDTEIRK, VALTEIRK,
 !                  KSTART(0:KMAX)) with the reference's order and packing
 !
+! and, for the SCF program rmcdhf90 (the second caller of the same pieces):
+!
+!   GCI_MCDHF_MANEIG  rmcdhf90's MANEIG solve of the loaded block
+!                  (rmcdhf90/maneig.f90:78-200): levels ICCMIN of the lowest
+!                  min(NVECT, NCF), cold (INIEST2 over 4000 CSFs) or warm
+!                  (previous EVAL/EVEC) start, GDVD above 4000 CSFs
+!   GCI_YZK_BATCH  YZK(K, I, J) (lib9290/yzk.f90) for all pairs of an SCF
+!                  iteration's potentials (rmcdhf90/ypot.f90:75,
+!                  xpot.f90:80): TB(1:N) and MTP per pair
+!
 ! Like the reference, these routines STOP on an error (here with the
 ! engine's message); the handle plays the role of hmat_C's module state.
 !-----------------------------------------------------------------------
@@ -28,7 +38,7 @@ module gci_rci
    implicit none
    private
    public :: gci_start, gci_load_hmat, gci_free_hmat, gci_set_op_mode, gci_spicmv2, gci_maneig_solve, &
-             gci_genintrk, gci_check
+             gci_genintrk, gci_check, gci_mcdhf_maneig, gci_yzk_batch
 
    type(c_ptr), save :: hmat = c_null_ptr
    integer, save :: hmat_n = 0
@@ -114,6 +124,71 @@ contains
       nmv = st%nmv
       ierr = st%ierr
    end subroutine gci_maneig_solve
+
+   ! rmcdhf90's MANEIG for the loaded block: EVAL(NLEV), EVEC(NCF, NLEV) of
+   ! the levels ICCMIN(1:NLEV) (H - EAV); on entry, with WARM, the previous
+   ! SCF iteration's values for those levels (the start vectors).
+   subroutine gci_mcdhf_maneig(nvect, nlev, iccmin, warm, eval, evec, nloops, nmv, ierr)
+      integer, intent(in) :: nvect, nlev
+      integer, intent(in) :: iccmin(nlev)
+      logical, intent(in) :: warm
+      real(c_double), intent(inout) :: eval(nlev)
+      real(c_double), intent(inout) :: evec(hmat_n, nlev)
+      integer, intent(out) :: nloops, nmv, ierr
+      type(gci_dvd_stats) :: st
+      integer(c_int32_t) :: lev(nlev)
+      integer(c_int) :: iw
+      lev = int(iccmin, c_int32_t)
+      iw = 0
+      if (warm) iw = 1
+      call gci_check(gci_maneig_mcdhf(hmat, int(nvect, c_int), int(nlev, c_int), lev, iw, eval, evec, st), &
+                     'GCI_MCDHF_MANEIG')
+      nloops = st%nloops
+      nmv = st%nmv
+      ierr = st%ierr
+   end subroutine gci_mcdhf_maneig
+
+   ! YZK for NPAIRS pairs at once: YK(1:NGRID, q) and MTP(q) as YZK(K(q),
+   ! I(q), J(q)) leaves TB and MTP in module tatb_C.  Grid and orbitals as
+   ! for GCI_GENINTRK.
+   subroutine gci_yzk_batch(ngrid, h, r, rp, rpor, nw, ld, pf, qf, mf, nak, nkl, nkj, npairs, k, i, j, yk, mtp)
+      integer, intent(in) :: ngrid, nw, ld, npairs
+      real(c_double), intent(in) :: h
+      real(c_double), intent(in), target :: r(*), rp(*), rpor(*)
+      real(c_double), intent(in), target :: pf(ld, nw), qf(ld, nw)
+      integer, intent(in) :: mf(nw), nak(nw), nkl(nw), nkj(nw)
+      integer, intent(in) :: k(npairs), i(npairs), j(npairs)
+      real(c_double), intent(out) :: yk(ngrid, npairs)
+      integer, intent(out) :: mtp(npairs)
+      type(gci_grid) :: g
+      type(gci_orbitals) :: o
+      integer(c_int32_t), allocatable, target :: imf(:), inak(:), inkl(:), inkj(:)
+      integer(c_int32_t) :: kk(npairs), ii(npairs), jj(npairs), mm(npairs)
+      allocate (imf(nw), inak(nw), inkl(nw), inkj(nw))
+      imf = int(mf, c_int32_t)
+      inak = int(nak, c_int32_t)
+      inkl = int(nkl, c_int32_t)
+      inkj = int(nkj, c_int32_t)
+      g%n = int(ngrid, c_int)
+      g%h = h
+      g%accy = h**6
+      g%r = c_loc(r)
+      g%rp = c_loc(rp)
+      g%rpor = c_loc(rpor)
+      o%nw = int(nw, c_int)
+      o%ld = int(ld, c_int)
+      o%p = c_loc(pf)
+      o%q = c_loc(qf)
+      o%mf = c_loc(imf)
+      o%nak = c_loc(inak)
+      o%nkl = c_loc(inkl)
+      o%nkj = c_loc(inkj)
+      kk = int(k, c_int32_t)
+      ii = int(i, c_int32_t)
+      jj = int(j, c_int32_t)
+      call gci_check(gci_yzk(g, o, int(npairs, c_int), kk, ii, jj, yk, mm), 'GCI_YZK_BATCH')
+      mtp = int(mm)
+   end subroutine gci_yzk_batch
 
    ! GENINTRK: every R^k(ab;cd) the orbitals can form, in the reference's
    ! order, as packed keys INDTEIRK and values VALTEIRK (1..NINT), with the

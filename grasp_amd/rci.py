@@ -133,6 +133,25 @@ def maneig(block: HamiltonianBlock, nvec: int):
     return ev, evec, {"nloops": st.nloops, "nmv": st.nmv, "ierr": st.ierr, "seconds": st.seconds}
 
 
+def maneig_mcdhf(block: HamiltonianBlock, nvect: int, iccmin, warm=None):
+    """MANEIG of the SCF program (rmcdhf90/maneig.f90:78-200): the levels
+    iccmin (1-based) of the lowest min(nvect, NCF); warm = (eval, evec) of
+    the previous SCF iteration for those levels, or None for a cold start
+    (INIEST2 over the leading 4000 CSFs).  Returns (eval, evec, stats)."""
+    n = block.n
+    lev = np.ascontiguousarray(iccmin, dtype=np.int32)
+    nlev = len(lev)
+    ev = np.zeros(nlev)
+    evec = np.zeros((n, nlev), order="F")
+    if warm is not None:
+        ev[:] = warm[0]
+        evec[:] = warm[1]
+    st = L.GciDvdStats()
+    check(L.lib().gci_maneig_mcdhf(block.handle, C.c_int(nvect), C.c_int(nlev), i32p(lev),
+                                   C.c_int(int(warm is not None)), dp(ev), dp(evec), C.byref(st)))
+    return ev, evec, {"nloops": st.nloops, "nmv": st.nmv, "ierr": st.ierr, "seconds": st.seconds}
+
+
 def _grid_struct(r, rp, rpor, n, h):
     r = np.ascontiguousarray(r, dtype=np.float64)
     rp = np.ascontiguousarray(rp, dtype=np.float64)
@@ -174,6 +193,21 @@ def genintrk(r, rp, rpor, n, h, orbitals):
     check(L.lib().gci_rk_table(C.byref(g), C.byref(o), C.byref(j2), i32p(kst), i32p(keys), dp(vals),
                                C.c_int64(nk), C.byref(cnt)))
     return kst[: j2.value + 2].copy(), keys[:nk], vals[:nk]
+
+
+def yzk(r, rp, rpor, n, h, orbitals, k, b, d):
+    """YZK (lib9290/yzk.f90) for a batch of (k, b, d), 1-based orbitals:
+    returns (yk[npairs, n], mtp[npairs]), TB(1:N) and MTP per pair."""
+    g, keep_g = _grid_struct(r, rp, rpor, n, h)
+    o, keep_o = _orb_struct(orbitals.P, orbitals.Q, orbitals.mf, orbitals.nak, orbitals.nkl, orbitals.nkj)
+    k = np.ascontiguousarray(k, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.int32)
+    d = np.ascontiguousarray(d, dtype=np.int32)
+    npairs = len(k)
+    yk = np.zeros((npairs, n))
+    mtp = np.zeros(npairs, dtype=np.int32)
+    check(L.lib().gci_yzk(C.byref(g), C.byref(o), C.c_int(npairs), i32p(k), i32p(b), i32p(d), dp(yk), i32p(mtp)))
+    return yk, mtp
 
 
 def one_body(r, rp, rpor, n, h, orbitals, zz, c):

@@ -77,6 +77,15 @@ typedef struct {
 int gci_rk_table(const gci_grid *grid, const gci_orbitals *orb, int32_t *j2max,
                  int32_t *kstart, int32_t *keys, double *vals, int64_t cap, int64_t *n);
 
+/* Hartree functions for a batch of (k, b, d) (lib9290/yzk.f90:3-223):
+ * yk[q*n + i] = TB(i+1) and mtp[q] = MTP after YZK(k[q], b[q], d[q]),
+ * orbitals 1-based.  Bit-identical to YZK (same marches, order and
+ * association).  The reference's SCF program calls YZK one pair at a time
+ * for its direct and exchange potentials (rmcdhf90/ypot.f90:75,
+ * xpot.f90:80); this is that seam for a whole iteration's pairs. */
+int gci_yzk(const gci_grid *grid, const gci_orbitals *orb, int npairs, const int32_t *k, const int32_t *b,
+            const int32_t *d, double *yk, int32_t *mtp);
+
 /* One-body Dirac integrals I(a,b) = RINTI(a,b,0) (lib9290/rinti.f90:49-103)
  * for every same-kappa pair a <= b, as IABINT memoises them
  * (rci90/iabint.f90).  zz: -r V_nuc(r), n entries (lib9290/nucpot.f90).
@@ -211,6 +220,23 @@ int gci_davidson(gci_hmat *h, const gci_dvd_params *p, double *eval, double *eve
  * eigenvectors sign-fixed so the largest component is positive
  * (maneig.f90:532-555). */
 int gci_maneig(gci_hmat *h, int nvec, double *eval, double *evec, gci_dvd_stats *st);
+
+/* MANEIG of the SCF program rmcdhf90 (rmcdhf90/maneig.f90:78-200), the
+ * second caller of GDVD(SPICMV2) (:161).  Computes the lowest
+ * NVEX = min(nvect, NCF) eigenpairs (nvect = NCMAXBLK, at most 16) and
+ * returns the levels iccmin[0..nlev) (1-based, ICCMIN): eval[nlev],
+ * evec n x nlev, sign-fixed (:179-196).  Cold start (warm == 0, the first
+ * SCF iteration) or NCF <= 4000: INIEST2 over the leading min(4000, NCF)
+ * block, which for NCF <= 4000 is the answer.  Warm start: eval/evec hold
+ * the previous iteration's levels on entry (orthonormal vectors, as INITDVD
+ * takes its start block unchanged) and start GDVD (:118-124); the reference
+ * leaves WORK's other start columns as allocated, here column c is the unit
+ * vector at the c-th smallest diagonal element, orthonormalised against the
+ * given columns and the filled ones before it.  GDVD runs with
+ * LIM = min(NCF, 2*nvect+40), MAXITR = min(200*nvect, NCF), MBLOCK = 1,
+ * CRITE 1e-17, CRITC = CRITR = 1e-9 (:104-162). */
+int gci_maneig_mcdhf(gci_hmat *h, int nvect, int nlev, const int32_t *iccmin, int warm, double *eval,
+                     double *evec, gci_dvd_stats *st);
 
 /* --------------------------------------------- batched + serving
  * Many blocks at once: each block's Davidson iterations are interleaved on

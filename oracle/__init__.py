@@ -233,6 +233,25 @@ def maneig(row_end, col, val, eav, nvec, nthreads: int = 1):
     return ev, evec.reshape(nvec, ncf).T, nl.value, nm.value, rc
 
 
+def maneig_mcdhf(row_end, col, val, eav, nvect, iccmin, warm=None, nthreads: int = 1):
+    """rmcdhf90's MANEIG for levels iccmin (1-based).  warm: None (cold
+    start) or (eval, evec) of the previous iteration for those levels.
+    Returns (eval of H - EAV, evec, nloops, nmv, rc)."""
+    ncf = len(row_end) - 1
+    lev = np.ascontiguousarray(iccmin, dtype=np.int32)
+    nlev = len(lev)
+    ev = np.zeros(nlev)
+    evec = np.zeros(ncf * nlev)
+    if warm is not None:
+        ev[:] = warm[0]
+        evec[:] = np.asfortranarray(warm[1]).ravel(order="F")
+    nl, nm = C.c_int(0), C.c_int(0)
+    rc = lib().orc_maneig_mcdhf(C.c_int(ncf), i64p(row_end), i32p(col), dp(val), C.c_double(eav),
+                                C.c_int(nvect), C.c_int(nlev), i32p(lev), C.c_int(int(warm is not None)),
+                                dp(ev), dp(evec), C.c_int(nthreads), C.byref(nl), C.byref(nm))
+    return ev, evec.reshape(nlev, ncf).T, nl.value, nm.value, rc
+
+
 def gdvd(row_end, col, val, diag, nvec, lim, niv_vectors, mblock=1, crite=1e-17, critc=1e-9,
          critr=1e-9, maxiter=None, nthreads: int = 1):
     """GDVD/DVDSON on a (pre-shifted) CSR; returns (eval, evec, nloops, nmv, ierr)."""

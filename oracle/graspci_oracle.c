@@ -1413,3 +1413,129 @@ int orc_maneig(int ncf, const int64_t *row_end, const int32_t *col, const double
     free(diag);
     return rc;
 }
+
+/* rmcdhf90/maneig.f90:78-200: MANEIG of the SCF program, the second caller
+ * of GDVD(SPICMV2).  Levels iccmin[0..nlev) (1-based, each <= NVEX) of the
+ * block; NVECT = NCMAXBLK; NVEX = NIV = min(NVECT, NCF); LIM = min(NCF,
+ * 2*NVECT+40) (:104); MAXITR = min(200*NVECT, NCF) (:112); N1000 = 4000.
+ * Cold start (warm == 0, DVDFIRST) or NCF <= 4000: INIEST2(4000) (:115-116),
+ * which for NCF <= 4000 is the whole answer.  Warm start: the previous
+ * iteration's vectors evec[j] (and eval[j]) placed at column iccmin[j]
+ * (:118-124).  The given vectors must be orthonormal: INITDVD takes its
+ * start block as it is.  The reference leaves the remaining columns of WORK
+ * as they were allocated; here column c then holds the unit vector at the
+ * c-th smallest diagonal element (first index on ties), orthonormalised
+ * against the given columns and the filled ones before it.  NCF > 4000: GDVD with
+ * MBLOCK = 1, CRITE 1e-17, CRITC = CRITR = 1e-9 (:139-162).  Output: eval
+ * and evec of the levels iccmin (:171-176), sign-fixed as :179-196.  The
+ * CSR is the unshifted H; its diagonal is shifted by -eav here. */
+int orc_maneig_mcdhf(int ncf, const int64_t *row_end, const int32_t *col, const double *val,
+                     double eav, int nvect, int nlev, const int32_t *iccmin, int warm,
+                     double *eval, double *evec, int nthreads, int *nloops, int *nmv)
+{
+    *nloops = 0;
+    *nmv = 0;
+    if (ncf == 1) {
+        for (int j = 0; j < nlev; ++j) {
+            eval[j] = 0.0;
+            evec[j] = 1.0;
+        }
+        return 0;
+    }
+    int64_t nnz = row_end[ncf];
+    double *sval = malloc(sizeof(double) * (size_t)(nnz ? nnz : 1));
+    memcpy(sval, val, sizeof(double) * (size_t)nnz);
+    double *diag = malloc(sizeof(double) * ncf);
+    for (int i = 1; i <= ncf; ++i) {
+        F1(sval, row_end[i]) = F1(sval, row_end[i]) - eav;
+        F1(diag, i) = F1(sval, row_end[i]);
+    }
+    const int nvex = nvect < ncf ? nvect : ncf;
+    const int lim = ncf < 2 * nvect + 40 ? ncf : 2 * nvect + 40;
+    const int maxitr = nvect * 200 < ncf ? nvect * 200 : ncf;
+    double *work = calloc((size_t)orc_gdvd_worksize(ncf, lim, nvex), sizeof(double));
+    int rc = 0;
+    if (!warm || ncf <= 4000) {
+        const int ns = ncf < 4000 ? ncf : 4000;
+        double *a = calloc((size_t)ns * ns, sizeof(double)), *w = malloc(sizeof(double) * ns);
+        for (int j = 1; j <= ns; ++j)
+            for (int64_t ir = row_end[j - 1] + 1; ir <= row_end[j]; ++ir) {
+                int i = F1(col, ir);
+                a[(size_t)(j - 1) * ns + (i - 1)] = F1(sval, ir);
+                a[(size_t)(i - 1) * ns + (j - 1)] = F1(sval, ir);
+            }
+        double *zv = malloc(sizeof(double) * (size_t)ns * nvex);
+        rc = orc_spevx_lowest(ns, a, nvex, w, zv, nthreads);
+        for (int j = 0; j < nvex; ++j) {
+            memcpy(work + (size_t)j * ncf, zv + (size_t)j * ns, sizeof(double) * ns);
+            work[(size_t)ncf * nvex + j] = w[j];
+        }
+        free(zv);
+        free(a);
+        free(w);
+    } else {
+        /* columns not given by the caller: unit vectors at the smallest
+         * diagonal elements, in ascending order */
+        int *given = calloc(nvex, sizeof(int));
+        for (int j = 0; j < nlev; ++j) {
+            const int c = iccmin[j] - 1;
+            memcpy(work + (size_t)c * ncf, evec + (size_t)j * ncf, sizeof(double) * ncf);
+            work[(size_t)ncf * nvex + c] = eval[j];
+            given[c] = 1;
+        }
+        char *used = calloc(ncf, 1);
+        for (int c = 0; c < nvex; ++c) {
+            int best = -1;
+            for (int i = 0; i < ncf; ++i)
+                if (!used[i] && (best < 0 || diag[i] < diag[best])) best = i;
+            used[best] = 1;
+            if (!given[c]) work[(size_t)c * ncf + best] = 1.0;
+        }
+        /* INITDVD takes its start vectors as orthonormal (dvdson.f90:
+         * 886-905 orthonormalises only the vectors it adds), so a filled
+         * column is made orthogonal to every given column and to the
+         * filled columns before it: two classical Gram-Schmidt passes */
+        for (int c = 0; c < nvex; ++c) {
+            if (given[c]) continue;
+            double *v = work + (size_t)c * ncf;
+            for (int pass = 0; pass < 2; ++pass)
+                for (int q = 0; q < nvex; ++q) {
+                    if (q == c || (!given[q] && q > c)) continue;
+                    const double *u = work + (size_t)q * ncf;
+                    double dot = 0.0;
+                    for (int i = 0; i < ncf; ++i) dot += u[i] * v[i];
+                    for (int i = 0; i < ncf; ++i) v[i] -= dot * u[i];
+                }
+            double nrm = 0.0;
+            for (int i = 0; i < ncf; ++i) nrm += v[i] * v[i];
+            nrm = sqrt(nrm);
+            for (int i = 0; i < ncf; ++i) v[i] /= nrm;
+        }
+        free(used);
+        free(given);
+    }
+    if (rc == 0 && ncf > 4000)
+        rc = orc_gdvd(ncf, row_end, col, sval, lim, diag, nvex, nvex, 1, 1.0e-17, 1.0e-9, 1.0e-9, maxitr,
+                      work, nthreads, nloops, nmv);
+    for (int j = 0; j < nlev; ++j) {
+        const int c = iccmin[j] - 1;
+        eval[j] = work[(size_t)ncf * nvex + c];
+        memcpy(evec + (size_t)j * ncf, work + (size_t)c * ncf, sizeof(double) * ncf);
+    }
+    for (int j = 0; j < nlev; ++j) {
+        double *v = evec + (size_t)j * ncf, amax = 0.0;
+        int ia = 0;
+        for (int i = 0; i < ncf; ++i) {
+            double wa = fabs(v[i]);
+            if (wa <= amax) continue;
+            amax = wa;
+            ia = i;
+        }
+        if (v[ia] >= 0.0) continue;
+        for (int i = 0; i < ncf; ++i) v[i] = -v[i];
+    }
+    free(work);
+    free(sval);
+    free(diag);
+    return rc;
+}

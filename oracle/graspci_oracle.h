@@ -156,6 +156,15 @@ int orc_maneig(int ncf, const int64_t *row_end, const int32_t *col,
                const double *val, double eav, int nvec, double *eval,
                double *evec, int nthreads, int *nloops, int *nmv);
 
+/* MANEIG of rmcdhf90 (rmcdhf90/maneig.f90:78-200), the SCF program's
+ * eigensolve: levels iccmin[0..nlev) (1-based) of the block, cold start
+ * (INIEST2 with N1000 = 4000) or warm start from the previous eval/evec
+ * (in/out), GDVD above 4000 CSFs with LIM = min(NCF, 2*NVECT+40),
+ * MAXITR = min(200*NVECT, NCF).  Returns 0 or the DVDSON error code. */
+int orc_maneig_mcdhf(int ncf, const int64_t *row_end, const int32_t *col, const double *val,
+                     double eav, int nvect, int nlev, const int32_t *iccmin, int warm,
+                     double *eval, double *evec, int nthreads, int *nloops, int *nmv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -231,6 +231,49 @@ def test_maneig_dense_and_davidson_agree():
         assert vec[np.argmax(np.abs(vec[:, j])), j] > 0
 
 
+def test_maneig_mcdhf_levels_and_branches():
+    """rmcdhf90's MANEIG: dense INIEST2 up to 4000 CSFs (levels ICCMIN picked
+    out of the lowest NVECT, skipping one), and the warm-started GDVD above
+    4000, against the library eigensolvers (numpy eigh, scipy eigsh)."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.linalg import eigsh
+
+    csr = W.clustered_block(600, 21, fill=0.06, sorted_diag=True)
+    eav = float(np.mean(csr.val[csr.row_end[1:] - 1]))
+    ev, vec, nl, nm, rc = O.maneig_mcdhf(csr.row_end, csr.col, csr.val, eav, 4, [2, 4], nthreads=4)
+    w, z = np.linalg.eigh(csr.dense() - eav * np.eye(csr.n))
+    assert rc == 0 and nl == 0
+    assert np.allclose(ev, w[[1, 3]], atol=1e-10)
+    for j, lev in enumerate([1, 3]):
+        assert abs(abs(vec[:, j] @ z[:, lev]) - 1.0) < 1e-9
+        assert vec[np.argmax(np.abs(vec[:, j])), j] > 0
+    # warm start above 4000 CSFs from the previous SCF iteration's vectors
+    # (the same block with off-diagonals scaled by 0.99); level 2 starts
+    # from its unit vector
+    n = 4200
+    big = W.clustered_block(n, 22, fill=0.02, sorted_diag=True)
+    eav = float(np.mean(big.val[big.row_end[1:] - 1]))
+    rows = np.repeat(np.arange(n), np.diff(big.row_end))
+    cols = big.col.astype(np.int64) - 1
+    diag_pos = big.row_end[1:] - 1
+
+    def lowest(scale):
+        vals = big.val * scale
+        vals[diag_pos] = big.val[diag_pos] - eav
+        A = csr_matrix((vals, (rows, cols)), shape=(n, n))
+        A = A + A.T - csr_matrix((vals[diag_pos], (np.arange(n), np.arange(n))), shape=(n, n))
+        w, z = eigsh(A, k=3, which="SA", tol=0)
+        order = np.argsort(w)
+        return w[order], z[:, order]
+
+    w0, z0 = lowest(0.99)
+    w, z = lowest(1.0)
+    ev, vec, nl, nm, rc = O.maneig_mcdhf(big.row_end, big.col, big.val, eav, 3, [1, 3],
+                                         warm=(w0[[0, 2]], z0[:, [0, 2]]), nthreads=4)
+    assert rc == 0 and nl > 0
+    assert np.allclose(ev, w[[0, 2]], atol=1e-9)
+
+
 def test_setham_resums_lists_in_order():
     # two pairs, hand-made lists: diagonal (1,1) and (2,1), (2,2)
     nw = 2
