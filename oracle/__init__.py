@@ -233,6 +233,58 @@ def maneig(row_end, col, val, eav, nvec, nthreads: int = 1):
     return ev, evec.reshape(nvec, ncf).T, nl.value, nm.value, rc
 
 
+class Breit:
+    """rci90's transverse Breit radial integrals (breit_oracle.c): BESSEL's
+    cache, the GENINTBREIT1/2 tables and the BRINT3..6 memo, all as
+    state-dependent as the reference.  e: orbital energies E (orb_C),
+    c: speed of light, wfact: WFACT (photon frequency scale)."""
+
+    def __init__(self, g: "Grid", o: "Orbitals", e, c: float, wfact: float = 1.0):
+        L = lib()
+        L.orc_breit_new.restype = C.c_void_p
+        L.orc_brintf.restype = C.c_double
+        L.orc_genintbreit.restype = C.c_int64
+        self._keep = (g, o, np.ascontiguousarray(e, dtype=np.float64))
+        self.e = self._keep[2]
+        self.nw = o.nw
+        self.h = L.orc_breit_new(C.c_int(g.n), dp(g.r), dp(g.rp), dp(g.rpor), dp(g.ncc), dp(g.cnc5c),
+                                 C.c_double(g.accy), C.c_int(o.nw), C.c_int(o.ld), dp(o.P), dp(o.Q),
+                                 i32p(o.mf), i32p(o.nak), i32p(o.nkl), i32p(o.nkj), dp(self.e),
+                                 C.c_double(c), C.c_double(wfact))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_breit_free(C.c_void_p(self.h))
+            self.h = None
+
+    def brintf(self, itype, ia, ib, ic, id_, k) -> float:
+        return lib().orc_brintf(C.c_void_p(self.h), C.c_int(itype), C.c_int(ia), C.c_int(ib), C.c_int(ic),
+                                C.c_int(id_), C.c_int(k))
+
+    def genintbreit(self, itype):
+        """(kstart[0..j2max+1], keys, vals) of GENINTBREIT1 (1) or 2 (2)."""
+        L = lib()
+        j2 = C.c_int(0)
+        ks = np.zeros(24, dtype=np.int32)
+        # count sweep on a scratch copy would change no state: the count
+        # needs no BRINTF call, the fill sweep does
+        n = L.orc_genintbreit(C.c_void_p(self.h), C.c_int(itype), C.byref(j2), i32p(ks), None, None)
+        keys = np.zeros(max(n, 1), dtype=np.int32)
+        vals = np.zeros(max(n, 1))
+        # the call above filled the table (and advanced BESSEL's cache) once;
+        # read it back without recomputing
+        L.orc_breit_table.restype = C.c_int64
+        L.orc_breit_table(C.c_void_p(self.h), C.c_int(itype), i32p(keys), dp(vals))
+        return ks[: j2.value + 2].copy(), keys[:n], vals[:n]
+
+    def brint(self, itype, ia, ib, ic, id_, k) -> float:
+        v = C.c_double(0.0)
+        if lib().orc_brint(C.c_void_p(self.h), C.c_int(itype), C.c_int(ia), C.c_int(ib), C.c_int(ic),
+                           C.c_int(id_), C.c_int(k), C.byref(v)):
+            raise KeyError("Breit integral not found")
+        return v.value
+
+
 def maneig_mcdhf(row_end, col, val, eav, nvect, iccmin, warm=None, nthreads: int = 1):
     """rmcdhf90's MANEIG for levels iccmin (1-based).  warm: None (cold
     start) or (eval, evec) of the previous iteration for those levels.

@@ -17,10 +17,14 @@ mkdir -p "$OUT/obj"
 cd "$OUT/obj"
 FFLAGS="-O2 -fPIC"
 MODS="vast_kind_param_M parameter_def_M grid_C tatb_C wave_C orb_C def_C ncc_C cnc_C debug_C
-      lic13_C sbc_C npot_C npar_C memory_man cteilsrk_C kkstart_C jlabl_C hmat_C"
+      lic13_C sbc_C npot_C npar_C memory_man cteilsrk_C kkstart_C jlabl_C hmat_C
+      bess_C bilst_C kkstartbreit_C wfac_C stor_C cons_C"
 LIB9290="yzk_I quad_I dpbdt_I es_I slater_I dinit_I dmerge_I rinti_I
          radgrd setqic quad yzk slater rinti dpbdt nucpot es dinit dmerge spicmv2"
-RCI="triangrk genintrk rkintc"
+RCI="triangrk genintrk rkintc
+     zkf_I rkint_I skint_I bessel_I brra_I brintf_I triangbreit1_I triangbreit2_I
+     zkf rkint skint bessel brra brintf triangbreit1 triangbreit2 genintbreit1 genintbreit2
+     brint1 brint2 brint3 brint4 brint5 brint6"
 OBJS=""
 for m in $MODS; do $FC $FFLAGS -c "$REF/lib/libmod/$m.f90" -o "$m.o"; OBJS="$OBJS $m.o"; done
 for f in $LIB9290; do $FC $FFLAGS -c "$REF/lib/lib9290/$f.f90" -o "$f.o"; OBJS="$OBJS $f.o"; done

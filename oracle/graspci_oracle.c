@@ -491,20 +491,24 @@ int orc_rkintc(int nw, const int32_t *kstart, const int32_t *keys, const double 
  * LNMS/LVP/LSMS/LTRANS), CUTOFF = 1e-12 (setham_gg.f90:105).  The one-body
  * integral lookup follows IABINT (rci90/iabint.f90): labels are swapped so
  * that IA <= IB before the lookup. */
-int64_t orc_setham(int ncf, int nw, int64_t npairs,
-                   const int32_t *pair_ic, const int32_t *pair_ir,
-                   const int32_t *one_ia, const int32_t *one_ib,
-                   const int64_t *tsh_off, const double *tshell,
-                   const int64_t *tb_beg, const int64_t *tb_end,
-                   const int32_t *label, const double *coeff,
-                   const double *iab,
-                   const int32_t *kstart, const int32_t *keys, const double *vals,
-                   int lfordr, int iccut,
-                   int64_t *row_end, int32_t *col, double *val, double *eav_sum)
+int64_t orc_setham_tb(int ncf, int nw, int64_t npairs,
+                      const int32_t *pair_ic, const int32_t *pair_ir,
+                      const int32_t *one_ia, const int32_t *one_ib,
+                      const int64_t *tsh_off, const double *tshell,
+                      const int64_t *tb_beg, const int64_t *tb_end,
+                      const int32_t *label, const double *coeff,
+                      const double *iab,
+                      const int32_t *kstart, const int32_t *keys, const double *vals,
+                      const int64_t *bt_beg, const int64_t *bt_end,
+                      const int32_t *blabel, const double *bcoeff, orc_breit *br,
+                      int lfordr, int iccut,
+                      int64_t *row_end, int32_t *col, double *val, double *eav_sum,
+                      double *elsto)
 {
     const double cutoff = 1.0e-12;
     int64_t nnz = 0, p = 0;
     double eav = 0.0;
+    if (elsto) *elsto = 0.0;
     row_end[0] = 0;
     for (int ic = 1; ic <= ncf; ++ic) {
         int has_diag = 0;
@@ -539,6 +543,10 @@ int64_t orc_setham(int ncf, int nw, int64_t npairs,
                     elemnt = elemnt + tegral * v;
                 }
             }
+            /* transverse Breit (setham_gg.f90:298-357) after the
+             * Dirac-Coulomb terms */
+            if (br && orc_setham_breit_pair(br, bt_beg[p], bt_end[p], blabel, bcoeff, &elemnt, elsto))
+                return -1;
             if (ir == ic || fabs(elemnt) > cutoff) {
                 col[nnz] = ir;
                 val[nnz] = elemnt;
@@ -555,6 +563,22 @@ int64_t orc_setham(int ncf, int nw, int64_t npairs,
     if (p != npairs) return -1;
     *eav_sum = eav;
     return nnz;
+}
+
+int64_t orc_setham(int ncf, int nw, int64_t npairs,
+                   const int32_t *pair_ic, const int32_t *pair_ir,
+                   const int32_t *one_ia, const int32_t *one_ib,
+                   const int64_t *tsh_off, const double *tshell,
+                   const int64_t *tb_beg, const int64_t *tb_end,
+                   const int32_t *label, const double *coeff,
+                   const double *iab,
+                   const int32_t *kstart, const int32_t *keys, const double *vals,
+                   int lfordr, int iccut,
+                   int64_t *row_end, int32_t *col, double *val, double *eav_sum)
+{
+    return orc_setham_tb(ncf, nw, npairs, pair_ic, pair_ir, one_ia, one_ib, tsh_off, tshell, tb_beg, tb_end,
+                         label, coeff, iab, kstart, keys, vals, NULL, NULL, NULL, NULL, NULL, lfordr, iccut,
+                         row_end, col, val, eav_sum, NULL);
 }
 
 /* ------------------------------------------------------------ SPICMV2 */

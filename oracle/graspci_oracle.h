@@ -103,6 +103,55 @@ int64_t orc_setham(int ncf, int nw, int64_t npairs,
                    int lfordr, int iccut,
                    int64_t *row_end, int32_t *col, double *val, double *eav_sum);
 
+/* ------------------------------------------------- transverse Breit
+ * State of rci90's Breit radial integrals (breit_oracle.c): grid, orbitals,
+ * orbital energies E (orb_C), C and WFACT (photon frequencies w = WFACT
+ * |E(a) - E(b)| / C, bessel.f90:80), BESSEL's cache (KEEP, WIJ, BESSJ,
+ * BESSN), the GENINTBREIT1/2 tables and the BRINT3..6 memo.  All arrays are
+ * the caller's and must outlive the state.  Labels are 1-based. */
+typedef struct orc_breit orc_breit;
+orc_breit *orc_breit_new(int n, const double *r, const double *rp, const double *rpor, const double *ncc,
+                         const double *cnc5c, double accy, int nw, int ld, const double *P, const double *Q,
+                         const int *mf, const int *nak, const int *nkl, const int *nkj, const double *e,
+                         double c, double wfact);
+void orc_breit_free(orc_breit *s);
+/* BRINTF (rci90/brintf.f90:44-96): one integral of type 1..6, through
+ * BESSEL's cache (state-dependent as the reference is). */
+double orc_brintf(orc_breit *s, int itype, int ia, int ib, int ic, int id, int k);
+/* TRIANGBREIT1 (type 1) / TRIANGBREIT2 (type 2) on (l, 2j) of a, b, c, d. */
+int orc_triangbreit(int type, int la, int ja, int lb, int jb, int lc, int jc, int ld, int jd, int k);
+/* GENINTBREIT1 / GENINTBREIT2 (rci90/genintbreit1.f90:45-104): the table of
+ * type 1 or 2 (keys ((ia*KEY+ib)*KEY+ic)*KEY+id, KEY = NW+1, values,
+ * KSTARTBREIT(0..j2max+1)), kept in the state for BRINT1/2.  keys/vals may
+ * be NULL.  Returns the count, -1 if J2MAX > KMAX (20). */
+int64_t orc_genintbreit(orc_breit *s, int type, int *j2max, int *kstart, int32_t *keys, double *vals);
+/* The table of type 1 or 2 held in the state; returns its length. */
+int64_t orc_breit_table(const orc_breit *s, int type, int32_t *keys, double *vals);
+/* BRINT1..6 (rci90/brint1.f90, brint3.f90): table lookup (1, 2) or the
+ * memoised BRINTF (3..6).  Returns 0, -1 for a missing table integral. */
+int orc_brint(orc_breit *s, int type, int ia, int ib, int ic, int id, int k, double *out);
+/* SETHAM's Breit terms of one pair (setham_gg.f90:298-357): entries
+ * [beg, end) of LABEL (6 per entry: a, b, c, d, k, type; type < 0: core)
+ * and COEFF. */
+int orc_setham_breit_pair(orc_breit *s, int64_t beg, int64_t end, const int32_t *label, const double *coeff,
+                          double *elemnt, double *elsto);
+/* orc_setham with the transverse Breit terms of every pair ([bt_beg,
+ * bt_end) of blabel/bcoeff, summed after the Dirac-Coulomb terms) and the
+ * core sum ELSTO (*elsto).  br == NULL: Dirac-Coulomb only. */
+int64_t orc_setham_tb(int ncf, int nw, int64_t npairs,
+                      const int32_t *pair_ic, const int32_t *pair_ir,
+                      const int32_t *one_ia, const int32_t *one_ib,
+                      const int64_t *tsh_off, const double *tshell,
+                      const int64_t *tb_beg, const int64_t *tb_end,
+                      const int32_t *label, const double *coeff,
+                      const double *iab,
+                      const int32_t *kstart, const int32_t *keys, const double *vals,
+                      const int64_t *bt_beg, const int64_t *bt_end,
+                      const int32_t *blabel, const double *bcoeff, orc_breit *br,
+                      int lfordr, int iccut,
+                      int64_t *row_end, int32_t *col, double *val, double *eav_sum,
+                      double *elsto);
+
 /* SPICMV2 + DMERGE (lib9290/spicmv2.f90:2-60, dmerge.f90:3-43).
  * row_end[0..n] cumulative ends (IENDC with IENDC(0)=0), cols 1-based,
  * diagonal last in each row.  B, C column-major n x m. */

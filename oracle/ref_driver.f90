@@ -190,4 +190,120 @@ contains
       call spicmv2(n, m, b, c)
       deallocate(EMT, IROW, IENDC)
    end subroutine ref_spicmv2
+   ! transverse Breit state: orbital energies E (orb_C) and WFACT (wfac_C);
+   ! BESSEL's cache and the BRINT3..6 memo start empty
+   subroutine ref_breit_setup(nw_in, e_in, wfact_in) bind(C, name="ref_breit_setup")
+      use orb_C, only: E
+      use wfac_C, only: WFACT
+      use stor_C, only: KEEP
+      use bess_C, only: WIJ, BESSJ, BESSN
+      use bilst_C, only: FIRST, NTPI, INDTP1, INDTP2, INDTP3, INDTP4, INDTP5, INDTP6, &
+                         VALTP1, VALTP2, VALTP3, VALTP4, VALTP5, VALTP6
+      integer(c_int), value :: nw_in
+      real(c_double), intent(in) :: e_in(nw_in)
+      real(c_double), value :: wfact_in
+      E(1:nw_in) = e_in(1:nw_in)
+      WFACT = wfact_in
+      KEEP = 0
+      WIJ = 0.0d0
+      BESSJ = 0.0d0
+      BESSN = 0.0d0
+      FIRST = .true.
+      NTPI = 0
+      if (associated(INDTP1)) deallocate(INDTP1, VALTP1)
+      if (associated(INDTP2)) deallocate(INDTP2, VALTP2)
+      if (associated(INDTP3)) deallocate(INDTP3, VALTP3)
+      if (associated(INDTP4)) deallocate(INDTP4, VALTP4)
+      if (associated(INDTP5)) deallocate(INDTP5, VALTP5)
+      if (associated(INDTP6)) deallocate(INDTP6, VALTP6)
+      nullify(INDTP1, VALTP1, INDTP2, VALTP2, INDTP3, VALTP3, INDTP4, VALTP4, INDTP5, VALTP5, INDTP6, VALTP6)
+   end subroutine ref_breit_setup
+
+   ! GENINTBREIT1 (type 1) or GENINTBREIT2 (type 2): the table and its
+   ! KSTARTBREIT; the table stays in bilst_C for BRINT1/2
+   function ref_genintbreit(itype, kstart_out, keys_out, vals_out, cap) result(nout) &
+         bind(C, name="ref_genintbreit")
+      use bilst_C, only: INDTP1, VALTP1, INDTP2, VALTP2
+      use kkstartbreit_C, only: KSTARTBREIT1, KSTARTBREIT2
+      integer(c_int), value :: itype, cap
+      integer(c_int), intent(out) :: kstart_out(*)
+      integer(c_int), intent(out) :: keys_out(*)
+      real(c_double), intent(out) :: vals_out(*)
+      integer(c_int) :: nout
+      integer :: nn, j2max
+      interface
+         subroutine genintbreit1(myid, nprocs, nb, j2max)
+            integer, intent(in) :: myid, nprocs
+            integer :: nb, j2max
+         end subroutine
+         subroutine genintbreit2(myid, nprocs, nb, j2max)
+            integer, intent(in) :: myid, nprocs
+            integer :: nb, j2max
+         end subroutine
+      end interface
+      if (itype == 1) then
+         call genintbreit1(0, 1, nn, j2max)
+         kstart_out(1:j2max+2) = KSTARTBREIT1(0:j2max+1)
+         if (nn <= cap) then
+            keys_out(1:nn) = INDTP1(1:nn)
+            vals_out(1:nn) = VALTP1(1:nn)
+         end if
+      else
+         call genintbreit2(0, 1, nn, j2max)
+         kstart_out(1:j2max+2) = KSTARTBREIT2(0:j2max+1)
+         if (nn <= cap) then
+            keys_out(1:nn) = INDTP2(1:nn)
+            vals_out(1:nn) = VALTP2(1:nn)
+         end if
+      end if
+      nout = nn
+   end function ref_genintbreit
+
+   ! BRINT1..6 (rci90/brint1.f90 .. brint6.f90) as SETHAM calls them
+   function ref_brint(itype, ia, ib, ic, id, k) result(res) bind(C, name="ref_brint")
+      integer(c_int), value :: itype, ia, ib, ic, id, k
+      real(c_double) :: res
+      real(kind(0.0d0)) :: t
+      interface
+         subroutine brint1(ia, ib, ic, id, k, tegral)
+            integer, intent(in) :: ia, ib, ic, id, k
+            real(kind(0.0d0)), intent(out) :: tegral
+         end subroutine
+         subroutine brint2(ia, ib, ic, id, k, tegral)
+            integer, intent(in) :: ia, ib, ic, id, k
+            real(kind(0.0d0)), intent(out) :: tegral
+         end subroutine
+         subroutine brint3(ia, ib, ic, id, nu, tegral)
+            integer, intent(in) :: ia, ib, ic, id, nu
+            real(kind(0.0d0)), intent(out) :: tegral
+         end subroutine
+         subroutine brint4(ia, ib, ic, id, nu, tegral)
+            integer, intent(in) :: ia, ib, ic, id, nu
+            real(kind(0.0d0)), intent(out) :: tegral
+         end subroutine
+         subroutine brint5(ia, ib, ic, id, nu, tegral)
+            integer, intent(in) :: ia, ib, ic, id, nu
+            real(kind(0.0d0)), intent(out) :: tegral
+         end subroutine
+         subroutine brint6(ia, ib, ic, id, nu, tegral)
+            integer, intent(in) :: ia, ib, ic, id, nu
+            real(kind(0.0d0)), intent(out) :: tegral
+         end subroutine
+      end interface
+      select case (itype)
+      case (1)
+         call brint1(ia, ib, ic, id, k, t)
+      case (2)
+         call brint2(ia, ib, ic, id, k, t)
+      case (3)
+         call brint3(ia, ib, ic, id, k, t)
+      case (4)
+         call brint4(ia, ib, ic, id, k, t)
+      case (5)
+         call brint5(ia, ib, ic, id, k, t)
+      case default
+         call brint6(ia, ib, ic, id, k, t)
+      end select
+      res = t
+   end function ref_brint
 end module ref_driver
