@@ -47,6 +47,10 @@ class GciPairLists(C.Structure):
                 ("nterms", C.c_int64)]
 
 
+class GciBreitLists(C.Structure):
+    _fields_ = [("bt_beg", i64_p), ("bt_end", i64_p), ("label", i32_p), ("coeff", d_p), ("nterms", C.c_int64)]
+
+
 class GciSpmmShape(C.Structure):
     _fields_ = [("R", C.c_int), ("block", C.c_int), ("nslice", C.c_int), ("depth", C.c_int),
                 ("target", C.c_int64)]
@@ -76,6 +80,7 @@ EXPORTS = [
     "gci_host_alloc", "gci_host_free",
     "gci_spmm_device_noalloc",
     "gci_yzk", "gci_maneig_mcdhf",
+    "gci_breit_create", "gci_breit_table", "gci_breit_integral", "gci_breit_free", "gci_h_assemble_tb",
 ]
 GCI_NOALLOC_SCRATCH_BYTES = 48 << 20
 

@@ -260,6 +260,11 @@ size_t spmm_scratch_need(const HMat &h, int m, int mode);
 void spmm_noalloc(HMat &h, int m, const double *x, int ldx, double *y, int ldy, int mode, hipStream_t s,
                   char *scratch, size_t cap);
 
+// The value of every transverse Breit term of a block's pairs, in SETHAM's
+// order (breit.hip): NaN where |COEFF| <= 1e-12 or the pair is skipped.
+void breit_term_values(gci_breit *b, int64_t npairs, const int32_t *pic, const int32_t *pir, int lfordr, int iccut,
+                       const gci_breit_lists *bl, std::vector<double> &tv);
+
 }  // namespace gci
 
 struct gci_hmat : gci::HMat {};

@@ -43,6 +43,11 @@ struct AsmIn {
     const int32_t *keys;
     const double *vals;
     int j2max;
+    // transverse Breit terms (optional): per pair [bbb, bbe) of the 6-label
+    // entries, their COEFF and integral values (tv)
+    const int64_t *bbb, *bbe;
+    const int32_t *blab;
+    const double *bcof, *btv;
 };
 
 // RKINTC (rci90/rkintc.f90:45-103); returns false where the Fortran STOPs.
@@ -74,7 +79,7 @@ __device__ bool rkintc(const AsmIn &in, int ia, int ib, int ic, int id, int k, d
 }
 
 __global__ __launch_bounds__(256) void k_elements(AsmIn in, double *__restrict__ elem, int *__restrict__ keep_off,
-                                                  int *__restrict__ err) {
+                                                  int *__restrict__ err, double *__restrict__ core) {
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < in.npairs;
          p += (int64_t)gridDim.x * blockDim.x) {
         const int ic = in.pic[p], ir = in.pir[p];
@@ -111,6 +116,18 @@ __global__ __launch_bounds__(256) void k_elements(AsmIn in, double *__restrict__
                 }
                 e = e + tegral * v;
             }
+        }
+        if (in.bbb) {  // transverse Breit, setham_gg.f90:299-357
+            double els = 0.0;
+            for (int64_t q = in.bbb[p]; q < in.bbe[p]; ++q) {
+                const double v = in.bcof[q];
+                if (fabs(v) > CUTOFF) {
+                    const double contr = v * in.btv[q];
+                    if (in.blab[6 * q + 5] > 0) e = e + contr;
+                    else els = els + contr;
+                }
+            }
+            core[p] = els;
         }
         elem[p] = e;
         keep_off[p] = (ir != ic) && (fabs(e) > CUTOFF);  // setham_gg.f90:364
@@ -167,10 +184,13 @@ DBuf<T> to_dev(const T *h, size_t n) {
 
 using namespace gci;
 
-extern "C" int gci_h_assemble(int ncf, int nw, const gci_pair_lists *L, const double *iab, const int32_t *kstart,
-                              const int32_t *keys, const double *vals, int64_t nkeys, int lfordr, int iccut,
-                              int precision, gci_hmat **out, double *eav, int64_t *nelmnt) {
-    return guarded([&] {
+namespace {
+
+// gci_h_assemble, with the transverse Breit terms when bl is given
+void assemble(int ncf, int nw, const gci_pair_lists *L, const gci_breit_lists *bl, gci_breit *br,
+              const double *iab, const int32_t *kstart, const int32_t *keys, const double *vals, int64_t nkeys,
+              int lfordr, int iccut, int precision, gci_hmat **out, double *eav, double *elsto, int64_t *nelmnt) {
+    {
         GCI_REQUIRE(L && iab && kstart && out && eav, "gci_h_assemble: null argument");
         GCI_REQUIRE(ncf >= 1 && nw >= 1 && nw <= 127, "gci_h_assemble: bad ncf/nw");
         GCI_REQUIRE(precision == GCI_FP64 || precision == GCI_SPLIT32, "gci_h_assemble: bad precision");
@@ -212,6 +232,23 @@ extern "C" int gci_h_assemble(int ncf, int nw, const gci_pair_lists *L, const do
                             l[3] <= nw && l[4] >= 0 && l[4] <= 20,
                         "gci_h_assemble: two-body label out of range");
         }
+        std::vector<double> tv;
+        if (bl) {
+            GCI_REQUIRE(br && bl->bt_beg && bl->bt_end && (bl->nterms == 0 || (bl->label && bl->coeff)),
+                        "gci_h_assemble_tb: null Breit argument");
+            for (int64_t p = 0; p < np; ++p)
+                GCI_REQUIRE(bl->bt_beg[p] >= 0 && bl->bt_beg[p] <= bl->bt_end[p] && bl->bt_end[p] <= bl->nterms,
+                            "gci_h_assemble_tb: Breit range out of bounds");
+            for (int64_t q = 0; q < bl->nterms; ++q) {
+                const int32_t *l = bl->label + 6 * q;
+                const int t = l[5] < 0 ? -l[5] : l[5];
+                GCI_REQUIRE(l[0] >= 1 && l[0] <= nw && l[1] >= 1 && l[1] <= nw && l[2] >= 1 && l[2] <= nw &&
+                                l[3] >= 1 && l[3] <= nw && l[4] >= 0 && l[4] <= 20 && t >= 1 && t <= 6,
+                            "gci_h_assemble_tb: Breit label out of range");
+            }
+            // the integrals every kept term needs, in SETHAM's order
+            breit_term_values(br, np, L->pair_ic, L->pair_ir, lfordr, iccut, bl, tv);
+        }
         hipStream_t s = nullptr;
         auto dpic = to_dev(L->pair_ic, np), dpir = to_dev(L->pair_ir, np);
         auto doia = to_dev(L->one_ia, np), doib = to_dev(L->one_ib, np);
@@ -225,13 +262,30 @@ extern "C" int gci_h_assemble(int ncf, int nw, const gci_pair_lists *L, const do
         auto dkeys = to_dev(keys, (size_t)nkeys);
         auto dvals = to_dev(vals, (size_t)nkeys);
         AsmIn in{np,     nw,     lfordr,  iccut,  dpic.p,  dpir.p,  doia.p, doib.p, dtso.p, dtsh.p,
-                 dtbb.p, dtbe.p, dlab.p,  dcof.p, diab.p,  dkst.p,  dkeys.p, dvals.p, j2max};
+                 dtbb.p, dtbe.p, dlab.p,  dcof.p, diab.p,  dkst.p,  dkeys.p, dvals.p, j2max,
+                 nullptr, nullptr, nullptr, nullptr, nullptr};
+        DBuf<int64_t> dbbb, dbbe;
+        DBuf<int32_t> dblab;
+        DBuf<double> dbcof, dbtv, dcore;
+        if (bl) {
+            dbbb = to_dev(bl->bt_beg, np);
+            dbbe = to_dev(bl->bt_end, np);
+            dblab = to_dev(bl->label, (size_t)bl->nterms * 6);
+            dbcof = to_dev(bl->coeff, (size_t)bl->nterms);
+            dbtv = to_dev(tv.data(), tv.size());
+            dcore.alloc(np);
+            in.bbb = dbbb.p;
+            in.bbe = dbbe.p;
+            in.blab = dblab.p;
+            in.bcof = dbcof.p;
+            in.btv = dbtv.p;
+        }
         DBuf<double> elem(np);
         DBuf<int> keep(np), err(1);
         DBuf<int64_t> pos(np + 1);
         GCI_HIP(hipMemset(err.p, 0, sizeof(int)));
         const int64_t blocks = std::min<int64_t>((np + 255) / 256, 8192);
-        hipLaunchKernelGGL(k_elements, dim3((unsigned)blocks), dim3(256), 0, s, in, elem.p, keep.p, err.p);
+        hipLaunchKernelGGL(k_elements, dim3((unsigned)blocks), dim3(256), 0, s, in, elem.p, keep.p, err.p, dcore.p);
         GCI_HIP(hipGetLastError());
         // exclusive scan of the off-diagonal keep flags -> CSR positions
         size_t tmp_bytes = 0;
@@ -273,7 +327,17 @@ extern "C" int gci_h_assemble(int ncf, int nw, const gci_pair_lists *L, const do
             GCI_HIP(hipStreamSynchronize(s));
             double sum = 0.0;
             for (int i = 0; i < ncf; ++i) sum = sum + dg[i];
-            const double e = sum / (double)ncf;
+            // ELSTO: the core Breit terms summed in SETHAM's pair order
+            double els = 0.0;
+            if (bl) {
+                std::vector<double> cs(np);
+                GCI_HIP(hipMemcpy(cs.data(), dcore.p, np * sizeof(double), hipMemcpyDeviceToHost));
+                for (int64_t p = 0; p < np; ++p)
+                    if (!(lfordr && L->pair_ir[p] > iccut && L->pair_ir[p] != L->pair_ic[p])) els = els + cs[p];
+            }
+            if (elsto) *elsto = els;
+            // EAV = sum / NCF + ELSTO (genmat2.f90:73)
+            const double e = sum / (double)ncf + els;
             *eav = e;
             if (precision == GCI_SPLIT32) {
                 h->hi.alloc(total);
@@ -292,5 +356,27 @@ extern "C" int gci_h_assemble(int ncf, int nw, const gci_pair_lists *L, const do
             throw;
         }
         *out = h;
+    }
+}
+
+}  // namespace
+
+extern "C" int gci_h_assemble(int ncf, int nw, const gci_pair_lists *L, const double *iab, const int32_t *kstart,
+                              const int32_t *keys, const double *vals, int64_t nkeys, int lfordr, int iccut,
+                              int precision, gci_hmat **out, double *eav, int64_t *nelmnt) {
+    return guarded([&] {
+        assemble(ncf, nw, L, nullptr, nullptr, iab, kstart, keys, vals, nkeys, lfordr, iccut, precision, out, eav,
+                 nullptr, nelmnt);
+    });
+}
+
+extern "C" int gci_h_assemble_tb(int ncf, int nw, const gci_pair_lists *L, const gci_breit_lists *bl,
+                                 gci_breit *br, const double *iab, const int32_t *kstart, const int32_t *keys,
+                                 const double *vals, int64_t nkeys, int lfordr, int iccut, int precision,
+                                 gci_hmat **out, double *eav, double *elsto, int64_t *nelmnt) {
+    return guarded([&] {
+        GCI_REQUIRE(bl && br, "gci_h_assemble_tb: null Breit argument");
+        assemble(ncf, nw, L, bl, br, iab, kstart, keys, vals, nkeys, lfordr, iccut, precision, out, eav, elsto,
+                 nelmnt);
     });
 }

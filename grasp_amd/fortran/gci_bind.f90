@@ -36,6 +36,11 @@ module gci_bind
       integer(c_int64_t) :: nterms
    end type gci_pair_lists
 
+   type, bind(C) :: gci_breit_lists
+      type(c_ptr) :: bt_beg, bt_end, label, coeff
+      integer(c_int64_t) :: nterms
+   end type gci_breit_lists
+
    type, bind(C) :: gci_dvd_params
       integer(c_int) :: nvec, lim, mblock, maxiter
       real(c_double) :: crite, critc, critr
@@ -135,6 +140,61 @@ module gci_bind
          integer(c_int64_t) :: nelmnt
          integer(c_int) :: st
       end function gci_h_assemble
+
+      function gci_breit_create(grid, orb, e, c, wfact, out) bind(C, name="gci_breit_create") result(st)
+         import :: c_int, c_double, c_ptr, gci_grid, gci_orbitals
+         type(gci_grid), intent(in) :: grid
+         type(gci_orbitals), intent(in) :: orb
+         real(c_double), intent(in) :: e(*)
+         real(c_double), value :: c, wfact
+         type(c_ptr) :: out
+         integer(c_int) :: st
+      end function gci_breit_create
+
+      function gci_breit_table(b, itype, j2max, kstart, keys, vals, cap, n) &
+         bind(C, name="gci_breit_table") result(st)
+         import :: c_int, c_int32_t, c_int64_t, c_ptr
+         type(c_ptr), value :: b
+         integer(c_int), value :: itype
+         integer(c_int32_t) :: j2max
+         integer(c_int32_t) :: kstart(*)
+         type(c_ptr), value :: keys, vals
+         integer(c_int64_t), value :: cap
+         integer(c_int64_t) :: n
+         integer(c_int) :: st
+      end function gci_breit_table
+
+      function gci_breit_integral(b, itype, ia, ib, ic, id, k, out) bind(C, name="gci_breit_integral") result(st)
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: b
+         integer(c_int), value :: itype, ia, ib, ic, id, k
+         real(c_double) :: out
+         integer(c_int) :: st
+      end function gci_breit_integral
+
+      function gci_breit_free(b) bind(C, name="gci_breit_free") result(st)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: b
+         integer(c_int) :: st
+      end function gci_breit_free
+
+      function gci_h_assemble_tb(ncf, nw, lists, breit, br, iab, kstart, keys, vals, nkeys, lfordr, iccut, &
+                                 precision, out, eav, elsto, nelmnt) bind(C, name="gci_h_assemble_tb") result(st)
+         import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr, gci_pair_lists, gci_breit_lists
+         integer(c_int), value :: ncf, nw
+         type(gci_pair_lists), intent(in) :: lists
+         type(gci_breit_lists), intent(in) :: breit
+         type(c_ptr), value :: br
+         real(c_double) :: iab(*)
+         integer(c_int32_t) :: kstart(*), keys(*)
+         real(c_double) :: vals(*)
+         integer(c_int64_t), value :: nkeys
+         integer(c_int), value :: lfordr, iccut, precision
+         type(c_ptr) :: out
+         real(c_double) :: eav, elsto
+         integer(c_int64_t) :: nelmnt
+         integer(c_int) :: st
+      end function gci_h_assemble_tb
 
       function gci_hmat_upload(n, iendc, irow, emt, index_base, shift, precision, out) &
          bind(C, name="gci_hmat_upload") result(st)

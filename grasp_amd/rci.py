@@ -221,9 +221,59 @@ def one_body(r, rp, rpor, n, h, orbitals, zz, c):
     return iab
 
 
-def setham(ncf, nw, lists, iab, kstart, keys, vals, lfordr=False, iccut=0, precision=L.GCI_FP64):
+class BreitTables:
+    """rci90's transverse Breit radial integrals on the GPU
+    (gci_breit_create: GENINTBREIT1/2, then BRINT3..6 on demand), with the
+    reference's module state carried from block to block.  e: orbital
+    energies E, c: speed of light, wfact: WFACT."""
+
+    def __init__(self, r, rp, rpor, n, h, orbitals, e, c, wfact=1.0):
+        self._g = _grid_struct(r, rp, rpor, n, h)
+        self._o = _orb_struct(orbitals.P, orbitals.Q, orbitals.mf, orbitals.nak, orbitals.nkl, orbitals.nkj)
+        self._e = np.ascontiguousarray(e, dtype=np.float64)
+        self.h = C.c_void_p()
+        check(L.lib().gci_breit_create(C.byref(self._g[0]), C.byref(self._o[0]), dp(self._e), C.c_double(c),
+                                       C.c_double(wfact), C.byref(self.h)))
+
+    def table(self, itype):
+        """(kstart[0..j2max+1], keys, vals) of GENINTBREIT1 (1) or 2 (2)."""
+        j2 = C.c_int32(0)
+        cnt = C.c_int64(0)
+        kst = np.zeros(22, dtype=np.int32)
+        check(L.lib().gci_breit_table(self.h, C.c_int(itype), C.byref(j2), i32p(kst), None, None, C.c_int64(0),
+                                      C.byref(cnt)))
+        nk = cnt.value
+        keys = np.zeros(max(nk, 1), dtype=np.int32)
+        vals = np.zeros(max(nk, 1))
+        check(L.lib().gci_breit_table(self.h, C.c_int(itype), C.byref(j2), i32p(kst), i32p(keys), dp(vals),
+                                      C.c_int64(nk), C.byref(cnt)))
+        return kst[: j2.value + 2].copy(), keys[:nk], vals[:nk]
+
+    def integral(self, itype, ia, ib, ic, id_, k) -> float:
+        """BRINTn(ia, ib, ic, id, k) as SETHAM calls it."""
+        v = C.c_double(0.0)
+        check(L.lib().gci_breit_integral(self.h, C.c_int(itype), C.c_int(ia), C.c_int(ib), C.c_int(ic),
+                                         C.c_int(id_), C.c_int(k), C.byref(v)))
+        return v.value
+
+    def free(self):
+        if self.h:
+            L.lib().gci_breit_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def setham(ncf, nw, lists, iab, kstart, keys, vals, lfordr=False, iccut=0, precision=L.GCI_FP64, breit=None,
+           tables=None):
     """SETHAM element loop on the GPU: returns (HamiltonianBlock with the
-    diagonal shifted by -EAV, EAV, NELMNT)."""
+    diagonal shifted by -EAV, EAV, NELMNT).  With breit (the lists' bt_beg,
+    bt_end, blabel, bcoeff) and tables (BreitTables), the transverse Breit
+    terms too; EAV then includes ELSTO, which is returned fourth."""
     arrs = {
         "pair_ic": np.ascontiguousarray(lists["pair_ic"], dtype=np.int32),
         "pair_ir": np.ascontiguousarray(lists["pair_ir"], dtype=np.int32),
@@ -247,12 +297,28 @@ def setham(ncf, nw, lists, iab, kstart, keys, vals, lfordr=False, iccut=0, preci
     h = C.c_void_p()
     eav = C.c_double(0.0)
     nel = C.c_int64(0)
-    check(L.lib().gci_h_assemble(C.c_int(ncf), C.c_int(nw), C.byref(pl), dp(iab), i32p(kstart), i32p(keys),
-                                 dp(vals), C.c_int64(len(keys)), C.c_int(int(lfordr)), C.c_int(iccut),
-                                 C.c_int(precision), C.byref(h), C.byref(eav), C.byref(nel)))
+    if breit is None:
+        check(L.lib().gci_h_assemble(C.c_int(ncf), C.c_int(nw), C.byref(pl), dp(iab), i32p(kstart), i32p(keys),
+                                     dp(vals), C.c_int64(len(keys)), C.c_int(int(lfordr)), C.c_int(iccut),
+                                     C.c_int(precision), C.byref(h), C.byref(eav), C.byref(nel)))
+    else:
+        b = {"bt_beg": np.ascontiguousarray(breit["bt_beg"], dtype=np.int64),
+             "bt_end": np.ascontiguousarray(breit["bt_end"], dtype=np.int64),
+             "blabel": np.ascontiguousarray(breit["blabel"], dtype=np.int32),
+             "bcoeff": np.ascontiguousarray(breit["bcoeff"], dtype=np.float64)}
+        bl = L.GciBreitLists(i64p(b["bt_beg"]), i64p(b["bt_end"]), i32p(b["blabel"]), dp(b["bcoeff"]),
+                             len(b["bcoeff"]))
+        elsto = C.c_double(0.0)
+        check(L.lib().gci_h_assemble_tb(C.c_int(ncf), C.c_int(nw), C.byref(pl), C.byref(bl), tables.h, dp(iab),
+                                        i32p(kstart), i32p(keys), dp(vals), C.c_int64(len(keys)),
+                                        C.c_int(int(lfordr)), C.c_int(iccut), C.c_int(precision), C.byref(h),
+                                        C.byref(eav), C.byref(elsto), C.byref(nel)))
     nn, nz, pr = C.c_int64(), C.c_int64(), C.c_int()
     check(L.lib().gci_hmat_info(h, C.byref(nn), C.byref(nz), C.byref(pr)))
-    return HamiltonianBlock(h, nn.value, nz.value, pr.value), eav.value, nel.value
+    blk = HamiltonianBlock(h, nn.value, nz.value, pr.value)
+    if breit is None:
+        return blk, eav.value, nel.value
+    return blk, eav.value, nel.value, elsto.value
 
 
 class _PinnedBlock:

@@ -132,6 +132,54 @@ int gci_h_assemble(int ncf, int nw, const gci_pair_lists *lists, const double *i
                    int64_t nkeys, int lfordr, int iccut, int precision, gci_hmat **out,
                    double *eav, int64_t *nelmnt);
 
+/* --------------------------------------------- transverse Breit
+ * Replaces rci90's Breit radial integrals: GENINTBREIT1/2
+ * (rci90/genintbreit1.f90:3-108, genintbreit2.f90; called at rci92.f90:198-
+ * 202) and the memoised BRINT3..6 (rci90/brint3.f90..brint6.f90) with their
+ * BRINTF/BESSEL/BRRA/RKINT/SKINT/ZKF chain.  The handle holds what the
+ * reference keeps in module state across blocks: both tables, BESSEL's four
+ * cached slots and frequencies (stor_C, bess_C; replayed in the reference's
+ * call order, so every integral reads the slots the reference's would), and
+ * the BRINT3..6 values.  e: orbital energies E (orb_C); c: speed of light;
+ * wfact: WFACT (getcid.f90:154-164, photon frequency w = WFACT |E(a) -
+ * E(b)| / c).  GENINTBREIT2's k = 0 entries go through ZKF(K-1 = -1), where
+ * the reference reads an unset power table (zkf.f90:46-48); here
+ * R**K = R**(-1) is used, and the angular code never asks for them. */
+typedef struct gci_breit gci_breit;
+int gci_breit_create(const gci_grid *grid, const gci_orbitals *orb, const double *e, double c, double wfact,
+                     gci_breit **out);
+/* The table of type 1 or 2 (keys ((ia*KEY+ib)*KEY+ic)*KEY+id, KEY = NW+1,
+ * values, KSTARTBREIT(0..j2max+1)); keys = vals = NULL: count only. */
+int gci_breit_table(const gci_breit *b, int type, int32_t *j2max, int32_t *kstart, int32_t *keys, double *vals,
+                    int64_t cap, int64_t *n);
+/* BRINTn(ia, ib, ic, id, k) as SETHAM calls it (1-based labels). */
+int gci_breit_integral(gci_breit *b, int type, int ia, int ib, int ic, int id, int k, double *out);
+int gci_breit_free(gci_breit *b);
+
+/* SETHAM's transverse Breit lists (RKCO_GG with BREID, setham_gg.f90:298-
+ * 302) per CSF pair of a gci_pair_lists: entries [bt_beg, bt_end) of LABEL
+ * (6 per entry: a, b, c, d, k and the integral type 1..6, negative for a
+ * core term) and COEFF. */
+typedef struct {
+    const int64_t *bt_beg;
+    const int64_t *bt_end;
+    const int32_t *label;
+    const double *coeff;
+    int64_t nterms;
+} gci_breit_lists;
+
+/* gci_h_assemble with the transverse Breit terms (setham_gg.f90:298-357):
+ * after a pair's Dirac-Coulomb terms, COEFF * BRINTn over its Breit entries
+ * with |COEFF| > 1e-12, in list order; core terms go to ELSTO instead of
+ * the element.  EAV = sum of diagonals / NCF + ELSTO (genmat2.f90:73), and
+ * the diagonal is shifted by -EAV (maneig.f90:326).  br carries the tables
+ * and the state from block to block, as the reference's modules do: assemble
+ * a run's blocks in the reference's block order. */
+int gci_h_assemble_tb(int ncf, int nw, const gci_pair_lists *lists, const gci_breit_lists *breit,
+                      gci_breit *br, const double *iab, const int32_t *kstart, const int32_t *keys,
+                      const double *vals, int64_t nkeys, int lfordr, int iccut, int precision,
+                      gci_hmat **out, double *eav, double *elsto, int64_t *nelmnt);
+
 /* --------------------------------------------- block Hamiltonian handle
  * .res-row ingestion (rci90/setham_gg.f90:378 layout): lower-triangle rows,
  * diagonal last, cumulative row ends iendc[0..n] (iendc[0] = 0), column

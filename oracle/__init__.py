@@ -188,6 +188,33 @@ def setham(ncf, nw, lists, iab, kstart, keys, vals, lfordr=False, iccut=0):
     return row_end, col[:nnz].copy(), val[:nnz].copy(), eav.value
 
 
+def setham_tb(ncf, nw, lists, breit, br: "Breit", iab, kstart, keys, vals, lfordr=False, iccut=0):
+    """SETHAM with the transverse Breit terms (breit: bt_beg, bt_end, blabel,
+    bcoeff; br: the Breit state, advanced as the reference's modules are).
+    Returns (row_end, col, val, eav_sum, elsto)."""
+    L = lib()
+    npairs = len(lists["pair_ic"])
+    cap = npairs + 1
+    row_end = np.zeros(ncf + 1, dtype=np.int64)
+    col = np.zeros(cap, dtype=np.int32)
+    val = np.zeros(cap)
+    eav = C.c_double(0.0)
+    elsto = C.c_double(0.0)
+    b = {k: np.ascontiguousarray(breit[k]) for k in ("bt_beg", "bt_end", "blabel", "bcoeff")}
+    L.orc_setham_tb.restype = C.c_int64
+    nnz = L.orc_setham_tb(C.c_int(ncf), C.c_int(nw), C.c_int64(npairs),
+                          i32p(lists["pair_ic"]), i32p(lists["pair_ir"]), i32p(lists["one_ia"]),
+                          i32p(lists["one_ib"]), i64p(lists["tsh_off"]), dp(lists["tshell"]),
+                          i64p(lists["tb_beg"]), i64p(lists["tb_end"]), i32p(lists["label"]),
+                          dp(lists["coeff"]), dp(np.ascontiguousarray(iab, dtype=np.float64)),
+                          i32p(kstart), i32p(keys), dp(vals), i64p(b["bt_beg"]), i64p(b["bt_end"]),
+                          i32p(b["blabel"]), dp(b["bcoeff"]), C.c_void_p(br.h), C.c_int(int(lfordr)),
+                          C.c_int(iccut), i64p(row_end), i32p(col), dp(val), C.byref(eav), C.byref(elsto))
+    if nnz < 0:
+        raise ValueError("setham_tb: malformed list or missing integral")
+    return row_end, col[:nnz].copy(), val[:nnz].copy(), eav.value, elsto.value
+
+
 def spicmv2(row_end, col, val, B, nthreads: int = 1):
     B = np.asfortranarray(B, dtype=np.float64)
     if B.ndim == 1:

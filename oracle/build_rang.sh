@@ -30,7 +30,7 @@ pending=()
 for d in lib/libmod lib/lib9290 lib/librang90; do
   for f in "$REF/$d"/*.f90; do pending+=("$f"); done
 done
-for f in talk_I cxk_I talk cxk; do pending+=("$REF/appl/rci90/$f.f90"); done
+for f in talk_I cxk_I talk cxk breid_I breid; do pending+=("$REF/appl/rci90/$f.f90"); done
 export FC
 compile_one() {
   f="$1"

@@ -10,15 +10,19 @@
 ! the two calls of the SETHAM element loop (rci90/setham_gg.f90:213-292):
 ! ONESCALAR(IC, IR, IA, IB, TSHELL) and RKCO_GG(IC, IR, CORD, INCOR=1, 1),
 ! whose TALK/SPEAK callbacks fill LABEL/COEFF (lib9290/speak.f90:46-59).
+! With a fourth argument 1 it also makes SETHAM's transverse Breit call
+! RKCO_GG(IC, IR, BREID, 1, 2) (setham_gg.f90:298-302), after setting
+! ICORE for the block as SETHAM does (setham_gg.f90:145-156).
 !
-! Usage: rang_lists <file.c> <block> <out.bin>
+! Usage: rang_lists <file.c> <block> <out.bin> [breit]
 ! Output (stream, native byte order):
-!   int32 NW, NCF, NCORE
+!   int32 NW, NCF, NCORE, BREIT
 !   NW x int32 (NP, NAK, NKL, NKJ)
 !   per pair (IC ascending, IR = 1..IC):
 !     int32 IC, IR, IA, IB, NT   (NT = NW if IA == IB /= 0, 1 if IA /= IB, 0 if IA == 0)
 !     NT x real64 TSHELL(1:NT)
 !     int32 NVCOEF, then NVCOEF x (5 x int32 LABEL(1:5, i), real64 COEFF(i))
+!     if BREIT: int32 NVCOEF, then NVCOEF x (6 x int32 LABEL(1:6, i), real64 COEFF(i))
 program rang_lists
    use vast_kind_param, only: DOUBLE
    use parameter_def, only: NNNW
@@ -37,11 +41,14 @@ program rang_lists
    use onescalar_I
    use rkco_gg_I
    use cord_I
+   use breid_I
+   use bcore_C, only: ICORE
+   use ichop_I
    implicit none
    integer, parameter :: NBLKIN = 100
    character(len=512) :: cfile, ofile, arg
    character(len=8) :: idblk(NBLKIN)
-   integer :: jb, ncore, ncftot, ic, ir, ia, ib, i, nt, u, iblk
+   integer :: jb, ncore, ncftot, ic, ir, ia, ib, i, nt, u, iblk, ibreit, j
    integer :: ncf_of(NBLKIN)
    real(DOUBLE), allocatable :: tshell(:)
 
@@ -49,6 +56,11 @@ program rang_lists
    call get_command_argument(2, arg)
    read (arg, *) jb
    call get_command_argument(3, ofile)
+   ibreit = 0
+   if (command_argument_count() >= 4) then
+      call get_command_argument(4, arg)
+      read (arg, *) ibreit
+   end if
    IBUG1 = 0; IBUG2 = 0; IBUG3 = 0; IBUG4 = 0; IBUG5 = 0; IBUG6 = 0
 
    ! SETMC (machine constants from LAPACK's DLAMCH, absent from the image)
@@ -79,9 +91,20 @@ program rang_lists
    end do
    call ALCBUF (1)
    allocate (tshell(NW))
+   if (ibreit /= 0) then
+      ! setham_gg.f90:145-156
+      do i = 1, NW
+         ICORE(i) = 0
+         do j = 1, NCF
+            if (ICHOP(i, j) <= 0) goto 2
+         end do
+         ICORE(i) = 1
+2        continue
+      end do
+   end if
 
    open (newunit=u, file=ofile, access='stream', form='unformatted', status='replace')
-   write (u) NW, NCF, ncore
+   write (u) NW, NCF, ncore, ibreit
    write (u) (NP(i), NAK(i), NKL(i), NKJ(i), i = 1, NW)
    do ic = 1, NCF
       do ir = 1, ic
@@ -103,6 +126,14 @@ program rang_lists
          do i = 1, NVCOEF
             write (u) LABEL(1:5, i), COEFF(i)
          end do
+         if (ibreit /= 0) then
+            NVCOEF = 0
+            call RKCO_GG (ic, ir, BREID, 1, 2)
+            write (u) NVCOEF
+            do i = 1, NVCOEF
+               write (u) LABEL(1:6, i), COEFF(i)
+            end do
+         end if
       end do
    end do
    close (u)
