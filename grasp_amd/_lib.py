@@ -81,6 +81,7 @@ EXPORTS = [
     "gci_spmm_device_noalloc",
     "gci_yzk", "gci_maneig_mcdhf",
     "gci_breit_create", "gci_breit_table", "gci_breit_integral", "gci_breit_free", "gci_h_assemble_tb",
+    "gci_hstream_begin", "gci_hstream_push", "gci_hstream_end", "gci_hstream_free",
 ]
 GCI_NOALLOC_SCRATCH_BYTES = 48 << 20
 

@@ -14,6 +14,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
+#include <cstring>
+#include <memory>
 
 #include "gci_internal.h"
 
@@ -48,6 +50,7 @@ struct AsmIn {
     const int64_t *bbb, *bbe;
     const int32_t *blab;
     const double *bcof, *btv;
+    int bstride;  // element stride of bcof and btv (2 when interleaved)
 };
 
 // RKINTC (rci90/rkintc.f90:45-103); returns false where the Fortran STOPs.
@@ -120,9 +123,9 @@ __global__ __launch_bounds__(256) void k_elements(AsmIn in, double *__restrict__
         if (in.bbb) {  // transverse Breit, setham_gg.f90:299-357
             double els = 0.0;
             for (int64_t q = in.bbb[p]; q < in.bbe[p]; ++q) {
-                const double v = in.bcof[q];
+                const double v = in.bcof[in.bstride * q];
                 if (fabs(v) > CUTOFF) {
-                    const double contr = v * in.btv[q];
+                    const double contr = v * in.btv[in.bstride * q];
                     if (in.blab[6 * q + 5] > 0) e = e + contr;
                     else els = els + contr;
                 }
@@ -263,7 +266,7 @@ void assemble(int ncf, int nw, const gci_pair_lists *L, const gci_breit_lists *b
         auto dvals = to_dev(vals, (size_t)nkeys);
         AsmIn in{np,     nw,     lfordr,  iccut,  dpic.p,  dpir.p,  doia.p, doib.p, dtso.p, dtsh.p,
                  dtbb.p, dtbe.p, dlab.p,  dcof.p, diab.p,  dkst.p,  dkeys.p, dvals.p, j2max,
-                 nullptr, nullptr, nullptr, nullptr, nullptr};
+                 nullptr, nullptr, nullptr, nullptr, nullptr, 1};
         DBuf<int64_t> dbbb, dbbe;
         DBuf<int32_t> dblab;
         DBuf<double> dbcof, dbtv, dcore;
@@ -380,3 +383,327 @@ extern "C" int gci_h_assemble_tb(int ncf, int nw, const gci_pair_lists *L, const
                  nelmnt);
     });
 }
+
+// ------------------------------------------------ streamed assembly
+//
+// The angular producer runs on the CPU row by row (setham_gg.f90:162-395).
+// gci_hstream_* let it hand its lists over in chunks of pairs while it
+// produces the next ones: each push copies the chunk into one of two
+// page-locked staging slots, queues its upload and the element kernel on the
+// stream, and returns; the slot is reused only after the kernel that read it
+// has finished (an event).  Element values and keep flags accumulate on the
+// device; the end compacts them into the block exactly as gci_h_assemble
+// does, so the block is identical for any chunking.
+
+namespace {
+
+struct Stage {
+    size_t cap = 0;
+    char *host = nullptr;
+    DBuf<char> dev;
+    hipEvent_t done = nullptr;
+    ~Stage() {
+        if (done) {
+            (void)hipEventSynchronize(done);
+            (void)hipEventDestroy(done);
+        }
+        if (host) (void)hipHostFree(host);
+    }
+    void ensure(size_t bytes) {
+        if (!done) GCI_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        GCI_HIP(hipEventSynchronize(done));  // the last kernel that read this slot
+        if (bytes <= cap) return;
+        if (host) GCI_HIP(hipHostFree(host));
+        host = nullptr;
+        const size_t c = std::max<size_t>(bytes, 2 * cap);
+        GCI_HIP(hipHostMalloc(reinterpret_cast<void **>(&host), c, hipHostMallocDefault));
+        dev.alloc(c);
+        cap = c;
+    }
+};
+
+// byte layout of one chunk in a staging slot (each array 256-byte aligned)
+struct ChunkLayout {
+    size_t off[14];
+    size_t total;
+};
+
+template <class T>
+std::unique_ptr<SBuf<T>> grow(std::unique_ptr<SBuf<T>> old, size_t used, size_t want, hipStream_t s) {
+    auto nb = std::make_unique<SBuf<T>>(want, s);
+    if (old && used) GCI_HIP(hipMemcpyAsync(nb->p, old->p, used * sizeof(T), hipMemcpyDeviceToDevice, s));
+    return nb;  // the old buffer is released in stream order
+}
+
+}  // namespace
+
+struct gci_hstream {
+    int ncf = 0, nw = 0, lfordr = 0, iccut = 0, precision = 0, j2max = 0;
+    gci_breit *br = nullptr;
+    DBuf<double> iab, vals;
+    DBuf<int32_t> kst, keys;
+    hipStream_t s = nullptr;
+    Stage st[2];
+    int cur = 0;
+    std::unique_ptr<SBuf<double>> elem, core;
+    std::unique_ptr<SBuf<int>> keep;
+    std::unique_ptr<SBuf<int32_t>> pic, pir;
+    int64_t np = 0, cap = 0;
+    std::vector<int64_t> row_first;
+    std::vector<int32_t> pir_h, pic_h;  // for LFORDR's ELSTO rule at the end
+    int prev_ic = 0, prev_ir = 0;
+    DBuf<int> err;
+    bool with_breit = false;
+    ~gci_hstream() {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            elem.reset();
+            core.reset();
+            keep.reset();
+            pic.reset();
+            pir.reset();
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
+};
+
+extern "C" {
+
+int gci_hstream_begin(int ncf, int nw, const double *iab, const int32_t *kstart, const int32_t *keys,
+                      const double *vals, int64_t nkeys, int lfordr, int iccut, int precision, gci_breit *br,
+                      gci_hstream **out) {
+    return guarded([&] {
+        GCI_REQUIRE(iab && kstart && out, "gci_hstream_begin: null argument");
+        GCI_REQUIRE(ncf >= 1 && nw >= 1 && nw <= 127, "gci_hstream_begin: bad ncf/nw");
+        GCI_REQUIRE(precision == GCI_FP64 || precision == GCI_SPLIT32, "gci_hstream_begin: bad precision");
+        auto h = std::make_unique<gci_hstream>();
+        h->ncf = ncf;
+        h->nw = nw;
+        h->lfordr = lfordr;
+        h->iccut = iccut;
+        h->precision = precision;
+        h->br = br;
+        h->with_breit = br != nullptr;
+        int j2max = 0;
+        while (j2max < 21 && kstart[j2max + 1] <= nkeys + 1 && kstart[j2max + 1] >= kstart[j2max]) {
+            if (kstart[j2max + 1] == nkeys + 1) break;
+            ++j2max;
+        }
+        h->j2max = j2max;
+        h->iab = to_dev(iab, (size_t)nw * nw);
+        h->kst = to_dev(kstart, (size_t)j2max + 2);
+        h->keys = to_dev(keys, (size_t)nkeys);
+        h->vals = to_dev(vals, (size_t)nkeys);
+        h->row_first.assign(ncf, -1);
+        h->err.alloc(1);
+        GCI_HIP(hipMemset(h->err.p, 0, sizeof(int)));
+        GCI_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+        *out = h.release();
+    });
+}
+
+int gci_hstream_push(gci_hstream *h, const gci_pair_lists *L, const gci_breit_lists *bl) {
+    return guarded([&] {
+        GCI_REQUIRE(h && L, "gci_hstream_push: null argument");
+        GCI_REQUIRE((bl != nullptr) == h->with_breit, "gci_hstream_push: Breit lists given iff the stream has tables");
+        const int64_t cn = L->npairs;
+        if (cn == 0) return;
+        const int nw = h->nw, ncf = h->ncf;
+        // the same validation as gci_h_assemble, continued across chunks
+        for (int64_t p = 0; p < cn; ++p) {
+            const int ic = L->pair_ic[p], ir = L->pair_ir[p];
+            GCI_REQUIRE(ic >= 1 && ic <= ncf && ir >= 1 && ir <= ic, "gci_hstream_push: pair out of the lower triangle");
+            if (ic != h->prev_ic) {
+                GCI_REQUIRE(ic == h->prev_ic + 1 && (h->prev_ic == 0 || h->prev_ir == h->prev_ic),
+                            "gci_hstream_push: rows must be consecutive and end with their diagonal");
+                h->row_first[ic - 1] = h->np + p;
+            } else {
+                GCI_REQUIRE(ir > h->prev_ir, "gci_hstream_push: columns not ascending");
+            }
+            h->prev_ic = ic;
+            h->prev_ir = ir;
+            const int ia = L->one_ia[p], ib = L->one_ib[p];
+            GCI_REQUIRE(ia >= 0 && ia <= nw && ib >= 0 && ib <= nw, "gci_hstream_push: one-body label out of range");
+            const int64_t cnt = ia == 0 ? 0 : (ia == ib ? nw : 1);
+            GCI_REQUIRE(cnt == 0 || (L->tsh_off[p] >= 0 && L->tsh_off[p] + cnt <= L->ntshell),
+                        "gci_hstream_push: TSHELL offset out of range");
+            GCI_REQUIRE(L->tb_beg[p] >= 0 && L->tb_beg[p] <= L->tb_end[p] && L->tb_end[p] <= L->nterms,
+                        "gci_hstream_push: two-body range out of bounds");
+            if (bl)
+                GCI_REQUIRE(bl->bt_beg[p] >= 0 && bl->bt_beg[p] <= bl->bt_end[p] && bl->bt_end[p] <= bl->nterms,
+                            "gci_hstream_push: Breit range out of bounds");
+        }
+        for (int64_t q = 0; q < L->nterms; ++q) {
+            const int32_t *l = L->label + 5 * q;
+            GCI_REQUIRE(l[0] >= 1 && l[0] <= nw && l[1] >= 1 && l[1] <= nw && l[2] >= 1 && l[2] <= nw && l[3] >= 1 &&
+                            l[3] <= nw && l[4] >= 0 && l[4] <= 20,
+                        "gci_hstream_push: two-body label out of range");
+        }
+        std::vector<double> tv;
+        if (bl) {
+            for (int64_t q = 0; q < bl->nterms; ++q) {
+                const int32_t *l = bl->label + 6 * q;
+                const int t = l[5] < 0 ? -l[5] : l[5];
+                GCI_REQUIRE(l[0] >= 1 && l[0] <= nw && l[1] >= 1 && l[1] <= nw && l[2] >= 1 && l[2] <= nw &&
+                                l[3] >= 1 && l[3] <= nw && l[4] >= 0 && l[4] <= 20 && t >= 1 && t <= 6,
+                            "gci_hstream_push: Breit label out of range");
+            }
+            breit_term_values(h->br, cn, L->pair_ic, L->pair_ir, h->lfordr, h->iccut, bl, tv);
+        }
+        h->pic_h.insert(h->pic_h.end(), L->pair_ic, L->pair_ic + cn);
+        h->pir_h.insert(h->pir_h.end(), L->pair_ir, L->pair_ir + cn);
+        // result arrays grow in stream order
+        if (h->np + cn > h->cap) {
+            const int64_t c = std::max<int64_t>(2 * h->cap, h->np + cn);
+            h->elem = grow(std::move(h->elem), h->np, c, h->s);
+            h->keep = grow(std::move(h->keep), h->np, c, h->s);
+            h->pic = grow(std::move(h->pic), h->np, c, h->s);
+            h->pir = grow(std::move(h->pir), h->np, c, h->s);
+            if (bl) h->core = grow(std::move(h->core), h->np, c, h->s);
+            h->cap = c;
+        }
+        // the chunk into a staging slot: pinned copy, one upload
+        const int64_t nb = bl ? bl->nterms : 0;
+        const size_t sizes[14] = {cn * 4ul, cn * 4ul, cn * 4ul, cn * 4ul, cn * 8ul, (size_t)L->ntshell * 8,
+                                  cn * 8ul, cn * 8ul, (size_t)L->nterms * 20, (size_t)L->nterms * 8,
+                                  bl ? cn * 8ul : 0, bl ? cn * 8ul : 0, (size_t)nb * 24, (size_t)nb * 16};
+        const void *srcs[14] = {L->pair_ic, L->pair_ir, L->one_ia, L->one_ib, L->tsh_off, L->tshell,
+                                L->tb_beg, L->tb_end, L->label, L->coeff, bl ? bl->bt_beg : nullptr,
+                                bl ? bl->bt_end : nullptr, bl ? bl->label : nullptr, nullptr};
+        ChunkLayout lay{};
+        size_t total = 0;
+        for (int i = 0; i < 14; ++i) {
+            lay.off[i] = total;
+            total += (sizes[i] + 255) & ~(size_t)255;
+        }
+        lay.total = total;
+        Stage &sg = h->st[h->cur];
+        sg.ensure(total);
+        for (int i = 0; i < 13; ++i)
+            if (sizes[i]) std::memcpy(sg.host + lay.off[i], srcs[i], sizes[i]);
+        if (bl && nb) {
+            // coefficients and their integral values side by side
+            double *cv = reinterpret_cast<double *>(sg.host + lay.off[13]);
+            for (int64_t q = 0; q < nb; ++q) {
+                cv[2 * q] = bl->coeff[q];
+                cv[2 * q + 1] = tv[q];
+            }
+        }
+        GCI_HIP(hipMemcpyAsync(sg.dev.p, sg.host, total, hipMemcpyHostToDevice, h->s));
+        auto D = [&](int i) { return sg.dev.p + lay.off[i]; };
+        GCI_HIP(hipMemcpyAsync(h->pic->p + h->np, D(0), cn * 4, hipMemcpyDeviceToDevice, h->s));
+        GCI_HIP(hipMemcpyAsync(h->pir->p + h->np, D(1), cn * 4, hipMemcpyDeviceToDevice, h->s));
+        AsmIn in{cn,
+                 nw,
+                 h->lfordr,
+                 h->iccut,
+                 reinterpret_cast<const int32_t *>(D(0)),
+                 reinterpret_cast<const int32_t *>(D(1)),
+                 reinterpret_cast<const int32_t *>(D(2)),
+                 reinterpret_cast<const int32_t *>(D(3)),
+                 reinterpret_cast<const int64_t *>(D(4)),
+                 reinterpret_cast<const double *>(D(5)),
+                 reinterpret_cast<const int64_t *>(D(6)),
+                 reinterpret_cast<const int64_t *>(D(7)),
+                 reinterpret_cast<const int32_t *>(D(8)),
+                 reinterpret_cast<const double *>(D(9)),
+                 h->iab.p,
+                 h->kst.p,
+                 h->keys.p,
+                 h->vals.p,
+                 h->j2max,
+                 bl ? reinterpret_cast<const int64_t *>(D(10)) : nullptr,
+                 bl ? reinterpret_cast<const int64_t *>(D(11)) : nullptr,
+                 bl ? reinterpret_cast<const int32_t *>(D(12)) : nullptr,
+                 bl ? reinterpret_cast<const double *>(D(13)) : nullptr,
+                 bl ? reinterpret_cast<const double *>(D(13)) + 1 : nullptr,
+                 2};
+        const int64_t blocks = std::min<int64_t>((cn + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_elements, dim3((unsigned)blocks), dim3(256), 0, h->s, in, h->elem->p + h->np,
+                           h->keep->p + h->np, h->err.p, bl ? h->core->p + h->np : nullptr);
+        GCI_HIP(hipGetLastError());
+        GCI_HIP(hipEventRecord(sg.done, h->s));
+        h->np += cn;
+        h->cur ^= 1;
+    });
+}
+
+int gci_hstream_end(gci_hstream *h, gci_hmat **out, double *eav, double *elsto, int64_t *nelmnt) {
+    return guarded([&] {
+        GCI_REQUIRE(h && out && eav, "gci_hstream_end: null argument");
+        GCI_REQUIRE(h->prev_ic == h->ncf && h->prev_ir == h->ncf, "gci_hstream_end: last row incomplete");
+        hipStream_t s = h->s;
+        const int64_t np = h->np, ncf = h->ncf;
+        SBuf<int64_t> pos(np + 1, s);
+        SBuf<int> keep1(np + 1, s);
+        GCI_HIP(hipMemcpyAsync(keep1.p, h->keep->p, np * sizeof(int), hipMemcpyDeviceToDevice, s));
+        GCI_HIP(hipMemsetAsync(keep1.p + np, 0, sizeof(int), s));
+        size_t tmp_bytes = 0;
+        GCI_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, keep1.p, pos.p, np + 1, s));
+        SBuf<char> tmp(tmp_bytes, s);
+        GCI_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, keep1.p, pos.p, np + 1, s));
+        int herr = 0;
+        int64_t total = 0;
+        GCI_HIP(hipMemcpyAsync(&herr, h->err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        GCI_HIP(hipMemcpyAsync(&total, pos.p + np, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        GCI_HIP(hipStreamSynchronize(s));
+        if (herr) throw Error{GCI_ERR_NOTFOUND, "Rkintc Integral not found"};
+        auto *m = new gci_hmat();
+        try {
+            m->n = ncf;
+            m->nnz = total;
+            m->precision = h->precision;
+            GCI_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+            m->rp.alloc(ncf + 1);
+            m->col.alloc(total);
+            m->val.alloc(total);
+            m->diag.alloc(ncf);
+            auto drf = to_dev(h->row_first.data(), h->row_first.size());
+            const int64_t blocks = std::min<int64_t>((np + 255) / 256, 8192);
+            hipLaunchKernelGGL(k_compact, dim3((unsigned)blocks), dim3(256), 0, s, np, h->pic->p, h->pir->p,
+                               h->elem->p, h->keep->p, pos.p, m->col.p, m->val.p, m->diag.p);
+            hipLaunchKernelGGL(k_row_ptr, dim3((unsigned)std::min<int64_t>((ncf + 256) / 256, 4096)), dim3(256), 0, s,
+                               ncf, drf.p, pos.p, total, m->rp.p);
+            GCI_HIP(hipGetLastError());
+            std::vector<double> dg(ncf);
+            GCI_HIP(hipMemcpyAsync(dg.data(), m->diag.p, ncf * sizeof(double), hipMemcpyDeviceToHost, s));
+            std::vector<double> cs;
+            if (h->with_breit) {
+                cs.resize(np);
+                GCI_HIP(hipMemcpyAsync(cs.data(), h->core->p, np * sizeof(double), hipMemcpyDeviceToHost, s));
+            }
+            GCI_HIP(hipStreamSynchronize(s));
+            double sum = 0.0;
+            for (int64_t i = 0; i < ncf; ++i) sum = sum + dg[i];
+            double els = 0.0;
+            for (int64_t p = 0; p < (int64_t)cs.size(); ++p)
+                if (!(h->lfordr && h->pir_h[p] > h->iccut && h->pir_h[p] != h->pic_h[p])) els = els + cs[p];
+            if (elsto) *elsto = els;
+            const double e = sum / (double)ncf + els;
+            *eav = e;
+            if (h->precision == GCI_SPLIT32) {
+                m->hi.alloc(total);
+                m->lo.alloc(total);
+            }
+            const int64_t nb = std::min<int64_t>((std::max<int64_t>(ncf, total) + 255) / 256, 8192);
+            hipLaunchKernelGGL(k_shift_split, dim3((unsigned)nb), dim3(256), 0, s, ncf, m->diag.p, -e, total, m->val.p,
+                               h->precision == GCI_SPLIT32 ? m->hi.p : nullptr,
+                               h->precision == GCI_SPLIT32 ? m->lo.p : nullptr);
+            GCI_HIP(hipGetLastError());
+            GCI_HIP(hipStreamSynchronize(s));
+            if (nelmnt) *nelmnt = total + ncf;
+        } catch (...) {
+            if (m->stream) (void)hipStreamDestroy(m->stream);
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int gci_hstream_free(gci_hstream *h) {
+    return guarded([&] { delete h; });
+}
+
+}  // extern "C"

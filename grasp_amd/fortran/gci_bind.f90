@@ -196,6 +196,43 @@ module gci_bind
          integer(c_int) :: st
       end function gci_h_assemble_tb
 
+      function gci_hstream_begin(ncf, nw, iab, kstart, keys, vals, nkeys, lfordr, iccut, precision, br, out) &
+         bind(C, name="gci_hstream_begin") result(st)
+         import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+         integer(c_int), value :: ncf, nw
+         real(c_double) :: iab(*)
+         integer(c_int32_t) :: kstart(*), keys(*)
+         real(c_double) :: vals(*)
+         integer(c_int64_t), value :: nkeys
+         integer(c_int), value :: lfordr, iccut, precision
+         type(c_ptr), value :: br
+         type(c_ptr) :: out
+         integer(c_int) :: st
+      end function gci_hstream_begin
+
+      function gci_hstream_push(s, chunk, breit_chunk) bind(C, name="gci_hstream_push") result(st)
+         import :: c_int, c_ptr, gci_pair_lists
+         type(c_ptr), value :: s
+         type(gci_pair_lists), intent(in) :: chunk
+         type(c_ptr), value :: breit_chunk
+         integer(c_int) :: st
+      end function gci_hstream_push
+
+      function gci_hstream_end(s, out, eav, elsto, nelmnt) bind(C, name="gci_hstream_end") result(st)
+         import :: c_int, c_int64_t, c_double, c_ptr
+         type(c_ptr), value :: s
+         type(c_ptr) :: out
+         real(c_double) :: eav, elsto
+         integer(c_int64_t) :: nelmnt
+         integer(c_int) :: st
+      end function gci_hstream_end
+
+      function gci_hstream_free(s) bind(C, name="gci_hstream_free") result(st)
+         import :: c_int, c_ptr
+         type(c_ptr), value :: s
+         integer(c_int) :: st
+      end function gci_hstream_free
+
       function gci_hmat_upload(n, iendc, irow, emt, index_base, shift, precision, out) &
          bind(C, name="gci_hmat_upload") result(st)
          import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr

@@ -321,6 +321,74 @@ def setham(ncf, nw, lists, iab, kstart, keys, vals, lfordr=False, iccut=0, preci
     return blk, eav.value, nel.value, elsto.value
 
 
+def _pair_struct(lists, lo, hi, t_lo, t_hi, s_lo, s_hi):
+    """gci_pair_lists for pairs [lo, hi) whose two-body entries are
+    [t_lo, t_hi) and TSHELL values [s_lo, s_hi) (offsets made chunk-local)."""
+    a = {
+        "pair_ic": np.ascontiguousarray(lists["pair_ic"][lo:hi], dtype=np.int32),
+        "pair_ir": np.ascontiguousarray(lists["pair_ir"][lo:hi], dtype=np.int32),
+        "one_ia": np.ascontiguousarray(lists["one_ia"][lo:hi], dtype=np.int32),
+        "one_ib": np.ascontiguousarray(lists["one_ib"][lo:hi], dtype=np.int32),
+        "tsh_off": np.ascontiguousarray(np.asarray(lists["tsh_off"][lo:hi]) - s_lo, dtype=np.int64),
+        "tshell": np.ascontiguousarray(lists["tshell"][s_lo:s_hi], dtype=np.float64),
+        "tb_beg": np.ascontiguousarray(np.asarray(lists["tb_beg"][lo:hi]) - t_lo, dtype=np.int64),
+        "tb_end": np.ascontiguousarray(np.asarray(lists["tb_end"][lo:hi]) - t_lo, dtype=np.int64),
+        "label": np.ascontiguousarray(lists["label"][5 * t_lo:5 * t_hi], dtype=np.int32),
+        "coeff": np.ascontiguousarray(lists["coeff"][t_lo:t_hi], dtype=np.float64),
+    }
+    pl = L.GciPairLists(hi - lo, i32p(a["pair_ic"]), i32p(a["pair_ir"]), i32p(a["one_ia"]), i32p(a["one_ib"]),
+                        i64p(a["tsh_off"]), dp(a["tshell"]), len(a["tshell"]), i64p(a["tb_beg"]), i64p(a["tb_end"]),
+                        i32p(a["label"]), dp(a["coeff"]), len(a["coeff"]))
+    return pl, a
+
+
+def setham_streamed(ncf, nw, lists, iab, kstart, keys, vals, chunk_pairs, lfordr=False, iccut=0,
+                    precision=L.GCI_FP64, breit=None, tables=None):
+    """SETHAM through gci_hstream_*: the lists pushed in chunks of about
+    chunk_pairs pairs.  Returns (HamiltonianBlock, EAV, NELMNT, ELSTO)."""
+    iab = np.ascontiguousarray(iab, dtype=np.float64)
+    kstart = np.ascontiguousarray(kstart, dtype=np.int32)
+    keys = np.ascontiguousarray(keys, dtype=np.int32)
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    hs = C.c_void_p()
+    check(L.lib().gci_hstream_begin(C.c_int(ncf), C.c_int(nw), dp(iab), i32p(kstart), i32p(keys), dp(vals),
+                                    C.c_int64(len(keys)), C.c_int(int(lfordr)), C.c_int(iccut), C.c_int(precision),
+                                    tables.h if tables is not None else None, C.byref(hs)))
+    try:
+        npairs = len(lists["pair_ic"])
+        tsh_off = np.asarray(lists["tsh_off"])
+        tsh_len = np.where(np.asarray(lists["one_ia"]) == 0, 0,
+                           np.where(np.asarray(lists["one_ia"]) == np.asarray(lists["one_ib"]), nw, 1))
+        keep = []
+        for lo in range(0, npairs, chunk_pairs):
+            hi = min(npairs, lo + chunk_pairs)
+            t_lo, t_hi = int(lists["tb_beg"][lo]), int(lists["tb_end"][hi - 1])
+            s_lo = int(tsh_off[lo])
+            s_hi = int(max(s_lo, (tsh_off[lo:hi] + tsh_len[lo:hi]).max()))
+            pl, a = _pair_struct(lists, lo, hi, t_lo, t_hi, s_lo, s_hi)
+            bptr = None
+            if breit is not None:
+                b_lo, b_hi = int(breit["bt_beg"][lo]), int(breit["bt_end"][hi - 1])
+                b = {"bt_beg": np.ascontiguousarray(np.asarray(breit["bt_beg"][lo:hi]) - b_lo, dtype=np.int64),
+                     "bt_end": np.ascontiguousarray(np.asarray(breit["bt_end"][lo:hi]) - b_lo, dtype=np.int64),
+                     "blabel": np.ascontiguousarray(breit["blabel"][6 * b_lo:6 * b_hi], dtype=np.int32),
+                     "bcoeff": np.ascontiguousarray(breit["bcoeff"][b_lo:b_hi], dtype=np.float64)}
+                bl = L.GciBreitLists(i64p(b["bt_beg"]), i64p(b["bt_end"]), i32p(b["blabel"]), dp(b["bcoeff"]),
+                                     len(b["bcoeff"]))
+                keep.append(b)
+                bptr = C.byref(bl)
+            check(L.lib().gci_hstream_push(hs, C.byref(pl), bptr))
+            keep.append(a)
+        h = C.c_void_p()
+        eav, elsto, nel = C.c_double(0.0), C.c_double(0.0), C.c_int64(0)
+        check(L.lib().gci_hstream_end(hs, C.byref(h), C.byref(eav), C.byref(elsto), C.byref(nel)))
+    finally:
+        L.lib().gci_hstream_free(hs)
+    nn, nz, pr = C.c_int64(), C.c_int64(), C.c_int()
+    check(L.lib().gci_hmat_info(h, C.byref(nn), C.byref(nz), C.byref(pr)))
+    return HamiltonianBlock(h, nn.value, nz.value, pr.value), eav.value, nel.value, elsto.value
+
+
 class _PinnedBlock:
     """Owner of a page-locked host allocation (freed with the last view)."""
 

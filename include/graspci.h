@@ -180,6 +180,25 @@ int gci_h_assemble_tb(int ncf, int nw, const gci_pair_lists *lists, const gci_br
                       const double *vals, int64_t nkeys, int lfordr, int iccut, int precision,
                       gci_hmat **out, double *eav, double *elsto, int64_t *nelmnt);
 
+/* Streamed assembly: the same element loop (with or without Breit) fed in
+ * chunks while the CPU producer makes the next ones.  The reference's SETHAM
+ * interleaves production and summation row by row (setham_gg.f90:162-395);
+ * here each gci_hstream_push copies one chunk of pairs (consecutive, in the
+ * order of gci_pair_lists; offsets relative to the chunk's own arrays) into
+ * a page-locked staging slot, queues its upload and the element kernel, and
+ * returns: the copy and the kernel run while the caller produces the next
+ * chunk.  Two slots alternate.  gci_hstream_end compacts the block; the
+ * result is identical to gci_h_assemble(_tb) on the whole list for any
+ * chunking.  br (may be NULL) makes it the Breit form; then every push
+ * passes the chunk's Breit lists. */
+typedef struct gci_hstream gci_hstream;
+int gci_hstream_begin(int ncf, int nw, const double *iab, const int32_t *kstart, const int32_t *keys,
+                      const double *vals, int64_t nkeys, int lfordr, int iccut, int precision, gci_breit *br,
+                      gci_hstream **out);
+int gci_hstream_push(gci_hstream *s, const gci_pair_lists *chunk, const gci_breit_lists *breit_chunk);
+int gci_hstream_end(gci_hstream *s, gci_hmat **out, double *eav, double *elsto, int64_t *nelmnt);
+int gci_hstream_free(gci_hstream *s);
+
 /* --------------------------------------------- block Hamiltonian handle
  * .res-row ingestion (rci90/setham_gg.f90:378 layout): lower-triangle rows,
  * diagonal last, cumulative row ends iendc[0..n] (iendc[0] = 0), column
