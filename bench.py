@@ -192,7 +192,8 @@ def serve_sweep(args, L, rci, synth, sync):
 
 
 def pieces(args, L, rci, sync):
-    """Pieces 1 and 2 of the path on their own, for kernel profiles:
+    """Pieces 1 and 2 of the path on their own, and the transverse Breit
+    tables, for kernel profiles:
     the R^k table of Fe XVII with n <= 6, l <= 5 (NW = 36, gci_rk_table)
     and H assembly (gci_h_assemble) from a synthetic angular list of
     --asm-ncf CSFs.  Prints one JSON line with the wall time per call and the
@@ -239,6 +240,28 @@ def pieces(args, L, rci, sync):
         blk.free()
     t_asm = (time.perf_counter() - t0) / args.steps
     log(f"assembly: {len(lists['pair_ic'])} pairs, {nent} list entries, {nel} stored, {1e3 * t_asm:.1f} ms (wall)")
+    # the same lists streamed in chunks (gci_hstream_*): the chunk slicing on
+    # the host stands for the producer, overlapped with the uploads and the
+    # element kernel of the previous chunks
+    chunk = 65536
+    blk, *_ = rci.setham_streamed(args.asm_ncf, orb.P.shape[0], lists, iab, kst, keys, vals, chunk)  # warm
+    blk.free()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        blk, *_ = rci.setham_streamed(args.asm_ncf, orb.P.shape[0], lists, iab, kst, keys, vals, chunk)
+        blk.free()
+    t_str = (time.perf_counter() - t0) / args.steps
+    log(f"streamed assembly ({chunk}-pair chunks): {1e3 * t_str:.1f} ms (wall)")
+    # transverse Breit tables (GENINTBREIT1 + GENINTBREIT2) for the same orbitals
+    bt = rci.BreitTables(r, rp, rpor, n, h, orb, orb.ebind, W.C_LIGHT, 1.0)  # warm
+    nbr = [len(bt.table(t)[1]) for t in (1, 2)]
+    bt.free()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bt = rci.BreitTables(r, rp, rpor, n, h, orb, orb.ebind, W.C_LIGHT, 1.0)
+        bt.free()
+    t_br = (time.perf_counter() - t0) / args.steps
+    log(f"Breit tables: {nbr[0]} type 1 + {nbr[1]} type 2 integrals, {1e3 * t_br:.1f} ms (wall)")
     print(json.dumps({
         "metric": "pieces 1-2: R^k table and H assembly", "n_gpus": 1, "steps": args.steps,
         "rk_table": {"workload": "Fe XVII n<=6 l<=5, NW=36, N=590", "integrals": int(len(keys)),
@@ -247,7 +270,12 @@ def pieces(args, L, rci, sync):
         "assembly": {"workload": f"synthetic lists on a {args.asm_ncf}-CSF clustered block at "
                                  f"{args.asm_fill:.0%} fill", "pairs": int(len(lists["pair_ic"])),
                      "list_entries": int(nent), "stored": int(nel), "alg_bytes": 12.0 * nent + 12.0 * nel,
-                     "wall_ms": round(1e3 * t_asm, 3), "kernel": "k_elements"},
+                     "wall_ms": round(1e3 * t_asm, 3), "kernel": "k_elements",
+                     "streamed_wall_ms": round(1e3 * t_str, 3), "stream_chunk_pairs": chunk},
+        "breit_tables": {"workload": "GENINTBREIT1 + GENINTBREIT2, Fe XVII n<=6 l<=5, NW=36, WFACT=1",
+                         "integrals": int(sum(nbr)), "type1": int(nbr[0]), "type2": int(nbr[1]),
+                         "wall_ms": round(1e3 * t_br, 3), "integrals_per_s": round(sum(nbr) / t_br),
+                         "kernels": "k_bessel, k_breit"},
     }), flush=True)
 
 

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <thread>
 
 #include "gci_internal.h"
 
@@ -580,8 +581,28 @@ int gci_hstream_push(gci_hstream *h, const gci_pair_lists *L, const gci_breit_li
         lay.total = total;
         Stage &sg = h->st[h->cur];
         sg.ensure(total);
-        for (int i = 0; i < 13; ++i)
-            if (sizes[i]) std::memcpy(sg.host + lay.off[i], srcs[i], sizes[i]);
+        // the host copy into page-locked memory is the push's main cost: large
+        // chunks are copied by several threads, in 1 MiB pieces
+        {
+            struct Piece {
+                char *dst;
+                const char *src;
+                size_t n;
+            };
+            std::vector<Piece> pieces;
+            for (int i = 0; i < 13; ++i)
+                for (size_t o = 0; o < sizes[i]; o += (1u << 20))
+                    pieces.push_back(Piece{sg.host + lay.off[i] + o, static_cast<const char *>(srcs[i]) + o,
+                                           std::min<size_t>(1u << 20, sizes[i] - o)});
+            const int nth = (int)std::min<size_t>(8, std::max<size_t>(1, total >> 23));
+            auto work = [&](int t) {
+                for (size_t j = t; j < pieces.size(); j += nth) std::memcpy(pieces[j].dst, pieces[j].src, pieces[j].n);
+            };
+            std::vector<std::thread> th;
+            for (int t = 1; t < nth; ++t) th.emplace_back(work, t);
+            work(0);
+            for (auto &x : th) x.join();
+        }
         if (bl && nb) {
             // coefficients and their integral values side by side
             double *cv = reinterpret_cast<double *>(sg.host + lay.off[13]);
