@@ -143,9 +143,10 @@ struct SpmmShape {
 struct Pcsc {
     int R = 0;                  // rows per panel (0 = not built)
     // Both triangles laid out (split-precision matrices).  A full layout
-    // keeps each segment's entries in row order and codes rows as one-byte
-    // deltas (drow) from the segment's first row (sfirst, per slice and
-    // lane); a gap above 255 rows takes zero-valued steps of 255.  Work
+    // keeps each segment's entries in row order and codes rows in two bits
+    // (drow: one byte per lane and step) from the segment's first row
+    // (sfirst, per slice and lane): steps of 1..3 rows, or an escape entry
+    // holding the next row in its value slot for a larger gap.  Work
     // items and wavefront walks start on slice boundaries, where every
     // lane's running row is its sfirst.  The lower-triangle layout (FP64
     // matrices) codes rows as 16-bit LDS offsets (lrow) in a bank schedule.
@@ -157,7 +158,7 @@ struct Pcsc {
     DBuf<int32_t> scol;         // nslice x 64 columns (-1: empty lane)
     DBuf<int64_t> sstep;        // nslice + 1 first steps
     DBuf<uint16_t> lrow;        // nstep x 256: (row - panel start) x 8 (lower layouts)
-    DBuf<uint8_t> drow;         // nstep x 256: row deltas (full layouts)
+    DBuf<uint8_t> drow;         // nstep x 64: two-bit row codes, a lane's four per byte (full layouts)
     DBuf<uint16_t> sfirst;      // nslice x 64: first row - panel start of each lane's segment (full layouts)
     DBuf<double> val;           // nstep x 256 (FP64 values, pairs of lanes' halves)
     DBuf<float> hi, lo;         // nstep x 256 (split values)

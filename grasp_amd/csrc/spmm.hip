@@ -17,10 +17,12 @@
 //
 // Full layouts (split-precision matrices, and their FP64 copy): both
 // triangles, so every entry H_ij appears as row i of column j and as row j
-// of column i.  Segments keep row order and code rows as one-byte deltas
-// from the lane's previous row; a gap above 255 rows takes zero-valued
-// escape entries of 255.  Per slice and lane the layout keeps the segment's
-// first row.  Columns are dealt into slices in column order.
+// of column i.  Segments keep row order and code each row in two bits: a
+// step of 1..3 rows from the lane's previous row, or 0 for an escape entry,
+// whose value slot holds the next row as an integer and which contributes
+// nothing (a gap above 3 rows takes one).  A lane's four codes of a step are
+// one byte.  Per slice and lane the layout keeps the segment's first row.
+// Columns are dealt into slices in column order.
 //
 // Lower-triangle layouts (FP64 matrices): the strictly-lower entries once,
 // with 16-bit row codes (the LDS byte offset (row - panel start) x 8).  A
@@ -380,14 +382,15 @@ __global__ void k_keys_full(int64_t n, int64_t nnz, const int64_t *__restrict__ 
     }
 }
 
-constexpr int MAXDELTA = 255;  // largest one-byte row step
+constexpr int MAXDELTA = 3;  // largest row step a two-bit code carries
 
-// Escape steps before sorted entry e: a gap of g > 255 rows from the
-// previous entry of its segment takes (g - 1) / 255 zero-valued steps of 255.
+// Escape entries before sorted entry e: a gap of more than 3 rows from the
+// previous entry of its segment takes one escape entry (code 0), whose value
+// slot holds the entry's row - 1 as an integer.
 __host__ __device__ inline int32_t escapes(const uint64_t *keys, const int32_t *flag, int64_t e) {
     if (flag[e]) return 0;
     const int32_t gap = (int32_t)(keys[e] & 0xFFFFull) - (int32_t)(keys[e - 1] & 0xFFFFull);
-    return gap > MAXDELTA ? (gap - 1) / MAXDELTA : 0;
+    return gap > MAXDELTA ? 1 : 0;
 }
 
 struct EscapesOf {
@@ -426,16 +429,28 @@ __global__ void k_slice_full(int64_t nseg, const int64_t *__restrict__ ustart, c
     }
 }
 
+// Row code of position t of a lane's quad in step `step`: two bits of the
+// lane's code byte (step * 64 + lane), packed four bytes to a word.
+__device__ inline void put_code(uint32_t *codes, int64_t step, int lane, int t, uint32_t c) {
+    const int64_t b = step * SW + lane;
+    atomicOr(codes + (b >> 2), c << (8 * (int)(b & 3) + 2 * t));
+}
+
 // The stream of a full layout: each entry at its position in its lane (the
-// entries before it in its segment plus their escape steps), with its row
-// delta and value; the escape steps before it get delta 255 (value 0).
+// entries before it in its segment plus their escape entries), with its
+// two-bit row code and value.  Code c = 1..3: the row is the lane's previous
+// row + c (the segment's first entry: its first row + 1 - 1); code 0: an
+// escape -- the lane's row becomes the integer in the value slot, and the
+// entry contributes nothing.  A gap above 3 rows puts an escape holding
+// row - 1 before the entry, which then has code 1.  Tail pads are escapes to
+// row 0 (all-zero code bytes and values).
 __global__ void k_fill_full(int64_t nnz, const uint64_t *__restrict__ keys, const int32_t *__restrict__ flag,
                             const int64_t *__restrict__ segid, const int64_t *__restrict__ ustart,
                             const int32_t *__restrict__ escx,
                             const int64_t *__restrict__ seg_slice, const int32_t *__restrict__ seg_lane,
                             const int64_t *__restrict__ sstep, const uint32_t *__restrict__ perm,
                             const double *__restrict__ val, const float *__restrict__ hi, const float *__restrict__ lo,
-                            uint8_t *__restrict__ drow, double *__restrict__ pval, float *__restrict__ phi,
+                            uint32_t *__restrict__ codes, double *__restrict__ pval, float *__restrict__ phi,
                             float *__restrict__ plo) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t sg = seg_of(flag, segid, e);
@@ -444,13 +459,21 @@ __global__ void k_fill_full(int64_t nnz, const uint64_t *__restrict__ keys, cons
         const int64_t pos = (e - us) + ((int64_t)escx[e] + p - (int64_t)escx[us]);
         const int lane = seg_lane[sg];
         const int64_t s0 = sstep[seg_slice[sg]];
-        const int64_t delta =
-            flag[e] ? 0 : (int64_t)(keys[e] & 0xFFFFull) - (int64_t)(keys[e - 1] & 0xFFFFull) - MAXDELTA * p;
-        for (int64_t q = 1; q <= p; ++q)
-            drow[slot_pos(s0 + (pos - q) / 4, lane, (int)((pos - q) & 3))] = (uint8_t)MAXDELTA;
+        const uint32_t row = (uint32_t)(keys[e] & 0xFFFFull);
+        const uint32_t code =
+            (flag[e] || p) ? 1u : (uint32_t)(row - (uint32_t)(keys[e - 1] & 0xFFFFull));
+        if (p) {  // the escape entry: code 0 (nothing to set), value slot = row - 1
+            const int64_t qs = s0 + (pos - 1) / 4;
+            const int qt = (int)((pos - 1) & 3);
+            const uint32_t r1 = row - 1;
+            // stored as integers: the row must survive as bits (a float or
+            // double view of a small integer is a denormal)
+            if (pval) reinterpret_cast<uint64_t *>(pval)[val_pos(qs, lane, qt)] = (uint64_t)r1;
+            if (phi) reinterpret_cast<uint32_t *>(phi)[slot_pos(qs, lane, qt)] = r1;
+        }
         const int64_t step = s0 + pos / 4;
         const int t = (int)(pos & 3);
-        drow[slot_pos(step, lane, t)] = (uint8_t)delta;
+        put_code(codes, step, lane, t, code);
         const int64_t src = (int64_t)perm[e];
         if (pval) pval[val_pos(step, lane, t)] = val[src];
         if (phi) {
@@ -508,8 +531,8 @@ struct StreamRsrc {
 // load is placed before any branch and is the same on all paths.
 //
 // Row part: DL = false (lower layouts), four 16-bit LDS byte offsets (one
-// 8-byte load); DL = true (full layouts), four one-byte row deltas (one
-// 4-byte load).
+// 8-byte load); DL = true (full layouts), four two-bit row codes (one
+// byte load).
 template <bool DL>
 struct QRows;
 template <>
@@ -524,7 +547,7 @@ template <>
 struct QRows<true> {
     uint32_t r;
     __device__ inline void load(const StreamRsrc &p, uint32_t e) {
-        r = __builtin_amdgcn_raw_buffer_load_b32(p.rows, e, 0, 0);
+        r = __builtin_amdgcn_raw_buffer_load_b8(p.rows, e >> 2, 0, 0);  // the lane's code byte
     }
     __device__ inline void settle() const { asm volatile("" ::"v"(r)); }
 };
@@ -532,38 +555,68 @@ struct QRows<true> {
 template <int MODE>
 struct QVals;
 
+// The value slots are kept as raw bits: an escape entry's slot holds a row
+// as an integer, which must not pass through a float register view.  An
+// element is copied out of the vector before it is reinterpreted
+// (__builtin_bit_cast applied to a vector subscript reads the vector's first
+// element).
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+__device__ inline float as_float(uint32_t u) { return __uint_as_float(u); }
+
 template <>
 struct QVals<GCI_OP_SPLIT_BULK> {
-    f4v h;
+    u4v h;
     __device__ inline void load(const StreamRsrc &p, uint32_t e) {
-        h = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(p.hi, e * 4u, 0, 0));
+        h = __builtin_bit_cast(u4v, __builtin_amdgcn_raw_buffer_load_b128(p.hi, e * 4u, 0, 0));
     }
-    __device__ inline double value(int j) const { return (double)h[j]; }
+    __device__ inline double value(int j) const {
+        const uint32_t u = h[j];
+        return (double)as_float(u);
+    }
+    __device__ inline uint32_t bits(int j) const {
+        const uint32_t u = h[j];
+        return u;
+    }
     __device__ inline void settle() const { asm volatile("" ::"v"(h)); }
 };
 
 template <>
 struct QVals<GCI_OP_SPLIT_FULL> {
-    f4v h, l;
+    u4v h;
+    f4v l;
+    __device__ inline uint32_t bits(int j) const {
+        const uint32_t u = h[j];
+        return u;
+    }
     __device__ inline void load(const StreamRsrc &p, uint32_t e) {
-        h = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(p.hi, e * 4u, 0, 0));
+        h = __builtin_bit_cast(u4v, __builtin_amdgcn_raw_buffer_load_b128(p.hi, e * 4u, 0, 0));
         l = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(p.lo, e * 4u, 0, 0));
     }
-    __device__ inline double value(int j) const { return (double)h[j] + (double)l[j]; }
+    __device__ inline double value(int j) const {
+        const uint32_t u = h[j];
+        const float lj = l[j];
+        return (double)as_float(u) + (double)lj;
+    }
     __device__ inline void settle() const { asm volatile("" ::"v"(h), "v"(l)); }
 };
 
 template <>
 struct QVals<GCI_OP_FP64> {
-    d2v a, b;
+    typedef unsigned long long q2v __attribute__((ext_vector_type(2)));
+    q2v a, b;
     // e = step * 256 + lane * 4: the lane's halves sit at step * 256 + lane * 2
     // and step * 256 + 128 + lane * 2 (val_pos)
     __device__ inline void load(const StreamRsrc &p, uint32_t e) {
         const uint32_t h0 = (e & ~(uint32_t)(SE - 1)) + ((e & (SE - 1)) >> 1);
-        a = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(p.val, h0 * 8u, 0, 0));
-        b = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(p.val, (h0 + SE / 2) * 8u, 0, 0));
+        a = __builtin_bit_cast(q2v, __builtin_amdgcn_raw_buffer_load_b128(p.val, h0 * 8u, 0, 0));
+        b = __builtin_bit_cast(q2v, __builtin_amdgcn_raw_buffer_load_b128(p.val, (h0 + SE / 2) * 8u, 0, 0));
     }
-    __device__ inline double value(int j) const { return j < 2 ? a[j] : b[j - 2]; }
+    __device__ inline uint64_t raw(int j) const {
+        const uint64_t u = j < 2 ? a[j] : b[j - 2];
+        return u;
+    }
+    __device__ inline double value(int j) const { return __longlong_as_double((long long)raw(j)); }
+    __device__ inline uint32_t bits(int j) const { return (uint32_t)raw(j); }
     __device__ inline void settle() const { asm volatile("" ::"v"(a), "v"(b)); }
 };
 
@@ -576,6 +629,7 @@ struct Quad {
         v.load(p, e);
     }
     __device__ inline double value(int j) const { return v.value(j); }
+    __device__ inline uint32_t bits(int j) const { return v.bits(j); }
     __device__ inline void settle() const {
         rw.settle();
         v.settle();
@@ -671,12 +725,16 @@ __device__ inline void consume(const Quad<MODE, DL> &q, const char *xs, char *ys
     using T = typename XT<MODE>::T;
     uint32_t off[4];
     T xv[4][M];
+    double av[4];
     if constexpr (DL) {
         // full layouts keep x row-major in LDS: a row's M values are one
-        // wide read
+        // wide read.  Code c = 1..3 steps the row by c; code 0 (an escape)
+        // sets it to the value slot's integer and contributes nothing.
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            lr += (q.rw.r >> (8 * j)) & 0xFFu;
+            const uint32_t c = (q.rw.r >> (2 * j)) & 3u;
+            lr = c ? lr + c : q.bits(j);
+            av[j] = c ? q.value(j) : 0.0;
             lds_row<M, T>(reinterpret_cast<const T *>(xs) + (size_t)lr * M, xv[j]);
         }
     } else {
@@ -693,7 +751,7 @@ __device__ inline void consume(const Quad<MODE, DL> &q, const char *xs, char *ys
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const double a = q.value(j);
+        const double a = DL ? av[j] : q.value(j);
 #pragma unroll
         for (int v = 0; v < M; ++v) acc[v] = fma(a, (double)xv[j][v], acc[v]);
         if constexpr (SCATTER) {
@@ -813,7 +871,7 @@ __device__ inline void sell_stream(const SpmmItem &it, const SpmmJob &jb, char *
     const int64_t e0 = (it.step0 + a) * (int64_t)SE;
     const uint32_t ne = (uint32_t)(b - a) * SE;
     StreamRsrc P;
-    P.rows = rsrc(jb.drow + e0, ne);
+    P.rows = rsrc(jb.drow + e0 / 4, ne / 4);  // one code byte per lane and step
     P.hi = rsrc(jb.hi + e0, ne * 4u);
     P.lo = rsrc(jb.lo + e0, ne * 4u);
     P.val = rsrc(jb.val + e0, ne * 8u);
@@ -823,7 +881,7 @@ __device__ inline void sell_stream(const SpmmItem &it, const SpmmJob &jb, char *
     auto enter = [&]() {
 #pragma unroll
         for (int v = 0; v < M; ++v) acc[v] = 0.0;
-        lr = fr[si * SW + lane];
+        lr = (uint32_t)fr[si * SW + lane] - 1u;  // the first entry's code is 1
     };
     // a slice's column sums leave through the wavefront's staging row: M
     // atomics, the j-th covering (lane, vector) pairs 64j .. 64j + 63, which
@@ -1381,7 +1439,7 @@ void retain_pool() {
 }
 
 // The full layout of a split-precision matrix: both triangles, each
-// segment's entries in row order with one-byte row deltas, no bank schedule
+// segment's entries in row order with two-bit row codes, no bank schedule
 // (a full layout's walk makes no LDS atomics).
 void build_full(HMat &h, int R) {
     Pcsc &pc = h.pc;
@@ -1481,8 +1539,8 @@ void build_full(HMat &h, int R) {
     const int64_t nstep = pc.sstep_h[nslice];
     pc.nstep = nstep;
     const size_t nst = (size_t)nstep * SE;
-    pc.drow.alloc(nst);
-    GCI_HIP(hipMemsetAsync(pc.drow.p, 0, nst, s));  // tail pads: delta 0, value 0
+    pc.drow.alloc(nst / 4);  // one code byte per lane and step
+    GCI_HIP(hipMemsetAsync(pc.drow.p, 0, nst / 4, s));  // tail pads: escapes to row 0, value 0
     if (h.val.p && h.layout_fp64) {
         pc.val.alloc(nst);
         GCI_HIP(hipMemsetAsync(pc.val.p, 0, nst * sizeof(double), s));
@@ -1492,8 +1550,8 @@ void build_full(HMat &h, int R) {
     GCI_HIP(hipMemsetAsync(pc.hi.p, 0, nst * sizeof(float), s));
     GCI_HIP(hipMemsetAsync(pc.lo.p, 0, nst * sizeof(float), s));
     hipLaunchKernelGGL(k_fill_full, dim3(eb), dim3(256), 0, s, nnz, k1.p, flag.p, segid.p, ustart.p, escx.p,
-                       seg_slice.p, seg_lane.p, pc.sstep.p, i1.p, h.val.p, h.hi.p, h.lo.p, pc.drow.p, pc.val.p,
-                       pc.hi.p, pc.lo.p);
+                       seg_slice.p, seg_lane.p, pc.sstep.p, i1.p, h.val.p, h.hi.p, h.lo.p,
+                       reinterpret_cast<uint32_t *>(pc.drow.p), pc.val.p, pc.hi.p, pc.lo.p);
     GCI_HIP(hipGetLastError());
     GCI_HIP(hipStreamSynchronize(s));
 }
@@ -1505,8 +1563,8 @@ void ensure_pcsc(HMat &h) {
     Pcsc &pc = h.pc;
     // Split-precision matrices (the batched operators, several vectors per
     // product) lay out both triangles: every output is then a column sum
-    // of gathered x, with no LDS atomics, at twice the entries (one-byte
-    // row deltas keep a stored entry at 5 bytes for fp32 values).  FP64
+    // of gathered x, with no LDS atomics, at twice the entries (two-bit
+    // row codes keep a stored entry at 4.25 bytes for fp32 values).  FP64
     // matrices (the plain single-vector form) keep the lower triangle.
     pc.full = h.precision == GCI_SPLIT32;
     if (pc.full) {

@@ -57,7 +57,7 @@ def test_spmm_segment_extremes(shape_guard, fill, n):
     # a nearly dense block (segments of whole panel height, one slice per
     # panel column group) and a very sparse one (segments of one entry,
     # slices that are one step long; in the full layouts of the split
-    # matrix, row gaps far above 255 that take escape entries)
+    # matrix, row gaps far above 3 that take escape entries)
     rci.set_spmm_shape(R=1024, block=512, depth=4, nslice=8, target=16384)
     csr = W.clustered_block(n, 41, fill=fill)
     h = HamiltonianBlock.from_csr(csr)
@@ -88,15 +88,15 @@ def _lower_csr(n, entries, seed):
 
 
 def test_spmm_full_layout_escapes(shape_guard):
-    """Row gaps at and around the one-byte delta limit in a full layout: 255
-    (no escape), 256 and 510 (one escape entry of 255 rows), 511 (two), 899
-    (three), a segment whose first row is far into its panel, and one in a
-    later panel.  The running row must land on every entry exactly, in all
-    operator modes of the split matrix and in the FP64 matrix's lower
-    layout."""
+    """Row gaps at and around the two-bit code limit in a full layout: 1, 2
+    and 3 (codes, no escape), 4 and 5 (one escape entry holding the row),
+    255, 256 and 511 (one escape each), a segment whose first row is the
+    first of its panel, one far into its panel, and one in a later panel.
+    The running row must land on every entry exactly, in all operator modes
+    of the split matrix and in the FP64 matrix's lower layout."""
     rci.set_spmm_shape(R=2048, block=512, depth=8, nslice=16, target=131072)
     n = 5000
-    entries = [(r, 5) for r in (6, 261, 517, 1027, 1538, 2047)]
+    entries = [(r, 5) for r in (6, 7, 9, 12, 16, 21, 276, 532, 1043, 1538, 2047)]
     entries += [(r, 0) for r in (300, 600, 1000, 2000, 2047)]
     entries += [(2040, 1000), (4100, 3000), (4999, 3000), (4999, 4998), (2048, 2047)]
     csr = _lower_csr(n, entries, 77)
@@ -120,8 +120,8 @@ def test_spmm_autotune_keeps_parity(shape_guard):
 
 
 def test_spmm_layout_overhead(shape_guard):
-    # a full layout pads each slice to its longest segment and adds escape
-    # entries for row gaps above 255; on a clustered block the stream stays
+    # a full layout pads each slice to its longest segment and adds an escape
+    # entry for every row gap above 3; on a clustered block the stream stays
     # close to the laid-out entries
     rci.set_spmm_shape(R=1024)
     csr = W.clustered_block(6000, 61, fill=0.05)
