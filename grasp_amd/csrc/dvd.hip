@@ -934,18 +934,27 @@ __global__ __launch_bounds__(256) void k_reduce(DvdProb *P, const double *partia
     }
 }
 
-// B[r, j] -= sum_i A[r, i] coef[i, j]
-template <int NC>  // NNEW <= NC for every problem of the launch
+// B[r, j] -= sum_i A[r, i] coef[i, j].  With GRAM the same sweep also
+// writes the chunk's partial Gram matrix of the projected block (B^T B, all
+// nnew x nnew entries, in k_gram's sel 1 layout for k_chol): each thread's
+// rows in registers, then the wavefronts' sums (fixed shuffle order) added
+// in wave order, so every run sums in the same order.
+template <int NC, bool GRAM>  // NNEW <= NC for every problem of the launch
 __global__ __launch_bounds__(256) void k_update(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
-                                                int want_phase, int pass2) {
+                                                int want_phase, int pass2, double *partial) {
     DvdProb &p = P[chunk_prob[blockIdx.x]];
     if (p.phase != want_phase || (pass2 && p.skip2)) return;
     __shared__ double cf[MAXLIM * MAXNUME];
+    constexpr int NG = GRAM ? NC * (NC + 1) / 2 : 1;  // packed upper triangle
+    __shared__ double red[GRAM ? 4 : 1][NG];
     const int k = p.kpass, nn = p.nnew;
     for (int i = threadIdx.x; i < k * nn; i += blockDim.x) cf[i] = p.coef[i];
     __syncthreads();
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
     double *B = p.basis + (int64_t)k * p.n;
+    double g[NG];
+#pragma unroll
+    for (int q = 0; q < NG; ++q) g[q] = 0.0;
     for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
         double acc[NC];
 #pragma unroll
@@ -957,9 +966,37 @@ __global__ __launch_bounds__(256) void k_update(DvdProb *P, const int *chunk_pro
             for (int j = 0; j < NC; ++j)
                 if (j < nn) acc[j] = fma(a, cf[i * nn + j], acc[j]);
         }
+        double w[NC];
 #pragma unroll
-        for (int j = 0; j < NC; ++j)
-            if (j < nn) B[r + (int64_t)j * p.n] -= acc[j];
+        for (int j = 0; j < NC; ++j) {
+            w[j] = 0.0;
+            if (j < nn) {
+                w[j] = B[r + (int64_t)j * p.n] - acc[j];
+                B[r + (int64_t)j * p.n] = w[j];
+            }
+        }
+        if constexpr (GRAM) {
+            int q = 0;
+#pragma unroll
+            for (int j = 0; j < NC; ++j)
+#pragma unroll
+                for (int l = j; l < NC; ++l, ++q) g[q] = fma(w[j], w[l], g[q]);
+        }
+    }
+    if constexpr (GRAM) {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+            double v = g[q];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) red[wave][q] = v;
+        }
+        __syncthreads();
+        for (int o = threadIdx.x; o < nn * nn; o += blockDim.x) {
+            const int i = o / nn, j = o % nn, a = min(i, j), c = max(i, j);
+            const int q = a * NC - a * (a - 1) / 2 + (c - a);
+            partial[(int64_t)blockIdx.x * PSTRIDE + o] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+        }
     }
 }
 
@@ -1978,9 +2015,14 @@ void DavidsonEngine::launch_orth(int passes, int phase) {
     for (int pass = 0; pass < passes; ++pass) {
         launch_gram(G, 0, phase, pass);
         launch_reduce(dprob_.p, cap_, (const double *)partial_.p, phase, 0, stream_, pass);
-        if (nume_max_ <= 8) launch(k_update<8>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
-        else launch(k_update<MAXNUME>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
-        launch_gram(G, 1, phase, pass);
+        // the projected block's Gram matrix: from the update's sweep while
+        // every NNEW <= 8, else its own pass
+        if (nume_max_ <= 8) {
+            launch(k_update<8, true>, G, 256, stream_, dprob_.p, cp, cr, phase, pass, partial_.p);
+        } else {
+            launch(k_update<MAXNUME, false>, G, 256, stream_, dprob_.p, cp, cr, phase, pass, partial_.p);
+            launch_gram(G, 1, phase, pass);
+        }
         launch(k_chol, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, phase, pass);
         if (nume_max_ <= 8) launch(k_scale<8>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
         else launch(k_scale<MAXNUME>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
