@@ -117,9 +117,48 @@ __device__ inline int rr_pos(int j, int r, int np) { return j == 0 ? 0 : ((j - 1
 // the reflectors (DOPMTR).  nval eigenvalues go to p.eigval, the first nvec
 // eigenvectors to p.svec (k x nvec, ld k).  Called by all 512 threads;
 // W (at least 6400 doubles of LDS) is scratch.
+// 1 / q through the hardware reciprocal refined by one Newton step (q normal
+// and nonzero): within a few units of rounding, without the long dependent
+// sequence of a correctly rounded division.  The Sturm counts and the
+// inverse iteration below divide inside serial recurrences, where the
+// division's latency is the recurrence's cost; both tolerate such errors
+// (a count changes only for shifts within rounding of an eigenvalue, and
+// inverse iteration converges to the same vector).
+__device__ inline double recip(double q) {
+    const double r = __builtin_amdgcn_rcp(q);
+    return fma(fma(-q, r, 1.0), r, r);
+}
+
+// Sum over aligned groups of G lanes (G a power of two, 2..64), the same
+// value in every lane of the group: the quad and row steps move lanes
+// through DPP (lane ^ 1, lane ^ 2, then the mirrored lane in the other quad
+// of 8 and the other half of 16), the 16- and 32-lane steps through LDS
+// permutes.  Each step adds a commutative pair, so every lane of a group
+// holds the same sum, formed in the same order on every run.  Every lane of
+// the groups must be active.
+template <int CTRL>
+__device__ inline double dpp_mov(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+template <int G>
+__device__ inline double group_sum(double v) {
+    static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "group_sum: G is a power of two in 2..64");
+    v += dpp_mov<0xB1>(v);                          // quad_perm [1, 0, 3, 2]
+    if constexpr (G >= 4) v += dpp_mov<0x4E>(v);    // quad_perm [2, 3, 0, 1]
+    if constexpr (G >= 8) v += dpp_mov<0x141>(v);   // row_half_mirror
+    if constexpr (G >= 16) v += dpp_mov<0x140>(v);  // row_mirror
+    if constexpr (G >= 32) v += __shfl_xor(v, 16, 64);
+    if constexpr (G >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+__device__ inline double wave_sum(double v) { return group_sum<64>(v); }
+
 __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k, int nval, int nvec) {
     constexpr double EPS = 2.220446049250313e-16, SAFMIN = 2.2250738585072014e-308;
-    __shared__ double d[MAXLIM], e[MAXLIM], bet[MAXLIM], pv[MAXLIM], lam[MAXNUME + 1];
+    __shared__ double d[MAXLIM], e[MAXLIM], e2[MAXLIM], bet[MAXLIM], pv[MAXLIM], lam[MAXNUME + 1];
     __shared__ double red[8];
     __shared__ double tnorm_s, pivmin_s;
     const int t = threadIdx.x, nt = blockDim.x, lane = t & 63, wave = t >> 6;
@@ -130,7 +169,7 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
         if (wave == 0) {
             double s = 0.0;
             for (int i = 1 + lane; i < L; i += 64) s = fma(A[j + 1 + i][j], A[j + 1 + i][j], s);
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            s = wave_sum(s);
             if (lane == 0) {
                 const double x0 = A[j + 1][j];
                 if (s == 0.0) {
@@ -156,13 +195,21 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
             double acc = 0.0;
             if (r < L)
                 for (int c = g; c < L; c += G) acc = fma(A[j + 1 + r][j + 1 + c], A[j + 1 + c][j], acc);
-            for (int off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+            switch (G) {
+                case 64: acc = group_sum<64>(acc); break;
+                case 32: acc = group_sum<32>(acc); break;
+                case 16: acc = group_sum<16>(acc); break;
+                case 8: acc = group_sum<8>(acc); break;
+                case 4: acc = group_sum<4>(acc); break;
+                case 2: acc = group_sum<2>(acc); break;
+                default: break;
+            }
             if (r < L && g == 0) pv[r] = b * acc;
             __syncthreads();
             if (wave == 0) {
                 double s = 0.0;
                 for (int i = lane; i < L; i += 64) s = fma(pv[i], A[j + 1 + i][j], s);
-                for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+                s = wave_sum(s);
                 if (lane == 0) red[0] = 0.5 * b * s;
             }
             __syncthreads();
@@ -180,15 +227,24 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
     for (int i = t; i < k; i += nt) d[i] = A[i][i];
     if (t == 0 && k >= 2) e[k - 2] = A[k - 1][k - 2];
     __syncthreads();
-    if (t == 0) {
+    // Gershgorin bound and the largest e^2 (maxima: the same in any order),
+    // and e^2 for the Sturm counts
+    for (int i = t; i < k - 1; i += nt) e2[i] = e[i] * e[i];
+    if (wave == 0) {
         double tn = 0.0, emax = 0.0;
-        for (int i = 0; i < k; ++i) {
+        for (int i = lane; i < k; i += 64) {
             const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < k - 1 ? fabs(e[i]) : 0.0);
             tn = fmax(tn, fabs(d[i]) + r);
             if (i < k - 1) emax = fmax(emax, e[i] * e[i]);
         }
-        tnorm_s = tn;
-        pivmin_s = SAFMIN * fmax(1.0, emax);
+        for (int off = 32; off > 0; off >>= 1) {
+            tn = fmax(tn, __shfl_xor(tn, off, 64));
+            emax = fmax(emax, __shfl_xor(emax, off, 64));
+        }
+        if (lane == 0) {
+            tnorm_s = tn;
+            pivmin_s = SAFMIN * fmax(1.0, emax);
+        }
     }
     __syncthreads();
     const double tnorm = tnorm_s, pivmin = pivmin_s;
@@ -204,8 +260,16 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
             double q = d[0] - x;
             if (fabs(q) < pivmin) q = -pivmin;
             c += q < 0.0;
+            // the next step's d and e^2 are read one step ahead, off the
+            // recurrence's dependent chain
+            double dn = k > 1 ? d[1] : 0.0, en = k > 1 ? e2[0] : 0.0;
             for (int i = 1; i < k; ++i) {
-                q = (d[i] - x) - e[i - 1] * e[i - 1] / q;
+                const double di = dn, ei = en;
+                if (i + 1 < k) {
+                    dn = d[i + 1];
+                    en = e2[i];
+                }
+                q = fma(-ei, recip(q), di - x);
                 if (fabs(q) < pivmin) q = -pivmin;
                 c += q < 0.0;
             }
@@ -223,24 +287,34 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
     __syncthreads();
     // --- inverse iteration, one thread per eigenvector: T - lam is
     // factorised once (DGTTRF, LU with partial pivoting, recurrences carried
-    // in registers), then three solves (DGTTS2); after every solve wavefront 0
-    // reorthogonalises each vector against the earlier members of its
-    // cluster (eigenvalues closer than 1e-3 |T| in a row, DSTEIN's rule) and
-    // normalises it.  Scratch W: nvec vectors, then 4 x MAXLIM per thread.
-    double *Z = W;  // nvec x MAXLIM
+    // in registers, pivots kept as reciprocals), then three solves (DGTTS2);
+    // after every solve wavefront 0 reorthogonalises each vector against the
+    // earlier members of its cluster (eigenvalues closer than 1e-3 |T| in a
+    // row, DSTEIN's rule) and normalises it.  Scratch W: nvec vectors, then
+    // 4 x MAXLIM per thread.  Odd strides (in doubles) put the vectors and
+    // the threads' arrays in different LDS banks.
+    constexpr int ZS = MAXLIM + 1, TS = 4 * MAXLIM + 1;
+    static_assert((MAXNUME + 1) * ZS + (MAXNUME + 1) * TS <= MAXLIM * (MAXLIM + 1), "rr_tridiag: scratch exceeds W");
+    double *Z = W;  // nvec (<= NUME + 1) vectors of stride ZS
     const double tiny = EPS * tnorm;
     auto piv = [&](double x) { return fabs(x) >= tiny ? x : (x >= 0.0 ? tiny : -tiny); };
     uint64_t swp0 = 0, swp1 = 0;
-    double *dd = W + MAXNUME * MAXLIM + t * 4 * MAXLIM, *dl = dd + MAXLIM, *du = dl + MAXLIM, *d2 = du + MAXLIM;
+    double *dd = W + (MAXNUME + 1) * ZS + t * TS, *dl = dd + MAXLIM, *du = dl + MAXLIM, *d2 = du + MAXLIM;
     if (t < nvec) {
         const double lm = lam[t];
-        double *z = Z + t * MAXLIM;
+        double *z = Z + t * ZS;
         double dcur = d[0] - lm, ucur = k > 1 ? e[0] : 0.0;  // row i's diagonal and superdiagonal
+        // row i + 1's entries are read one step ahead, off the dependent chain
+        double sub = ucur, dnx = k > 1 ? d[1] : 0.0, unx = k > 2 ? e[1] : 0.0;
         for (int i = 0; i < k - 1; ++i) {
-            const double sub = e[i], dn = d[i + 1] - lm, un = i + 1 < k - 1 ? e[i + 1] : 0.0;
+            const double dn = dnx - lm, un = unx;
+            if (i + 2 < k) {
+                dnx = d[i + 2];
+                unx = i + 2 < k - 1 ? e[i + 2] : 0.0;
+            }
             if (fabs(dcur) >= fabs(sub)) {
                 if (dcur == 0.0) dcur = tiny;
-                const double f = sub / dcur;
+                const double f = sub * recip(dcur);
                 dd[i] = dcur;
                 dl[i] = f;
                 du[i] = ucur;
@@ -248,7 +322,7 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
                 dcur = dn - f * ucur;
                 ucur = un;
             } else {
-                const double f = dcur / sub;
+                const double f = dcur * recip(sub);
                 dd[i] = sub;
                 dl[i] = f;
                 du[i] = dn;
@@ -258,8 +332,12 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
                 if (i < 64) swp0 |= 1ull << i;
                 else swp1 |= 1ull << (i - 64);
             }
+            sub = un;  // e[i + 1]
         }
         dd[k - 1] = dcur;
+        // the solves multiply by the pivots' reciprocals (independent of each
+        // other, so they are formed side by side here)
+        for (int i = 0; i < k; ++i) dd[i] = recip(piv(dd[i]));
         for (int i = 0; i < k; ++i) {
             const double f = (double)(i + 1) * 0.6180339887498949 * (double)(t + 1);
             z[i] = (f - floor(f)) - 0.5;
@@ -267,24 +345,37 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
     }
     for (int it = 0; it < 3; ++it) {
         if (t < nvec) {
-            double *z = Z + t * MAXLIM;
+            double *z = Z + t * ZS;
             double zc = z[0];
+            double znx = z[1], dlx = dl[0];  // read a step ahead (k >= 3)
             for (int i = 0; i < k - 1; ++i) {  // forward, with the row swaps
                 const bool sw = ((i < 64 ? swp0 >> i : swp1 >> (i - 64)) & 1) != 0;
-                const double zn = z[i + 1];
+                const double zn = znx, dli = dlx;
+                if (i + 2 < k) {
+                    znx = z[i + 2];
+                    dlx = dl[i + 1];
+                }
                 if (!sw) {
                     z[i] = zc;
-                    zc = zn - dl[i] * zc;
+                    zc = zn - dli * zc;
                 } else {
                     z[i] = zn;
-                    zc = zc - dl[i] * zn;
+                    zc = zc - dli * zn;
                 }
             }
             z[k - 1] = zc;
-            double z1 = zc / piv(dd[k - 1]), z2 = 0.0;  // x[i+1], x[i+2]
+            double z1 = zc * dd[k - 1], z2 = 0.0;  // x[i+1], x[i+2]
             z[k - 1] = z1;
+            double zx = z[k - 2], dux = du[k - 2], d2x = d2[k - 2], ddx = dd[k - 2];  // a step ahead
             for (int i = k - 2; i >= 0; --i) {
-                const double x = (z[i] - du[i] * z1 - d2[i] * z2) / piv(dd[i]);
+                const double zi = zx, dui = dux, d2i = d2x, ddi = ddx;
+                if (i > 0) {
+                    zx = z[i - 1];
+                    dux = du[i - 1];
+                    d2x = d2[i - 1];
+                    ddx = dd[i - 1];
+                }
+                const double x = (zi - dui * z1 - d2i * z2) * ddi;
                 z[i] = x;
                 z2 = z1;
                 z1 = x;
@@ -293,19 +384,19 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
         __syncthreads();
         if (wave == 0) {
             for (int m = 0; m < nvec; ++m) {
-                double *z = Z + m * MAXLIM;
+                double *z = Z + m * ZS;
                 int q0 = m;
                 while (q0 > 0 && lam[q0] - lam[q0 - 1] < 1.0e-3 * tnorm) --q0;
                 for (int q = q0; q < m; ++q) {
-                    const double *zq = Z + q * MAXLIM;
+                    const double *zq = Z + q * ZS;
                     double sd = 0.0;
                     for (int i = lane; i < k; i += 64) sd = fma(zq[i], z[i], sd);
-                    for (int off = 32; off > 0; off >>= 1) sd += __shfl_xor(sd, off, 64);
+                    sd = wave_sum(sd);
                     for (int i = lane; i < k; i += 64) z[i] = fma(-sd, zq[i], z[i]);
                 }
                 double nr = 0.0;
                 for (int i = lane; i < k; i += 64) nr = fma(z[i], z[i], nr);
-                for (int off = 32; off > 0; off >>= 1) nr += __shfl_xor(nr, off, 64);
+                nr = wave_sum(nr);
                 const double sc = 1.0 / sqrt(nr);
                 for (int i = lane; i < k; i += 64) z[i] *= sc;
             }
@@ -314,19 +405,19 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
     }
     // --- back-transformation x = H_0 ... H_{k-3} z: a 64-lane wave per vector
     for (int m = wave; m < nvec; m += nt / 64) {
-        double *z = Z + m * MAXLIM;
+        double *z = Z + m * ZS;
         for (int j = k - 3; j >= 0; --j) {
             const double b = bet[j];
             if (b == 0.0) continue;
             const int L = k - j - 1;
             double s = 0.0;
             for (int i = lane; i < L; i += 64) s = fma(A[j + 1 + i][j], z[j + 1 + i], s);
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            s = wave_sum(s);
             for (int i = lane; i < L; i += 64) z[j + 1 + i] = fma(-b * s, A[j + 1 + i][j], z[j + 1 + i]);
         }
     }
     __syncthreads();
-    for (int idx = t; idx < k * nvec; idx += nt) p.svec[idx] = Z[(idx / k) * MAXLIM + idx % k];
+    for (int idx = t; idx < k * nvec; idx += nt) p.svec[idx] = Z[(idx / k) * ZS + idx % k];
     for (int m = t; m < nval; m += nt) p.eigval[m] = lam[m];
 }
 
@@ -987,8 +1078,7 @@ __global__ __launch_bounds__(256) void k_update(DvdProb *P, const int *chunk_pro
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
         for (int q = 0; q < NG; ++q) {
-            double v = g[q];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            const double v = wave_sum(g[q]);
             if (lane == 0) red[wave][q] = v;
         }
         __syncthreads();
