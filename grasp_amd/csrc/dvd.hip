@@ -1346,8 +1346,18 @@ __global__ void k_init_stage(DvdProb *P, int nprob, int stage) {
             p.init_iend += nw;
         }
     } else {
-        kp = 0;
-        nw = p.init_iend;
+        // S = B^T AB in blocks of 16 columns (the Gram kernel's B tile):
+        // stage -1 - b takes columns 16 b .. 16 b + 15 of the INIT_IEND
+        // built, against basis columns 0 .. 16 b + 15 (ADDS per block); a
+        // problem with no columns left keeps its KPASS
+        const int b = -1 - stage;
+        kp = 16 * b;
+        nw = max(0, min(16, p.init_iend - kp));
+        if (nw == 0) {
+            p.phase = PH_INIT_IDLE;
+            p.nnew = 0;
+            return;
+        }
     }
     p.phase = nw > 0 ? PH_INIT : PH_INIT_IDLE;
     p.kpass = kp;
@@ -2077,10 +2087,15 @@ void DavidsonEngine::run_initdvd(const std::vector<int> &slots, bool unit_op) {
         launch_orth(2, PH_INIT);
         launch_op(PH_INIT, init_width, modes);
     }
-    launch(k_init_stage, pg, 64, stream_, dprob_.p, cap_, -1);
-    launch_gram(G, 2, (int)PH_INIT, 0);
-    launch_reduce(dprob_.p, cap_, (const double *)partial_.p, (int)PH_INIT, 1, stream_);
-    launch(k_adds, cap_, 64, stream_, dprob_.p, (int)PH_INIT, 0);
+    // INIT_IEND = max(NIV, NUME + 1) columns, S in blocks of 16
+    int iend_max = 1;
+    for (int sl : slots) iend_max = std::max(iend_max, std::max(niv_[sl], spec_[sl].nume + 1));
+    for (int b = 0; b < (iend_max + 15) / 16; ++b) {
+        launch(k_init_stage, pg, 64, stream_, dprob_.p, cap_, -1 - b);
+        launch_gram(G, 2, (int)PH_INIT, 0);
+        launch_reduce(dprob_.p, cap_, (const double *)partial_.p, (int)PH_INIT, 1, stream_);
+        launch(k_adds, cap_, 64, stream_, dprob_.p, (int)PH_INIT, 0);
+    }
     launch(k_init_done, pg, 64, stream_, dprob_.p, cap_);
     GCI_HIP(hipGetLastError());
     for (int sl : slots) {

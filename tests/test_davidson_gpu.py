@@ -108,16 +108,25 @@ def _fp64_residuals(csr, eav, V, ev):
     return np.linalg.norm(A - V * ev, axis=0)
 
 
-def test_maneig_nvec_above_ten(gpu):
-    # LIM = MIN(NCF, 2*NVECMX+60) = 84 for 12 eigenpairs (maneig.f90:389)
+@pytest.mark.parametrize("nev", [12, 16])
+def test_maneig_nvec_above_ten(gpu, nev):
+    # LIM = MIN(NCF, 2*NVECMX+60) = 84 for 12 eigenpairs and 92 for 16, the
+    # largest NVECMX the engine takes (maneig.f90:389); with 16, INITDVD
+    # builds NUME + 1 = 17 columns (S = B^T AB in two blocks of the Gram
+    # kernel's 16-column tile) and the Rayleigh-Ritz step computes 17 Ritz
+    # vectors
     csr = W.clustered_block(3000, 12, fill=0.05, sorted_diag=True)
     eav = _eav(csr)
     h = HamiltonianBlock.from_csr(csr, shift=-eav)
-    ev, evec, st = maneig(h, 12)
-    want, wvec, nl, nm, rc = O.maneig(csr.row_end, csr.col, csr.val, eav, 12, nthreads=8)
+    ev, evec, st = maneig(h, nev)
+    want, wvec, nl, nm, rc = O.maneig(csr.row_end, csr.col, csr.val, eav, nev, nthreads=8)
     assert rc == 0 and st["ierr"] == 0
     assert np.all(_rel(ev, want, eav) < 1e-10), _rel(ev, want, eav)
-    assert np.max(_fp64_residuals(csr, eav, evec, ev)) < 1e-6
+    # DVDSON stops a vector on any of CRITE, CRITC, CRITR, so a residual can
+    # stay above CRITR: the same as the oracle's (one vector at 4.2e-6 for
+    # 16 pairs on this block), else below 1e-6
+    ours, theirs = _fp64_residuals(csr, eav, evec, ev), _fp64_residuals(csr, eav, wvec, want)
+    assert np.all(ours < np.maximum(1e-6, 1.01 * theirs)), (ours, theirs)
 
 
 def test_davidson_rejects_nvec_beyond_limit(gpu):
