@@ -2121,15 +2121,19 @@ void DavidsonEngine::launch_orth(int passes, int phase) {
         launch_gram(G, 0, phase, pass);
         launch_reduce(dprob_.p, cap_, (const double *)partial_.p, phase, 0, stream_, pass);
         // the projected block's Gram matrix: from the update's sweep while
-        // every NNEW <= 8, else its own pass
-        if (nume_max_ <= 8) {
+        // every NNEW <= 8 (register accumulators sized for the widest
+        // block), else its own pass
+        if (nume_max_ <= 4) {
+            launch(k_update<4, true>, G, 256, stream_, dprob_.p, cp, cr, phase, pass, partial_.p);
+        } else if (nume_max_ <= 8) {
             launch(k_update<8, true>, G, 256, stream_, dprob_.p, cp, cr, phase, pass, partial_.p);
         } else {
             launch(k_update<MAXNUME, false>, G, 256, stream_, dprob_.p, cp, cr, phase, pass, partial_.p);
             launch_gram(G, 1, phase, pass);
         }
         launch(k_chol, cap_, 256, stream_, dprob_.p, (const double *)partial_.p, phase, pass);
-        if (nume_max_ <= 8) launch(k_scale<8>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
+        if (nume_max_ <= 4) launch(k_scale<4>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
+        else if (nume_max_ <= 8) launch(k_scale<8>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
         else launch(k_scale<MAXNUME>, G, 256, stream_, dprob_.p, cp, cr, phase, pass);
     }
 }
@@ -2205,7 +2209,8 @@ void DavidsonEngine::launch_iteration() {
     launch(k_rr, cap_, 512, stream_, dprob_.p);
     launch(k_rotate, G, 256, stream_, dprob_.p, cp, cr, 0);
     launch(k_ovf_fix, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_);
-    if (nume_max_ <= 8) launch(k_newvec<8>, G, 256, stream_, dprob_.p, cp, cr, partial_.p);
+    if (nume_max_ <= 4) launch(k_newvec<4>, G, 256, stream_, dprob_.p, cp, cr, partial_.p);
+    else if (nume_max_ <= 8) launch(k_newvec<8>, G, 256, stream_, dprob_.p, cp, cr, partial_.p);
     else launch(k_newvec<MAXNUME>, G, 256, stream_, dprob_.p, cp, cr, partial_.p);
     launch(k_newvec_decide, cap_, 256, stream_, dprob_.p, (const double *)partial_.p);
     launch(k_precond, G, 256, stream_, dprob_.p, cp, cr);
