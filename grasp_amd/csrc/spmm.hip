@@ -688,11 +688,17 @@ __host__ __device__ constexpr SellLds sell_lds(int R, int NS) {
     m.total = m.ss + round16((size_t)(NS + 1) * sizeof(int));
     return m;
 }
+// Elements per x row of the stream walk's LDS panel: M, except five FP64
+// values, which are padded to six so that a row is three aligned 16-byte
+// reads instead of five 8-byte ones.
+template <int M, class T>
+__host__ __device__ constexpr int xrow() { return (M == 5 && sizeof(T) == 8) ? 6 : M; }
+
 template <int M, class T>
 __host__ __device__ constexpr SellLds stream_lds(int R, int NI, int W) {
     SellLds m{};
     m.xs = 0;
-    m.cl = round16((size_t)R * M * sizeof(T));
+    m.cl = round16((size_t)R * xrow<M, T>() * sizeof(T));
     m.fr = m.cl + (size_t)NI * SW * sizeof(int);
     m.ss = m.fr + (size_t)NI * SW * sizeof(uint16_t);
     m.st = m.ss + round16((size_t)(NI + 1) * sizeof(int));
@@ -704,7 +710,17 @@ __host__ __device__ constexpr SellLds stream_lds(int R, int NI, int W) {
 // few reads as the row's size and 16-byte alignment allow.
 template <int M, class T>
 __device__ inline void lds_row(const T *p, T (&o)[M]) {
-    if constexpr ((M & (M - 1)) == 0 && M * sizeof(T) >= 16) {
+    if constexpr (xrow<M, T>() != M) {  // padded row: whole 16-byte reads, the pad dropped
+        constexpr int Q = xrow<M, T>() * (int)sizeof(T) / 16;
+        T w[Q * 16 / sizeof(T)];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const f4v v = reinterpret_cast<const f4v *>(p)[q];
+            __builtin_memcpy(reinterpret_cast<char *>(w) + 16 * q, &v, 16);
+        }
+#pragma unroll
+        for (int v = 0; v < M; ++v) o[v] = w[v];
+    } else if constexpr ((M & (M - 1)) == 0 && M * sizeof(T) >= 16) {
 #pragma unroll
         for (int q = 0; q < (int)(M * sizeof(T) / 16); ++q) {
             const f4v w = reinterpret_cast<const f4v *>(p)[q];
@@ -740,7 +756,7 @@ __device__ inline void consume(const Quad<MODE, DL> &q, const char *xs, char *ys
             const uint32_t c = (q.rw.r >> (2 * j)) & 3u;
             lr = c ? lr + c : q.bits(j);
             av[j] = c ? q.value(j) : 0.0;
-            lds_row<M, T>(reinterpret_cast<const T *>(xs) + (size_t)lr * M, xv[j]);
+            lds_row<M, T>(reinterpret_cast<const T *>(xs) + (size_t)lr * xrow<M, T>(), xv[j]);
         }
     } else {
         off[0] = q.rw.r.x & 0xFFFFu;
@@ -816,7 +832,7 @@ __device__ inline void sell_stream(const SpmmItem &it, const SpmmJob &jb, char *
             const int rr = t + (j0 + j) * BS;
             if (rr < rows) {
 #pragma unroll
-                for (int v = 0; v < M; ++v) xs[rr * M + v] = xr[j][v];
+                for (int v = 0; v < M; ++v) xs[rr * xrow<M, T>() + v] = xr[j][v];
             }
         }
     }
