@@ -61,6 +61,31 @@ def test_batch_solve_mixed_sizes(gpu):
         _assert_residuals(csr, eav, evecs[b], evals[b], tol)
 
 
+@pytest.mark.parametrize("engines", [2, 3])
+def test_batch_solve_engine_groups_identical(gpu, monkeypatch, engines):
+    """GCI_BATCH_ENGINES splits the blocks into contiguous groups, one
+    Davidson engine and stream each, driven in turn: every block's
+    eigenpairs agree with the one-engine solve's to rounding (the SpMM's
+    column sums arrive through atomics in no fixed order, and a group's
+    launch width follows its own blocks)."""
+    sizes = [120, 350, 900, 1600, 2400, 240, 3000, 700]
+    fills = [0.1, 0.08, 0.05, 0.04, 0.03, 0.1, 0.02, 0.06]
+    nev = [4, 2, 4, 3, 4, 1, 4, 2]
+    data = _blocks(sizes, 300, fills)
+    blocks = [rci.HamiltonianBlock.from_csr(c, shift=-e, precision=L.GCI_SPLIT32) for c, e in data]
+    eavs = [e for _, e in data]
+    monkeypatch.setenv("GCI_BATCH_ENGINES", "1")
+    one = rci.batch_solve(blocks, nev, 1e-9, eavs)
+    monkeypatch.setenv("GCI_BATCH_ENGINES", str(engines))
+    many = rci.batch_solve(blocks, nev, 1e-9, eavs)
+    for b in range(len(blocks)):
+        rel = np.abs(one[0][b] - many[0][b]) / np.abs(one[0][b] + eavs[b])
+        assert np.all(rel < 1e-12), (b, rel)
+        overlap = np.abs(np.sum(one[1][b] * many[1][b], axis=0))
+        assert np.all(np.abs(overlap - 1.0) < 1e-9), (b, overlap)
+        assert many[2][b]["ierr"] == 0
+
+
 def test_batch_solve_fp64_blocks(gpu):
     data = _blocks([400, 1500], 200, [0.1, 0.05])
     blocks = [rci.HamiltonianBlock.from_csr(c, shift=-e) for c, e in data]
