@@ -154,6 +154,8 @@ class DavidsonEngine {
         waited_seq_ = launched_;
     }
     int active() const;
+    // admitted slots still iterating (neither done nor empty), as of the last poll
+    int iterating() const;
     int capacity() const { return cap_; }
     int phase(int slot) const { return phase_[slot]; }
     static size_t workspace_bytes(const DvdSpec &s);

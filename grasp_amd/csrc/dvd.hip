@@ -1831,6 +1831,12 @@ int DavidsonEngine::active() const {
     return a;
 }
 
+int DavidsonEngine::iterating() const {
+    int a = 0;
+    for (int i = 0; i < cap_; ++i) a += phase_[i] != PH_EMPTY && phase_[i] != PH_DONE;
+    return a;
+}
+
 void DavidsonEngine::rebuild_chunks() {
     hprob_.clear();
     hrow_.clear();
