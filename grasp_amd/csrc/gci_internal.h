@@ -136,7 +136,7 @@ struct SpmmShape {
     int R = 2048;
     int block = 512;
     int64_t target = 131072;
-    int nslice = 16;
+    int nslice = 64;
     int depth = 8;
 };
 
