@@ -1025,6 +1025,25 @@ __global__ __launch_bounds__(256) void k_reduce(DvdProb *P, const double *partia
     }
 }
 
+// The second Gram-Schmidt pass changes nothing that matters when the first
+// left every new column orthogonal to the basis to within PASS2_TOL: its
+// coefficients (the cosines between the unit new columns and the basis
+// vectors) are then all at most that, and the block is kept as the first
+// pass normalised it.  Evaluated by every thread of the block alike (the
+// same coefficients, the same answer); NaN counts as large.
+constexpr double PASS2_TOL = 1.0e-13;
+__device__ bool pass2_negligible(const DvdProb &p, int k, int nn) {
+    __shared__ int big;
+    if (threadIdx.x == 0) big = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < k * nn; i += blockDim.x)
+        if (!(fabs(p.coef[i]) <= PASS2_TOL)) big = 1;
+    __syncthreads();
+    const bool r = big == 0;
+    __syncthreads();  // every thread has read the flag before a later call resets it
+    return r;
+}
+
 // B[r, j] -= sum_i A[r, i] coef[i, j].  With GRAM the same sweep also
 // writes the chunk's partial Gram matrix of the projected block (B^T B, all
 // nnew x nnew entries, in k_gram's sel 1 layout for k_chol): each thread's
@@ -1041,6 +1060,7 @@ __global__ __launch_bounds__(256) void k_update(DvdProb *P, const int *chunk_pro
     const int k = p.kpass, nn = p.nnew;
     for (int i = threadIdx.x; i < k * nn; i += blockDim.x) cf[i] = p.coef[i];
     __syncthreads();
+    if (pass2 && pass2_negligible(p, k, nn)) return;
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
     double *B = p.basis + (int64_t)k * p.n;
     double g[NG];
@@ -1104,6 +1124,10 @@ __global__ void k_chol(DvdProb *P, const double *partial, int want_phase, int pa
     __shared__ double R[MAXNUME][MAXNUME + 1];
     __shared__ double w0[MAXNUME];  // squared norms of the new columns before this pass's projection
     const int nn = p.nnew, k = p.kpass;
+    if (pass == 1 && pass2_negligible(p, k, nn)) {  // k_update skipped the second pass too
+        if (threadIdx.x == 0) p.skip2 = 1;       // and so does k_scale
+        return;
+    }
     reduce_partials(p, partial, nn * nn, G);
     // |w|^2 before projection = |w_projected|^2 + sum_i coef_ij^2 (orthonormal basis)
     for (int j = threadIdx.x; j < nn; j += blockDim.x) {
