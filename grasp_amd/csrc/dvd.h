@@ -246,6 +246,10 @@ class DavidsonEngine {
     hipGraphExec_t gexec_ = nullptr;
     bool graph_valid_ = false;
     uint64_t shape_version_ = 0;  // SpMM shape the captured graph was built for
+    // the captured iteration carries OVFLOW's rotation and reset (k_rotate,
+    // k_ovf_fix): some slot's KPASS may reach LIM in the iterations queued
+    bool ovf_in_graph_ = true;
+    bool ovflow_possible(int iters) const;
 };
 
 // Operator profiling (gci_profile_*): per operator mode, launches, summed

@@ -2237,8 +2237,10 @@ void DavidsonEngine::launch_iteration() {
     const int *cp = chunk_prob_.p;
     const int64_t *cr = chunk_row_.p;
     launch(k_rr, cap_, 512, stream_, dprob_.p);
-    launch(k_rotate, G, 256, stream_, dprob_.p, cp, cr, 0);
-    launch(k_ovf_fix, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_);
+    if (ovf_in_graph_) {
+        launch(k_rotate, G, 256, stream_, dprob_.p, cp, cr, 0);
+        launch(k_ovf_fix, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_);
+    }
     if (nume_max_ <= 4) launch(k_newvec<4>, G, 256, stream_, dprob_.p, cp, cr, partial_.p);
     else if (nume_max_ <= 8) launch(k_newvec<8>, G, 256, stream_, dprob_.p, cp, cr, partial_.p);
     else launch(k_newvec<MAXNUME>, G, 256, stream_, dprob_.p, cp, cr, partial_.p);
@@ -2268,8 +2270,29 @@ void DavidsonEngine::drain_profile() {
     prof_events_.clear();
 }
 
+// Whether some slot's KPASS can reach LIM (OVFLOW) within the next `iters`
+// iterations: after its (re)start a slot enters DVDRVR with KPASS =
+// max(NIV, NUME + 1) and adds at most MBLOCK columns per iteration, so in
+// its t-th iteration the Rayleigh-Ritz step sees KPASS <= that + (t-1) MBLOCK.
+bool DavidsonEngine::ovflow_possible(int iters) const {
+    for (int i = 0; i < cap_; ++i) {
+        if (phase_[i] == PH_EMPTY || phase_[i] == PH_DONE) continue;
+        const DvdSpec &sp = spec_[i];
+        const int64_t kp0 = std::max(niv_[i], sp.nume + 1);
+        const int64_t t = launched_ - restart_seq_[i] + iters;  // the last iteration queued by this step
+        if (kp0 + (t - 1) * (int64_t)sp.mblock >= sp.lim) return true;
+    }
+    return false;
+}
+
 void DavidsonEngine::step(int iters) {
     if (total_chunks_ == 0) return;
+    // OVFLOW's kernels join the iteration once a slot could overflow
+    const bool ovf = ovflow_possible(iters);
+    if (ovf != ovf_in_graph_) {
+        ovf_in_graph_ = ovf;
+        graph_valid_ = false;
+    }
     if (use_graph && !op_profile().on) {
         if (!graph_valid_) {
             if (gexec_) {
