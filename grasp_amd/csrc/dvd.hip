@@ -2175,10 +2175,11 @@ void DavidsonEngine::update_modes() {
         const DvdSpec &sp = spec_[i];
         nmax = std::max(nmax, sp.nume);
         lmax = std::max(lmax, sp.lim);
+        mw = std::max(mw, sp.mblock);
+        if (phase_[i] == PH_DONE) continue;  // no operator launches for finished slots
         if (sp.mode == GCI_OP_FP64) modes |= 1 << GCI_OP_FP64;
         else if (refined_[i]) modes |= 1 << GCI_OP_SPLIT_FULL;
         else modes |= 1 << sp.mode;
-        mw = std::max(mw, sp.mblock);
     }
     if (modes != modes_used_ || mw != mwidth_iter_ || nmax != nume_max_ || (lmax > 80) != (lim_max_ > 80) ||
         spmm_shape_version() != shape_version_)
@@ -2287,7 +2288,10 @@ bool DavidsonEngine::ovflow_possible(int iters) const {
 
 void DavidsonEngine::step(int iters) {
     if (total_chunks_ == 0) return;
-    // OVFLOW's kernels join the iteration once a slot could overflow
+    // the operator modes of the slots still iterating (a mode whose slots
+    // have all finished leaves the captured iteration), and OVFLOW's
+    // kernels once a slot could overflow
+    update_modes();
     const bool ovf = ovflow_possible(iters);
     if (ovf != ovf_in_graph_) {
         ovf_in_graph_ = ovf;
