@@ -152,6 +152,7 @@ class DavidsonEngine {
     void sync() {
         GCI_HIP(hipStreamSynchronize(stream_));
         waited_seq_ = launched_;
+        drain_profile(true);
     }
     int active() const;
     // admitted slots still iterating (neither done nor empty), as of the last poll
@@ -174,12 +175,14 @@ class DavidsonEngine {
     // unit_op: every starting problem has unit start vectors (k_unit_merge),
     // whose first product is read from H's columns instead of an SpMM pass
     void run_initdvd(const std::vector<int> &slots, bool unit_op = false);
-    void drain_profile();
+    void drain_profile(bool wait);
+    hipEvent_t pooled_event();
     struct ProfEv {
         int mode;
         hipEvent_t e0, e1;
     };
     std::vector<ProfEv> prof_events_;
+    std::vector<hipEvent_t> ev_pool_;  // events of drained profile pairs, reused
 
     int cap_;
     hipStream_t stream_;

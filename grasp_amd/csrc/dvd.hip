@@ -858,24 +858,30 @@ __global__ __launch_bounds__(256) void k_precond(DvdProb *P, const int *chunk_pr
 //      1 = (basis[:,kpass:+nnew], itself)                 (Cholesky-QR)
 //      2 = (basis[:,0:kpass+nnew], ab[:,kpass:+nnew])     (ADDS)
 //      3 = (ab[:,0:nume] - eigval*basis[:,0:nume]) norms  (exit residuals)
-// Sel 0-2 are tall-skinny A^T B products on the FP64 matrix cores: the
-// chunk is staged through LDS 64 rows at a time; wave w accumulates rows
-// 16w..16w+15 of every staged tile into all 16 x 16 output tiles (16
-// columns of A x up to 16 of B) with v_mfma_f64_16x16x4f64, 4 rows per
-// instruction in ascending order, and the four waves' sums are added in
-// wave order, so every output is summed in the same order on every run.
+// Sel 0-2 are tall-skinny A^T B products on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64).  Wave w of the chunk's workgroup takes rows
+// w*CHUNK/4 .. (w+1)*CHUNK/4 - 1 in 16-row groups and loads each group's
+// operands straight from HBM into the instruction's lane layout (lane l
+// holds A[r][i] and B[r][j] for row r = 4 ks + l/16 and column i, j = l%16),
+// the next group's loads in flight while the matrix cores take the current
+// one; no LDS staging, so occupancy is set by registers alone.  The waves'
+// sums are added in wave order, so every output is summed in the same order
+// on every run.
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 // LIMC: the largest LIM among the resident problems the launch serves (80
-// or MAXLIM); the A tile and the accumulators are sized for it.
+// or MAXLIM); the accumulators are sized for it.
 template <int LIMC>
 __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob, const int64_t *chunk_row,
                                               double *partial, int sel, int want_phase, int pass2) {
     static_assert(LIMC % 16 == 0 && LIMC <= MAXLIM, "k_gram: LIMC must be a multiple of 16 up to MAXLIM");
+    static_assert(CHUNK % 64 == 0, "k_gram: a wave's rows are whole 16-row groups");
     DvdProb &p = P[chunk_prob[blockIdx.x]];
     if (p.phase != want_phase || (sel == 3 && bulk_exit(p)) || (pass2 && p.skip2)) return;
-    __shared__ double ta[64][LIMC + 1];
-    __shared__ double tb[64][MAXNUME + 1];
+    constexpr int NT = LIMC / 16;
+    constexpr int RED = NT * 4 * 64;  // one wave's accumulators
+    static_assert(3 * RED >= 64 * (MAXNUME + 1), "k_gram: sel 3's row tile exceeds the reduction scratch");
+    __shared__ double red[3 * RED];
     const int k = p.kpass, nn = p.nnew;
     const double *A, *B;
     int ka, kb;
@@ -886,77 +892,59 @@ __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob,
     const int64_t r0 = chunk_row[blockIdx.x], r1 = min(r0 + CHUNK, p.n);
     if (sel != 3) {
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int nti = (ka + 15) >> 4;  // 16-column tiles of A (<= LIMC / 16); B fits one tile (kb <= 16)
-        constexpr int NT = LIMC / 16;
-        // every wave takes 16 of each 64-row tile's rows for all output
-        // tiles, so the chains of dependent MFMAs are 4 long per tile
+        const int nti = (ka + 15) >> 4;  // 16-column tiles of A (<= NT); B fits one tile (kb <= 16)
+        const int cl = lane & 15, kl = lane >> 4;
+        const int64_t w0 = r0 + (int64_t)wave * (CHUNK / 4), w1 = min(w0 + CHUNK / 4, r1);
+        const int64_t n = p.n;
         d4v acc[NT];
 #pragma unroll
         for (int it = 0; it < NT; ++it) acc[it] = d4v{0.0, 0.0, 0.0, 0.0};
-        // the next tile's elements are loaded into registers while the
-        // matrix cores work on the current one (256 threads, 64 rows)
-        constexpr int QA = 64 * LIMC / 256, QB = 64 * MAXNUME / 256;
-        double ra[QA], rb[QB];
-        auto fetch = [&](int64_t rt) {
-            const int rows = (int)min<int64_t>(64, r1 - rt);
-#pragma unroll
-            for (int q = 0; q < QA; ++q) {
-                const int idx = threadIdx.x + q * 256, rr = idx & 63, i = idx >> 6;
-                ra[q] = (i < ka && rr < rows) ? A[rt + rr + (int64_t)i * p.n] : 0.0;
-            }
-#pragma unroll
-            for (int q = 0; q < QB; ++q) {
-                const int idx = threadIdx.x + q * 256, rr = idx & 63, j = idx >> 6;
-                rb[q] = (j < kb && rr < rows) ? B[rt + rr + (int64_t)j * p.n] : 0.0;
-            }
-        };
-        fetch(r0);
-        for (int64_t rt = r0; rt < r1; rt += 64) {
-#pragma unroll
-            for (int q = 0; q < QA; ++q) {
-                const int idx = threadIdx.x + q * 256;
-                if ((idx >> 6) < ka) ta[idx & 63][idx >> 6] = ra[q];
-            }
-#pragma unroll
-            for (int q = 0; q < QB; ++q) {
-                const int idx = threadIdx.x + q * 256;
-                if ((idx >> 6) < kb) tb[idx & 63][idx >> 6] = rb[q];
-            }
-            __syncthreads();
-            if (rt + 64 < r1) fetch(rt + 64);
-            // operands: lane l holds A^T[i = l%16][r = l/16] and B[r = l/16][j = l%16]
-            const int j = lane & 15;
+        double ga0[4][NT], gb0[4];
+        auto load = [&](double (&ga)[4][NT], double (&gb)[4], int64_t g) {
 #pragma unroll
             for (int ks = 0; ks < 4; ++ks) {
-                const int r = wave * 16 + ks * 4 + (lane >> 4);
-                const double bv = j < kb ? tb[r][j] : 0.0;
+                const int64_t q = g + ks * 4 + kl;
+                const bool ok = q < w1;
+                gb[ks] = (ok && cl < kb) ? B[q + (int64_t)cl * n] : 0.0;
 #pragma unroll
                 for (int it = 0; it < NT; ++it) {
-                    if (it < nti) {
-                        const int i = it * 16 + (lane & 15);
-                        const double av = i < ka ? ta[r][i] : 0.0;
-                        acc[it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[it], 0, 0, 0);
-                    }
+                    const int i = it * 16 + cl;
+                    ga[ks][it] = (it < nti && ok && i < ka) ? A[q + (int64_t)i * n] : 0.0;
                 }
             }
-            __syncthreads();
+        };
+        auto mul = [&](const double (&ga)[4][NT], const double (&gb)[4]) {
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+                for (int it = 0; it < NT; ++it)
+                    if (it < nti) acc[it] = __builtin_amdgcn_mfma_f64_16x16x4f64(ga[ks][it], gb[ks], acc[it], 0, 0, 0);
+        };
+        for (int64_t g = w0; g < w1; g += 16) {
+            load(ga0, gb0, g);
+            mul(ga0, gb0);
         }
-        // the four waves' sums (lane l, element q holds D[l/16 + 4q][l%16])
-        // added in wave order through LDS
-        double *red = &ta[0][0];  // 4 x NT x 4 x 64 doubles fit the 64 x (LIMC + 1) tile
-        static_assert(4 * NT * 4 * 64 <= 64 * (LIMC + 1), "k_gram: reduction scratch exceeds the tile");
+        // waves 1..3 hand their sums over LDS; wave 0 adds them in wave
+        // order (lane l, element q holds D[l/16 + 4q][l%16])
+        if (wave > 0) {
 #pragma unroll
-        for (int it = 0; it < NT; ++it)
-            if (it < nti)
+            for (int it = 0; it < NT; ++it)
+                if (it < nti)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) red[((wave * NT + it) * 4 + q) * 64 + lane] = acc[it][q];
+                    for (int q = 0; q < 4; ++q) red[(wave - 1) * RED + (it * 4 + q) * 64 + lane] = acc[it][q];
+        }
         __syncthreads();
-        for (int o = threadIdx.x; o < nti * 256; o += 256) {
-            const int it = o >> 8, q = (o >> 6) & 3, l = o & 63;
-            double sum = 0.0;
-            for (int w = 0; w < 4; ++w) sum += red[((w * NT + it) * 4 + q) * 64 + l];
-            const int i = it * 16 + (l >> 4) + 4 * q, jj = l & 15;
-            if (i < ka && jj < kb) partial[(int64_t)blockIdx.x * PSTRIDE + i * kb + jj] = sum;
+        if (wave == 0) {
+#pragma unroll
+            for (int it = 0; it < NT; ++it)
+                if (it < nti)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        double sum = acc[it][q];
+                        for (int w = 0; w < 3; ++w) sum += red[w * RED + (it * 4 + q) * 64 + lane];
+                        const int i = it * 16 + (lane >> 4) + 4 * q, jj = lane & 15;
+                        if (i < ka && jj < kb) partial[(int64_t)blockIdx.x * PSTRIDE + i * kb + jj] = sum;
+                    }
         }
         return;
     }
@@ -966,6 +954,7 @@ __global__ __launch_bounds__(256) void k_gram(DvdProb *P, const int *chunk_prob,
     double accv[NQ];  // outputs t, t+256, ...
 #pragma unroll
     for (int q = 0; q < NQ; ++q) accv[q] = 0.0;
+    double(*tb)[MAXNUME + 1] = reinterpret_cast<double(*)[MAXNUME + 1]>(red);  // 64 x (MAXNUME + 1) row tile
     for (int64_t rt = r0; rt < r1; rt += 64) {
         const int rows = (int)min<int64_t>(64, r1 - rt);
         for (int idx = threadIdx.x; idx < rows * kb; idx += blockDim.x) {
@@ -1838,6 +1827,11 @@ DavidsonEngine::~DavidsonEngine() {
     for (hipEvent_t e : ev_)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : vec_ev_) (void)hipEventDestroy(e);
+    for (const ProfEv &pe : prof_events_) {
+        (void)hipEventDestroy(pe.e0);
+        (void)hipEventDestroy(pe.e1);
+    }
+    for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
     if (chunk_ev_) (void)hipEventDestroy(chunk_ev_);
     if (stage_ev_) (void)hipEventDestroy(stage_ev_);
     if (hchunk_pin_) (void)hipHostFree(hchunk_pin_);
@@ -2210,8 +2204,8 @@ void DavidsonEngine::launch_op(int phase, int maxcols, int modes) {
             launch(k_pack_slots, G, 256, stream_, dprob_.p, cp, cr, phase, M, c0, cb, mode);
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (pf.on) {
-                GCI_HIP(hipEventCreate(&e0));
-                GCI_HIP(hipEventCreate(&e1));
+                e0 = pooled_event();
+                e1 = pooled_event();
                 GCI_HIP(hipEventRecord(e0, stream_));
             }
             spmm_items(ditems_.p, nitems_, djobs_.p, M, mode, stream_);
@@ -2256,19 +2250,40 @@ void DavidsonEngine::launch_iteration() {
     launch(k_snapshot, 1, 256, stream_, (const DvdProb *)dprob_.p, cap_, dseq_.p, dsnap_host_);
 }
 
-void DavidsonEngine::drain_profile() {
+// Adds the timed operator launches to the profile: with wait, all of them
+// (after the stream drains); without, the leading ones whose closing event
+// has completed, so polling does not stall the queued iterations.
+void DavidsonEngine::drain_profile(bool wait) {
     if (prof_events_.empty()) return;
-    GCI_HIP(hipStreamSynchronize(stream_));
+    if (wait) GCI_HIP(hipStreamSynchronize(stream_));
     OpProfile &pf = op_profile();
-    for (auto &pe : prof_events_) {
+    size_t k = 0;
+    for (; k < prof_events_.size(); ++k) {
+        const ProfEv &pe = prof_events_[k];
+        if (!wait) {
+            const hipError_t q = hipEventQuery(pe.e1);
+            if (q == hipErrorNotReady) break;
+            GCI_HIP(q);
+        }
         float ms = 0.0f;
         GCI_HIP(hipEventElapsedTime(&ms, pe.e0, pe.e1));
         pf.ms[pe.mode] += ms;
         pf.launches[pe.mode] += 1.0;
-        (void)hipEventDestroy(pe.e0);
-        (void)hipEventDestroy(pe.e1);
+        ev_pool_.push_back(pe.e0);
+        ev_pool_.push_back(pe.e1);
     }
-    prof_events_.clear();
+    prof_events_.erase(prof_events_.begin(), prof_events_.begin() + k);
+}
+
+hipEvent_t DavidsonEngine::pooled_event() {
+    if (ev_pool_.empty()) {
+        hipEvent_t e = nullptr;
+        GCI_HIP(hipEventCreate(&e));
+        return e;
+    }
+    hipEvent_t e = ev_pool_.back();
+    ev_pool_.pop_back();
+    return e;
 }
 
 // Whether some slot's KPASS can reach LIM (OVFLOW) within the next `iters`
@@ -2325,7 +2340,7 @@ void DavidsonEngine::step(int iters) {
 }
 
 std::vector<int> DavidsonEngine::poll_lagged() {
-    drain_profile();
+    drain_profile(false);
     std::vector<int> done;
     if (nev_rec_ == 0) return done;
     // the previous step's event when there is one, else the last
@@ -2352,7 +2367,7 @@ bool DavidsonEngine::all_done() const {
 }
 
 std::vector<int> DavidsonEngine::poll() {
-    drain_profile();
+    drain_profile(true);
     GCI_HIP(hipMemcpy2DAsync(dphase_.p, sizeof(int), &dprob_.p[0].phase, sizeof(DvdProb), sizeof(int), cap_,
                              hipMemcpyDeviceToDevice, stream_));
     int *ph = hphase_pin_;
@@ -2497,6 +2512,7 @@ void DavidsonEngine::copy_vectors(const std::vector<int> &slots, double *evec) {
 }
 
 void DavidsonEngine::release_all() {
+    drain_profile(true);
     launch(k_set_phase, (cap_ + 63) / 64, 64, stream_, dprob_.p, cap_, -1, (int)PH_EMPTY);
     GCI_HIP(hipGetLastError());
     for (int i = 0; i < cap_; ++i) {
