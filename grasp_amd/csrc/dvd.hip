@@ -191,7 +191,7 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
             // pv = b A22 v: G threads per row, G a power of two with G L <= 512
             int G = 64;
             while (G > 1 && G * L > nt) G >>= 1;
-            const int r = t / G, g = t % G;
+            const int r = t >> __builtin_ctz(G), g = t & (G - 1);
             double acc = 0.0;
             if (r < L)
                 for (int c = g; c < L; c += G) acc = fma(A[j + 1 + r][j + 1 + c], A[j + 1 + c][j], acc);
@@ -214,12 +214,16 @@ __device__ void rr_tridiag(DvdProb &p, double (*A)[MAXLIM + 1], double *W, int k
             }
             __syncthreads();
             const double K = red[0];
-            // A22 -= v w^T + w v^T, w = pv - K v
-            for (int idx = t; idx < L * L; idx += nt) {
-                const int r2 = idx / L, c2 = idx % L;
-                const double vr = A[j + 1 + r2][j], vc = A[j + 1 + c2][j];
-                const double wr = fma(-K, vr, pv[r2]), wc = fma(-K, vc, pv[c2]);
-                A[j + 1 + r2][j + 1 + c2] -= fma(vr, wc, wr * vc);
+            // A22 -= v w^T + w v^T, w = pv - K v; rows t/32 + 16q, columns
+            // t%32 + 32u (no integer division in the loop)
+            for (int r2 = t >> 5; r2 < L; r2 += nt >> 5) {
+                const double vr = A[j + 1 + r2][j];
+                const double wr = fma(-K, vr, pv[r2]);
+                for (int c2 = t & 31; c2 < L; c2 += 32) {
+                    const double vc = A[j + 1 + c2][j];
+                    const double wc = fma(-K, vc, pv[c2]);
+                    A[j + 1 + r2][j + 1 + c2] -= fma(vr, wc, wr * vc);
+                }
             }
         }
         __syncthreads();
@@ -683,7 +687,7 @@ __global__ __launch_bounds__(256) void k_rotate_exit(DvdProb *P, const int *chun
             double acc[KO];
 #pragma unroll
             for (int j = 0; j < KO; ++j) acc[j] = 0.0;
-#pragma unroll 4
+#pragma unroll 8
             for (int i = 0; i < kin; ++i) {
                 const double a = M[r + (int64_t)i * p.n];
 #pragma unroll

@@ -187,6 +187,37 @@ def test_batch_solve_result_buffers(gpu):
         rci.batch_solve(blocks, nev, 1e-9, eav, out=np.empty(total - 1))
 
 
+@pytest.mark.parametrize("engines", [1, 2])
+def test_batch_solve_pinned_vectors(gpu, monkeypatch, engines):
+    """A page-locked result buffer (direct copy) over blocks that finish at
+    different iterations, with one and two engine groups: every block's
+    vectors are its final ones (FP64-operator residuals within the
+    tolerance, the pageable-buffer solve's vectors), a second solve into
+    the same buffer overwrites it whole, and the guard elements past the
+    end stay untouched."""
+    sizes = [150, 3200, 400, 2600, 90, 1800]
+    fills = [0.1, 0.02, 0.08, 0.03, 0.1, 0.04]
+    nev = [2, 4, 3, 4, 1, 3]
+    data = _blocks(sizes, 900, fills)
+    blocks = [rci.HamiltonianBlock.from_csr(c, shift=-e, precision=L.GCI_SPLIT32) for c, e in data]
+    eavs = [e for _, e in data]
+    total = sum(n * k for n, k in zip(sizes, nev))
+    monkeypatch.setenv("GCI_BATCH_ENGINES", str(engines))
+    ev0, V0, _ = rci.batch_solve(blocks, nev, 1e-9, eavs, out=np.empty(total))
+    pinned = rci.pinned_empty(total + 3)
+    for rep in range(2):
+        pinned[:] = np.nan
+        pinned[total:] = 7.0
+        ev1, V1, st = rci.batch_solve(blocks, nev, 1e-9, eavs, out=pinned)
+        assert np.all(np.isfinite(pinned[:total])), rep
+        assert np.all(pinned[total:] == 7.0)
+        for b, (csr, eav) in enumerate(data):
+            assert st[b]["ierr"] == 0
+            _assert_residuals(csr, eav, V1[b], ev1[b], 1e-9)
+            overlap = np.abs(np.sum(V0[b] * V1[b], axis=0))
+            assert np.all(np.abs(overlap - 1.0) < 1e-8), (rep, b, overlap)
+
+
 def test_batch_solve_decoupled_block(gpu):
     """A very sparse block whose Ritz vectors are exact eigenvectors: at the
     refinement restart A x lies in the basis span, which the Gram-Schmidt
