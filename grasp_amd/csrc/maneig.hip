@@ -1,5 +1,6 @@
 // maneig.hip -- MANEIG dispatch, GDVD entry point and the batched solve.
 #include <algorithm>
+#include <limits>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -87,20 +88,20 @@ DvdSpec resolve(const HMat &h, const gci_dvd_params *p) {
 // queues the next iteration before reading the state after the previous
 // one, so the device does not wait for the host's decisions.
 // Slots whose bulk phase has ended wait for their FP64 refinement restart
-// until half of the engine's slots are waiting (GCI_REFINE_BATCH sets the
-// count) or nothing else iterates, so that one restart launch sequence and
-// one wide FP64 product serve many: on the 64-block batch, 8.78 ms per
-// solve against 9.3 ms with a restart per poll (waiting for all 64: 8.71 ms,
-// but a slow block would then hold every other block back).
+// until no slot of the engine is still iterating (GCI_REFINE_BATCH=n
+// restarts as soon as n slots wait), so that one restart launch sequence
+// and one wide FP64 product serve every block, and no iteration carries a
+// second operator mode: on the 64-block batch, 8.08 ms per solve against
+// 8.20-8.24 ms when half of the slots start the restart and 8.75 ms with a
+// restart at every poll.
 int refine_batch() {
     const char *v = std::getenv("GCI_REFINE_BATCH");
-    return v && *v ? std::max(1, std::atoi(v)) : 0;  // 0: half of the slots
+    return v && *v ? std::max(1, std::atoi(v)) : std::numeric_limits<int>::max();
 }
 
 void collect_refinements(DavidsonEngine &eng, const std::vector<int> &done, std::vector<int> &pending, int batch) {
     for (int sl : done)
         if (eng.needs_refine(sl)) pending.push_back(sl);
-    if (batch <= 0) batch = std::max(1, eng.active() / 2);
     if (!pending.empty() && ((int)pending.size() >= batch || eng.iterating() == 0)) {
         eng.refine_restart(pending);
         pending.clear();
